@@ -1,0 +1,319 @@
+"""ctypes binding of libgrudon_hip.so (C-ABI in include/grudon_hip.h).
+
+This is the Python face of the HIP library: it raises if the library is not
+built or a call fails; there is no CPU fallback.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libgrudon_hip.so")
+
+PR, CC, SSSP, PR_PULL = 0, 1, 2, 3
+OPT_PUSH_SPARSE, OPT_PULL_SPARSE, OPT_PUSH_DENSE, OPT_PULL_DENSE = 0, 1, 2, 3
+SCHED_PLAIN, SCHED_HEURISTIC, SCHED_LEARNED, SCHED_FIXED = 0, 1, 2, 3
+RMAT_ABC = (0.57, 0.19, 0.19)
+MLP_PARAMS = 5508
+
+# every extern "C" symbol declared in include/grudon_hip.h
+EXPORTS = [
+    "grd_last_error", "grd_create", "grd_destroy", "grd_set_host_threads",
+    "grd_generate_rmat", "grd_generate_er", "grd_graph_from_csr", "grd_graph_rmat", "grd_graph_er",
+    "grd_graph_load", "grd_graph_free", "grd_graph_info", "grd_graph_table", "grd_graph_out_degree",
+    "grd_state_create", "grd_state_free", "grd_init", "grd_broadcast", "grd_traverse", "grd_reduce",
+    "grd_update", "grd_aggregate_one", "grd_run", "grd_download", "grd_features", "grd_snapshot",
+    "grd_restore", "grd_selector_load", "grd_selector_train", "grd_selector_infer", "grd_selector_grad",
+    "grd_batch_create", "grd_batch_free", "grd_batch_tune", "grd_batch_autotune", "grd_batch_pagerank",
+    "grd_batch_cc", "grd_batch_download",
+]
+
+
+class Params(ct.Structure):
+    _fields_ = [("source", ct.c_uint32), ("weighted", ct.c_uint32), ("delta", ct.c_uint32),
+                ("max_iter", ct.c_uint32), ("damping", ct.c_double), ("tolerance", ct.c_double),
+                ("sched", ct.c_uint32), ("fixed_option", ct.c_uint32), ("time_iters", ct.c_uint32),
+                ("reserved", ct.c_uint32)]
+
+
+class IterRec(ct.Structure):
+    _fields_ = [("iteration", ct.c_uint32), ("option", ct.c_uint32), ("frontier", ct.c_uint64),
+                ("frontier_edges", ct.c_uint64), ("dirty", ct.c_uint64), ("next_frontier", ct.c_uint64),
+                ("ms", ct.c_double), ("feat", ct.c_float * 16)]
+
+
+class RunStats(ct.Structure):
+    _fields_ = [("iterations", ct.c_uint64), ("bytes_exchanged", ct.c_uint64), ("edges_examined", ct.c_uint64),
+                ("hit_max_iter", ct.c_uint64), ("ms", ct.c_double), ("residual", ct.c_double),
+                ("option_counts", ct.c_uint64 * 4), ("fallbacks", ct.c_uint64)]
+
+    def as_dict(self):
+        return {"iterations": int(self.iterations), "bytes_exchanged": int(self.bytes_exchanged),
+                "edges_examined": int(self.edges_examined), "hit_max_iter": int(self.hit_max_iter),
+                "ms": float(self.ms), "residual": float(self.residual),
+                "option_counts": [int(x) for x in self.option_counts], "fallbacks": int(self.fallbacks)}
+
+
+_lib = None
+
+
+class GrdError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GrdError("libgrudon_hip.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = ct.CDLL(LIB_PATH)
+        L.grd_last_error.restype = ct.c_char_p
+        vp = ct.c_void_p
+        pp = ct.POINTER(ct.c_void_p)
+        sigs = {
+            "grd_create": [ct.c_int, pp], "grd_destroy": [vp], "grd_set_host_threads": [vp, ct.c_int],
+            "grd_generate_rmat": [vp, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_double, ct.c_uint64,
+                                  ct.c_int, vp, vp, ct.c_uint64, ct.POINTER(ct.c_uint64)],
+            "grd_generate_er": [vp, ct.c_uint32, ct.c_double, ct.c_uint64, vp, vp, ct.c_uint64,
+                                ct.POINTER(ct.c_uint64)],
+            "grd_graph_from_csr": [vp, ct.c_uint32, vp, vp, vp, ct.c_int, ct.c_int, vp, pp],
+            "grd_graph_rmat": [vp, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_double, ct.c_uint64, ct.c_int,
+                               ct.c_uint64, ct.c_int, ct.c_int, pp],
+            "grd_graph_er": [vp, ct.c_uint32, ct.c_double, ct.c_uint64, ct.c_uint64, ct.c_int, ct.c_int, pp],
+            "grd_graph_load": [vp, ct.c_char_p, ct.c_char_p, ct.c_int, ct.c_int, pp],
+            "grd_graph_free": [vp], "grd_graph_info": [vp, vp], "grd_graph_table": [vp, ct.c_int, ct.c_int, vp,
+                                                                                   ct.POINTER(ct.c_uint64)],
+            "grd_graph_out_degree": [vp, vp],
+            "grd_state_create": [vp, ct.c_int, pp], "grd_state_free": [vp], "grd_init": [vp, ct.POINTER(Params)],
+            "grd_broadcast": [vp, ct.c_int], "grd_traverse": [vp, ct.c_int], "grd_reduce": [vp, ct.c_int],
+            "grd_update": [vp, ct.POINTER(ct.c_uint64)], "grd_aggregate_one": [vp, ct.c_uint32, vp],
+            "grd_run": [vp, ct.POINTER(RunStats), vp, ct.c_uint32], "grd_download": [vp, vp],
+            "grd_features": [vp, vp], "grd_snapshot": [vp], "grd_restore": [vp],
+            "grd_selector_load": [vp, vp, ct.c_uint64, ct.c_float],
+            "grd_selector_train": [vp, vp, vp, vp, ct.c_uint64, ct.c_uint32, ct.c_uint32, ct.c_float, ct.c_uint64,
+                                   vp, ct.POINTER(ct.c_float)],
+            "grd_selector_infer": [vp, vp, ct.c_uint64, vp],
+            "grd_selector_grad": [vp, vp, vp, vp, vp, ct.c_uint64, ct.POINTER(ct.c_float), vp],
+            "grd_batch_create": [vp, ct.c_uint32, vp, vp, vp, vp, pp], "grd_batch_free": [vp],
+            "grd_batch_tune": [vp, ct.c_uint32, ct.c_uint32, ct.c_uint32], "grd_batch_autotune": [vp, vp],
+            "grd_batch_pagerank": [vp, ct.c_double, ct.c_double, ct.c_uint32, vp, ct.POINTER(ct.c_double)],
+            "grd_batch_cc": [vp, ct.c_uint32, vp, ct.POINTER(ct.c_double)],
+            "grd_batch_download": [vp, ct.c_uint32, ct.c_int, vp],
+        }
+        for name, args in sigs.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = ct.c_int
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise GrdError("%s failed (status %d): %s" % (what, rc, lib().grd_last_error().decode()))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Context:
+    def __init__(self, device=0):
+        h = ct.c_void_p()
+        _check(lib().grd_create(device, ct.byref(h)), "grd_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().grd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_host_threads(self, n):
+        _check(lib().grd_set_host_threads(self.h, n), "grd_set_host_threads")
+
+    def generate_rmat(self, scale, ef=16, seed=1, symmetrize=False, abc=RMAT_ABC):
+        n = 1 << scale
+        cap = (ef << scale) * (2 if symmetrize else 1)
+        rp = np.zeros(n + 1, dtype=np.uint32)
+        col = np.zeros(max(cap, 1), dtype=np.uint32)
+        m = ct.c_uint64()
+        _check(lib().grd_generate_rmat(self.h, scale, ef, abc[0], abc[1], abc[2], seed, int(symmetrize),
+                                       _ptr(rp), _ptr(col), cap, ct.byref(m)), "grd_generate_rmat")
+        return rp, col[: m.value].copy()
+
+    def generate_er(self, n, mean_degree, seed):
+        cap = int(n * mean_degree / 2.0) * 2
+        rp = np.zeros(n + 1, dtype=np.uint32)
+        col = np.zeros(max(cap, 1), dtype=np.uint32)
+        m = ct.c_uint64()
+        _check(lib().grd_generate_er(self.h, n, mean_degree, seed, _ptr(rp), _ptr(col), cap, ct.byref(m)),
+               "grd_generate_er")
+        return rp, col[: m.value].copy()
+
+    # selector
+    def selector_load(self, w_bf16, min_margin=0.0):
+        w = np.ascontiguousarray(w_bf16, dtype=np.uint16)
+        _check(lib().grd_selector_load(self.h, _ptr(w), w.size, min_margin), "grd_selector_load")
+
+    def selector_train(self, feat, label, cost=None, epochs=20, batch=256, lr=3e-3, seed=1):
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        lb = np.ascontiguousarray(label, dtype=np.uint32)
+        c = None if cost is None else np.ascontiguousarray(cost, dtype=np.float32)
+        out = np.zeros(MLP_PARAMS, dtype=np.uint16)
+        loss = ct.c_float()
+        _check(lib().grd_selector_train(self.h, _ptr(f), _ptr(lb), _ptr(c), lb.size, epochs, batch, lr, seed,
+                                        _ptr(out), ct.byref(loss)), "grd_selector_train")
+        return out, loss.value
+
+    def selector_infer(self, feat):
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        n = f.shape[0]
+        out = np.zeros((n, 4), dtype=np.float32)
+        _check(lib().grd_selector_infer(self.h, _ptr(f), n, _ptr(out)), "grd_selector_infer")
+        return out
+
+    def selector_grad(self, w, feat, label, sample_weight=None):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        lb = np.ascontiguousarray(label, dtype=np.uint32)
+        sw = None if sample_weight is None else np.ascontiguousarray(sample_weight, dtype=np.float32)
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        loss = ct.c_float()
+        _check(lib().grd_selector_grad(self.h, _ptr(w), _ptr(f), _ptr(lb), _ptr(sw), lb.size, ct.byref(loss),
+                                       _ptr(g)), "grd_selector_grad")
+        return loss.value, g
+
+
+class Graph:
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+        info = np.zeros(16, dtype=np.uint64)
+        _check(lib().grd_graph_info(self.h, _ptr(info)), "grd_graph_info")
+        self.n, self.m, self.C, self.M = int(info[0]), int(info[1]), int(info[2]), int(info[3])
+        self.slots, self.proxies, self.cross = int(info[4]), int(info[5]), int(info[6])
+        self.weighted = bool(info[7])
+        self.device_bytes = int(info[8])
+
+    @classmethod
+    def from_csr(cls, ctx, rp, col, w=None, C=1, M=1, mirror=None):
+        rp = np.ascontiguousarray(rp, dtype=np.uint32)
+        col = np.ascontiguousarray(col, dtype=np.uint32)
+        w = None if w is None else np.ascontiguousarray(w, dtype=np.uint32)
+        mirror = None if mirror is None else np.ascontiguousarray(mirror, dtype=np.uint32)
+        h = ct.c_void_p()
+        _check(lib().grd_graph_from_csr(ctx.h, len(rp) - 1, _ptr(rp), _ptr(col), _ptr(w), C, M, _ptr(mirror),
+                                        ct.byref(h)), "grd_graph_from_csr")
+        return cls(ctx, h)
+
+    @classmethod
+    def rmat(cls, ctx, scale, ef=16, seed=1, symmetrize=False, wseed=0, C=1, M=1, abc=RMAT_ABC):
+        h = ct.c_void_p()
+        _check(lib().grd_graph_rmat(ctx.h, scale, ef, abc[0], abc[1], abc[2], seed, int(symmetrize), wseed, C, M,
+                                    ct.byref(h)), "grd_graph_rmat")
+        return cls(ctx, h)
+
+    @classmethod
+    def er(cls, ctx, n, mean_degree, seed, wseed=0, C=1, M=1):
+        h = ct.c_void_p()
+        _check(lib().grd_graph_er(ctx.h, n, mean_degree, seed, wseed, C, M, ct.byref(h)), "grd_graph_er")
+        return cls(ctx, h)
+
+    @classmethod
+    def load(cls, ctx, path, partition_file=None, C=1, M=1):
+        h = ct.c_void_p()
+        _check(lib().grd_graph_load(ctx.h, path.encode(), (partition_file or "").encode(), C, M, ct.byref(h)),
+               "grd_graph_load")
+        return cls(ctx, h)
+
+    def table(self, c, m):
+        cnt = ct.c_uint64()
+        _check(lib().grd_graph_table(self.h, c, m, None, ct.byref(cnt)), "grd_graph_table")
+        out = np.zeros(max(cnt.value, 1), dtype=np.uint32)
+        _check(lib().grd_graph_table(self.h, c, m, _ptr(out), ct.byref(cnt)), "grd_graph_table")
+        return out[: cnt.value]
+
+    def out_degree(self):
+        out = np.zeros(self.n, dtype=np.uint32)
+        _check(lib().grd_graph_out_degree(self.h, _ptr(out)), "grd_graph_out_degree")
+        return out
+
+    def free(self):
+        if self.h:
+            lib().grd_graph_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class State:
+    def __init__(self, graph, algo):
+        self.graph = graph
+        self.algo = algo
+        h = ct.c_void_p()
+        _check(lib().grd_state_create(graph.h, algo, ct.byref(h)), "grd_state_create")
+        self.h = h
+
+    def init(self, source=0, weighted=False, delta=0, max_iter=1000, damping=0.85, tolerance=1e-6,
+             sched=SCHED_PLAIN, fixed_option=0, time_iters=False):
+        p = Params(source, int(weighted), delta, max_iter, damping, tolerance, sched, fixed_option,
+                   int(time_iters), 0)
+        _check(lib().grd_init(self.h, ct.byref(p)), "grd_init")
+
+    def run(self, trace_cap=0):
+        st = RunStats()
+        tr = (IterRec * trace_cap)() if trace_cap else None
+        _check(lib().grd_run(self.h, ct.byref(st), tr, trace_cap), "grd_run")
+        if trace_cap:
+            n = min(int(st.iterations), trace_cap)
+            return st.as_dict(), [tr[i] for i in range(n)]
+        return st.as_dict()
+
+    def step(self, option):
+        _check(lib().grd_broadcast(self.h, option), "grd_broadcast")
+        _check(lib().grd_traverse(self.h, option), "grd_traverse")
+        _check(lib().grd_reduce(self.h, option), "grd_reduce")
+        f = ct.c_uint64()
+        _check(lib().grd_update(self.h, ct.byref(f)), "grd_update")
+        return f.value
+
+    def result(self):
+        dt = np.float32 if self.algo in (PR, PR_PULL) else np.uint32
+        out = np.zeros(self.graph.n, dtype=dt)
+        _check(lib().grd_download(self.h, _ptr(out)), "grd_download")
+        return out
+
+    def features(self):
+        f = np.zeros(16, dtype=np.float32)
+        _check(lib().grd_features(self.h, _ptr(f)), "grd_features")
+        return f
+
+    def snapshot(self):
+        _check(lib().grd_snapshot(self.h), "grd_snapshot")
+
+    def restore(self):
+        _check(lib().grd_restore(self.h), "grd_restore")
+
+    def free(self):
+        if self.h:
+            lib().grd_state_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,546 @@
+// api.cpp -- extern "C" entry points of libgrudon_hip.so (include/grudon_hip.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "grd_internal.hpp"
+
+namespace grd {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+grd_status fail(grd_status st, const std::string &msg)
+{
+  g_last_error = msg;
+  return st;
+}
+
+grd_status read_graph(const std::string &path, uint32_t &n, std::vector<uint32_t> &rp, std::vector<uint32_t> &col,
+                      std::vector<uint32_t> &w, bool &has_w);
+grd_status read_partition_file(const std::string &path, uint32_t n, int M, std::vector<uint32_t> &mirror);
+
+// ------------------------------------------------------------------------
+// schedule policies
+// ------------------------------------------------------------------------
+
+// Direction-optimizing heuristic (plain fallback and judge of the learned
+// choice): pull when the frontier's out-edges exceed a quarter of the arcs
+// (a push edge costs an atomic read-modify-write, a pull edge a gather of a
+// bit and a value), dense sync when more than 1/16 of the vertices or of the
+// proxies are moving.
+int heuristic_option(const grd_state *s)
+{
+  const grd_graph *g = s->g;
+  const bool pull = (double)s->last_frontier_edges * 4.0 > (double)g->m;
+  const bool dense = (double)s->last_frontier > (double)g->n / 16.0 || (double)s->last_dirty > (double)g->P / 16.0;
+  return (pull ? 1 : 0) | (dense ? 2 : 0);
+}
+
+// 16 selector features of the state before an iteration
+void make_features(const grd_state *s, float *f)
+{
+  const grd_graph *g = s->g;
+  const double n = (double)g->n, m = std::max<double>(1.0, (double)g->m), P = (double)g->P;
+  const double fr = (double)s->last_frontier, fe = (double)s->last_frontier_edges;
+  f[0] = (float)(std::log2(1.0 + fr) / 32.0);
+  f[1] = (float)(fr / n);
+  f[2] = (float)(fe / m);
+  f[3] = (float)(std::log2(1.0 + fe) / 40.0);
+  f[4] = (float)((double)s->last_dirty / P);
+  f[5] = (float)((double)s->last_maxdeg / (fe + 1.0));
+  f[6] = (float)(std::min(4.0, (fr > 0 ? fe / fr : 0.0) / std::max(1e-9, g->avg_degree * 4.0)));
+  f[7] = (float)(std::log2(n) / 32.0);
+  f[8] = (float)(std::min(4.0, g->avg_degree / 64.0));
+  f[9] = (float)(g->M / 16.0);
+  f[10] = (float)(g->C / 16.0);
+  f[11] = (float)(std::min(4.0, (double)g->X / n / 4.0));
+  double ratio = (s->prev_frontier > 0) ? fr / (double)s->prev_frontier : 1.0;
+  f[12] = (float)(std::max(-1.0, std::min(1.0, std::log2(std::max(ratio, 1e-9)) / 16.0)) * 0.5 + 0.5);
+  f[13] = s->algo == GRD_PR ? 1.0f : 0.0f;
+  f[14] = s->algo == GRD_CC ? 1.0f : 0.0f;
+  f[15] = s->algo == GRD_SSSP ? 1.0f : 0.0f;
+}
+
+static uint64_t bitvector_bytes(const grd_graph *g)
+{
+  uint64_t b = 0;
+  for (int c = 0; c < g->C; c++)
+    for (int m = 0; m < g->M; m++) {
+      uint64_t T = (uint64_t)g->ccnt[(size_t)c * g->M + m] + g->xcnt[(size_t)m * g->C + c];
+      b += 8 * ((T + 63) / 64);
+    }
+  return b;
+}
+
+// one BSP superstep with the given option; reads the counters back
+static grd_status superstep(grd_state *s, int option)
+{
+  GRD_TRY(op_zero_counters(s));
+  GRD_TRY(op_broadcast(s, option));
+  GRD_TRY(op_traverse(s, option));
+  GRD_TRY(op_reduce(s, option));
+  GRD_TRY(op_update(s));
+  GRD_TRY(op_read_counters(s));
+  const Counters &c = *s->hcnt;
+  const grd_graph *g = s->g;
+  // reduce bytes (GraphAlgorithm.cpp:409-423): bitvector + dirty values
+  // (sparse) or every translation-table entry (dense)
+  uint64_t vals = (option & 2) ? (uint64_t)g->n + g->X : c.dirty;
+  s->bytes += bitvector_bytes(g) + vals * s->vbytes;
+  s->edges += c.edges;
+  s->prev_frontier = s->last_frontier;
+  s->last_dirty = c.dirty;
+  s->last_frontier = c.frontier;
+  s->last_frontier_edges = c.frontier_edges;
+  s->last_maxdeg = c.frontier_maxdeg;
+  s->last_option = option;
+  s->iteration++;
+  if (c.frontier == 0 && s->algo == GRD_SSSP && s->prm.delta) {
+    bool any = false;
+    GRD_TRY(op_select_bucket(s, &any));
+    if (any) {
+      s->last_frontier = s->hcnt->frontier;
+      s->last_frontier_edges = s->hcnt->frontier_edges;
+      s->last_maxdeg = s->hcnt->frontier_maxdeg;
+    }
+  }
+  return GRD_OK;
+}
+
+}  // namespace grd
+
+using namespace grd;
+
+extern "C" {
+
+const char *grd_last_error(void) { return grd::g_last_error.c_str(); }
+
+grd_status grd_create(int device, grd_ctx **out)
+{
+  if (!out) return fail(GRD_ERR_ARG, "grd_create: out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  GRD_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(GRD_ERR_ARG, "grd_create: no such device");
+  GRD_HIP(hipSetDevice(device));
+  grd_ctx *c = new (std::nothrow) grd_ctx();
+  if (!c) return fail(GRD_ERR_OOM, "grd_create: host allocation failed");
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(GRD_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  for (auto &ev : c->ev) (void)hipEventCreate(&ev);
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    c->num_cus = cus;
+  *out = c;
+  return GRD_OK;
+}
+
+grd_status grd_destroy(grd_ctx *c)
+{
+  if (!c) return GRD_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &ev : c->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->sel_w) (void)hipFree(c->sel_w);
+  if (c->sel_out) (void)hipFree(c->sel_out);
+  if (c->sel_out_host) (void)hipHostFree(c->sel_out_host);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return GRD_OK;
+}
+
+grd_status grd_set_host_threads(grd_ctx *c, int nthreads)
+{
+  if (!c || nthreads < 0) return fail(GRD_ERR_ARG, "grd_set_host_threads: bad argument");
+  c->host_threads = nthreads;
+  return GRD_OK;
+}
+
+grd_status grd_generate_rmat(grd_ctx *c, int scale, int ef, double a, double b, double cc, uint64_t seed, int sym,
+                             uint32_t *row_ptr, uint32_t *col, uint64_t cap, uint64_t *m_out)
+{
+  if (!c || !row_ptr || !m_out) return fail(GRD_ERR_ARG, "grd_generate_rmat: NULL argument");
+  try {
+    std::vector<uint32_t> rp, cl;
+    GRD_TRY(generate_rmat(c, scale, ef, a, b, cc, seed, sym, rp, cl));
+    *m_out = cl.size();
+    if (cl.size() > cap || (!col && !cl.empty())) return fail(GRD_ERR_ARG, "grd_generate_rmat: col capacity too small");
+    memcpy(row_ptr, rp.data(), rp.size() * 4);
+    if (!cl.empty()) memcpy(col, cl.data(), cl.size() * 4);
+    return GRD_OK;
+  } catch (const std::bad_alloc &) {
+    return fail(GRD_ERR_OOM, "grd_generate_rmat: host allocation failed");
+  }
+}
+
+grd_status grd_generate_er(grd_ctx *c, uint32_t n, double deg, uint64_t seed, uint32_t *row_ptr, uint32_t *col,
+                           uint64_t cap, uint64_t *m_out)
+{
+  if (!c || !row_ptr || !m_out) return fail(GRD_ERR_ARG, "grd_generate_er: NULL argument");
+  try {
+    std::vector<uint32_t> rp, cl;
+    GRD_TRY(generate_er(c, n, deg, seed, rp, cl));
+    *m_out = cl.size();
+    if (cl.size() > cap || (!col && !cl.empty())) return fail(GRD_ERR_ARG, "grd_generate_er: col capacity too small");
+    memcpy(row_ptr, rp.data(), rp.size() * 4);
+    if (!cl.empty()) memcpy(col, cl.data(), cl.size() * 4);
+    return GRD_OK;
+  } catch (const std::bad_alloc &) {
+    return fail(GRD_ERR_OOM, "grd_generate_er: host allocation failed");
+  }
+}
+
+grd_status grd_graph_from_csr(grd_ctx *c, uint32_t n, const uint32_t *rp, const uint32_t *col, const uint32_t *w,
+                              int C, int M, const uint32_t *mirror, grd_graph **out)
+{
+  if (!c || !rp || !out || (!col && n && rp[n])) return fail(GRD_ERR_ARG, "grd_graph_from_csr: NULL argument");
+  GRD_HIP(hipSetDevice(c->device));
+  return build_graph(c, n, rp, col, w, C, M, mirror, out);
+}
+
+grd_status grd_graph_rmat(grd_ctx *c, int scale, int ef, double a, double b, double cc, uint64_t seed, int sym,
+                          uint64_t wseed, int C, int M, grd_graph **out)
+{
+  if (!c || !out) return fail(GRD_ERR_ARG, "grd_graph_rmat: NULL argument");
+  try {
+    std::vector<uint32_t> rp, cl, w;
+    GRD_TRY(generate_rmat(c, scale, ef, a, b, cc, seed, sym, rp, cl));
+    uint32_t n = 1u << scale;
+    if (wseed) {
+      w.resize(cl.size());
+      host_weights(n, rp.data(), cl.data(), wseed, w.data(), host_threads(c));
+    }
+    return build_graph(c, n, rp.data(), cl.data(), wseed ? w.data() : nullptr, C, M, nullptr, out);
+  } catch (const std::bad_alloc &) {
+    return fail(GRD_ERR_OOM, "grd_graph_rmat: host allocation failed");
+  }
+}
+
+grd_status grd_graph_er(grd_ctx *c, uint32_t n, double deg, uint64_t seed, uint64_t wseed, int C, int M, grd_graph **out)
+{
+  if (!c || !out) return fail(GRD_ERR_ARG, "grd_graph_er: NULL argument");
+  try {
+    std::vector<uint32_t> rp, cl, w;
+    GRD_TRY(generate_er(c, n, deg, seed, rp, cl));
+    if (wseed) {
+      w.resize(cl.size());
+      host_weights(n, rp.data(), cl.data(), wseed, w.data(), host_threads(c));
+    }
+    return build_graph(c, n, rp.data(), cl.data(), wseed ? w.data() : nullptr, C, M, nullptr, out);
+  } catch (const std::bad_alloc &) {
+    return fail(GRD_ERR_OOM, "grd_graph_er: host allocation failed");
+  }
+}
+
+grd_status grd_graph_load(grd_ctx *c, const char *path, const char *pfile, int C, int M, grd_graph **out)
+{
+  if (!c || !path || !out) return fail(GRD_ERR_ARG, "grd_graph_load: NULL argument");
+  try {
+    uint32_t n = 0;
+    std::vector<uint32_t> rp, cl, w, mir;
+    bool has_w = false;
+    GRD_TRY(read_graph(path, n, rp, cl, w, has_w));
+    // Partitioner.hpp:73-85: with one memory partition the file is ignored
+    if (pfile && pfile[0] && M > 1) GRD_TRY(read_partition_file(pfile, n, M, mir));
+    return build_graph(c, n, rp.data(), cl.data(), has_w ? w.data() : nullptr, C, M, mir.empty() ? nullptr : mir.data(),
+                       out);
+  } catch (const std::bad_alloc &) {
+    return fail(GRD_ERR_OOM, "grd_graph_load: host allocation failed");
+  }
+}
+
+grd_status grd_graph_free(grd_graph *g)
+{
+  if (!g) return GRD_OK;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  delete g;
+  return GRD_OK;
+}
+
+grd_status grd_graph_info(const grd_graph *g, uint64_t *info)
+{
+  if (!g || !info) return fail(GRD_ERR_ARG, "grd_graph_info: NULL argument");
+  memset(info, 0, 16 * sizeof(uint64_t));
+  info[0] = g->n;
+  info[1] = g->m;
+  info[2] = (uint64_t)g->C;
+  info[3] = (uint64_t)g->M;
+  info[4] = g->S;
+  info[5] = g->P;
+  info[6] = g->X;
+  info[7] = g->weighted ? 1 : 0;
+  info[8] = g->mem.bytes;
+  info[9] = g->n_heavy_in;
+  info[10] = g->heavy_in_threshold;
+  return GRD_OK;
+}
+
+grd_status grd_graph_table(const grd_graph *g, int c, int m, uint32_t *out, uint64_t *count)
+{
+  if (!g || !count || c < 0 || m < 0 || c >= g->C || m >= g->M) return fail(GRD_ERR_ARG, "grd_graph_table: bad pair");
+  const size_t k = (size_t)m * g->C + c;
+  const uint32_t own = g->ocnt[k], cross = g->xcnt[k];
+  *count = (uint64_t)own + cross;
+  if (out) {
+    for (uint32_t i = 0; i < own; i++) out[i] = g->h_proxy_gid[g->obase[k] + i];
+    for (uint32_t i = 0; i < cross; i++) out[own + i] = g->h_proxy_gid[g->xbase[k] + i];
+  }
+  return GRD_OK;
+}
+
+grd_status grd_graph_out_degree(const grd_graph *g, uint32_t *out)
+{
+  if (!g || !out) return fail(GRD_ERR_ARG, "grd_graph_out_degree: NULL argument");
+  memcpy(out, g->h_deg.data(), (size_t)g->n * 4);
+  return GRD_OK;
+}
+
+grd_status grd_state_create(grd_graph *g, int algo, grd_state **out)
+{
+  if (!g || !out) return fail(GRD_ERR_ARG, "grd_state_create: NULL argument");
+  if (algo < GRD_PR || algo > GRD_PR_PULL) return fail(GRD_ERR_ARG, "grd_state_create: unknown algorithm");
+  GRD_HIP(hipSetDevice(g->ctx->device));
+  grd_state *s = new (std::nothrow) grd_state();
+  if (!s) return fail(GRD_ERR_OOM, "grd_state_create: host allocation failed");
+  s->g = g;
+  s->algo = algo;
+  grd_status st = state_alloc(s);
+  if (st != GRD_OK) {
+    if (s->hcnt) (void)hipHostFree(s->hcnt);
+    delete s;
+    return st;
+  }
+  *out = s;
+  return GRD_OK;
+}
+
+grd_status grd_state_free(grd_state *s)
+{
+  if (!s) return GRD_OK;
+  (void)hipStreamSynchronize(s->g->ctx->stream);
+  if (s->hcnt) (void)hipHostFree(s->hcnt);
+  delete s;
+  return GRD_OK;
+}
+
+grd_status grd_init(grd_state *s, const grd_params *p)
+{
+  if (!s || !p) return fail(GRD_ERR_ARG, "grd_init: NULL argument");
+  if (s->algo == GRD_SSSP && p->source >= s->g->n) return fail(GRD_ERR_ARG, "grd_init: SSSP source out of range");
+  if (s->algo == GRD_SSSP && p->weighted && !s->g->weighted)
+    return fail(GRD_ERR_ARG, "grd_init: weighted SSSP on a graph without weights");
+  if (p->sched > GRD_SCHED_FIXED || p->fixed_option > 3) return fail(GRD_ERR_ARG, "grd_init: bad schedule");
+  s->prm = *p;
+  if (s->prm.max_iter == 0) s->prm.max_iter = 1000;
+  if (s->prm.damping == 0.0) s->prm.damping = 0.85;
+  if (s->prm.tolerance == 0.0) s->prm.tolerance = 1e-6;
+  GRD_HIP(hipSetDevice(s->g->ctx->device));
+  if (s->algo == GRD_PR_PULL) GRD_TRY(prp_init(s));
+  else GRD_TRY(state_init(s));
+  s->initialized = true;
+  return GRD_OK;
+}
+
+grd_status grd_broadcast(grd_state *s, int option)
+{
+  if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_broadcast: state not initialized");
+  if (s->algo == GRD_PR_PULL) return fail(GRD_ERR_STATE, "grd_broadcast: PR_PULL runs whole iterations");
+  if (option < 0 || option > 3) return fail(GRD_ERR_ARG, "grd_broadcast: bad option");
+  GRD_TRY(op_zero_counters(s));
+  return op_broadcast(s, option);
+}
+
+grd_status grd_traverse(grd_state *s, int option)
+{
+  if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_traverse: state not initialized");
+  if (option < 0 || option > 3) return fail(GRD_ERR_ARG, "grd_traverse: bad option");
+  return op_traverse(s, option);
+}
+
+grd_status grd_reduce(grd_state *s, int option)
+{
+  if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_reduce: state not initialized");
+  if (option < 0 || option > 3) return fail(GRD_ERR_ARG, "grd_reduce: bad option");
+  s->last_option = option;
+  return op_reduce(s, option);
+}
+
+grd_status grd_update(grd_state *s, uint64_t *frontier_out)
+{
+  if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_update: state not initialized");
+  if (s->algo == GRD_PR_PULL) return fail(GRD_ERR_STATE, "grd_update: PR_PULL runs whole iterations");
+  GRD_TRY(op_update(s));
+  GRD_TRY(op_read_counters(s));
+  const Counters &c = *s->hcnt;
+  uint64_t vals = (s->last_option & 2) ? (uint64_t)s->g->n + s->g->X : c.dirty;
+  s->bytes += bitvector_bytes(s->g) + vals * s->vbytes;
+  s->edges += c.edges;
+  s->prev_frontier = s->last_frontier;
+  s->last_dirty = c.dirty;
+  s->last_frontier = c.frontier;
+  s->last_frontier_edges = c.frontier_edges;
+  s->last_maxdeg = c.frontier_maxdeg;
+  s->iteration++;
+  if (c.frontier == 0 && s->algo == GRD_SSSP && s->prm.delta) {
+    bool any = false;
+    GRD_TRY(op_select_bucket(s, &any));
+    if (any) {
+      s->last_frontier = s->hcnt->frontier;
+      s->last_frontier_edges = s->hcnt->frontier_edges;
+    }
+  }
+  if (frontier_out) *frontier_out = s->last_frontier;
+  return GRD_OK;
+}
+
+grd_status grd_aggregate_one(grd_state *s, uint32_t gid, const void *value)
+{
+  if (!s || !value || !s->initialized) return fail(GRD_ERR_STATE, "grd_aggregate_one: bad state");
+  if (gid >= s->g->n) return fail(GRD_ERR_ARG, "grd_aggregate_one: gid out of range");
+  if (s->algo == GRD_PR_PULL) return fail(GRD_ERR_STATE, "grd_aggregate_one: not defined for PR_PULL");
+  return op_aggregate_one(s, s->g->h_gid_slot[gid], value);
+}
+
+grd_status grd_run(grd_state *s, grd_run_stats *stats, grd_iter_rec *trace, uint32_t trace_cap)
+{
+  if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_run: state not initialized");
+  grd_ctx *ctx = s->g->ctx;
+  GRD_HIP(hipSetDevice(ctx->device));
+  grd_run_stats st{};
+  const bool timing = s->prm.time_iters != 0;
+  GRD_HIP(hipStreamSynchronize(ctx->stream));
+  GRD_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  bool done = false;
+  while (!done) {
+    if (s->iteration >= s->prm.max_iter) {
+      st.hit_max_iter = 1;
+      break;
+    }
+    grd_iter_rec rec{};
+    rec.iteration = s->iteration;
+    if (s->algo == GRD_PR_PULL) {
+      if (timing) GRD_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+      GRD_TRY(prp_iteration(s));
+      if (timing) GRD_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
+      GRD_TRY(op_read_counters(s));
+      s->iteration++;
+      st.residual = s->hcnt->residual;
+      rec.option = GRD_OPT_PULL_DENSE;
+      rec.frontier = s->g->n;
+      rec.frontier_edges = s->g->m;
+      rec.dirty = s->g->n + s->g->X;
+      rec.next_frontier = s->g->n;
+      st.option_counts[GRD_OPT_PULL_DENSE]++;
+      done = st.residual < s->prm.tolerance;
+    } else {
+      float feat[16];
+      make_features(s, feat);
+      int option = GRD_OPT_PUSH_SPARSE;
+      switch (s->prm.sched) {
+      case GRD_SCHED_PLAIN: option = GRD_OPT_PUSH_SPARSE; break;
+      case GRD_SCHED_FIXED: option = (int)s->prm.fixed_option; break;
+      case GRD_SCHED_HEURISTIC: option = heuristic_option(s); break;
+      case GRD_SCHED_LEARNED: {
+        int h = heuristic_option(s);
+        if (!ctx->sel_loaded) {
+          option = h;
+          st.fallbacks++;
+        } else {
+          float margin = 0.0f;
+          int o = h;
+          GRD_TRY(mlp_select(ctx, feat, &o, &margin));
+          if (o != h && margin < ctx->sel_margin) {
+            option = h;  // not confident enough to overrule the heuristic
+            st.fallbacks++;
+          } else {
+            option = o;
+          }
+        }
+        break;
+      }
+      }
+      rec.option = (uint32_t)option;
+      rec.frontier = s->last_frontier;
+      rec.frontier_edges = s->last_frontier_edges;
+      memcpy(rec.feat, feat, sizeof(feat));
+      if (timing) GRD_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+      GRD_TRY(superstep(s, option));
+      if (timing) GRD_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
+      rec.dirty = s->last_dirty;
+      rec.next_frontier = s->last_frontier;
+      st.option_counts[option]++;
+      done = (s->last_frontier == 0);
+    }
+    if (timing) {
+      float ms = 0.0f;
+      GRD_HIP(hipEventSynchronize(ctx->ev[3]));
+      GRD_HIP(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+      rec.ms = ms;
+    }
+    if (trace && rec.iteration < trace_cap) trace[rec.iteration] = rec;
+  }
+  GRD_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+  GRD_HIP(hipEventSynchronize(ctx->ev[1]));
+  float ms = 0.0f;
+  GRD_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  st.ms = ms;
+  st.iterations = s->iteration;
+  st.bytes_exchanged = s->bytes;
+  st.edges_examined = s->edges;
+  if (stats) *stats = st;
+  return GRD_OK;
+}
+
+grd_status grd_download(grd_state *s, void *out)
+{
+  if (!s || !out || !s->initialized) return fail(GRD_ERR_STATE, "grd_download: bad state");
+  GRD_HIP(hipSetDevice(s->g->ctx->device));
+  return op_download(s, out);
+}
+
+grd_status grd_features(grd_state *s, float *out16)
+{
+  if (!s || !out16) return fail(GRD_ERR_ARG, "grd_features: NULL argument");
+  make_features(s, out16);
+  return GRD_OK;
+}
+
+grd_status grd_snapshot(grd_state *s)
+{
+  if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_snapshot: bad state");
+  hipStream_t st = s->g->ctx->stream;
+  for (size_t i = 0; i < s->snap.size(); i++) {
+    if (!s->snap[i].second) GRD_TRY(s->snap_mem.alloc(&s->snap[i].second, s->snap_bytes[i]));
+    GRD_HIP(hipMemcpyAsync(s->snap[i].second, s->snap[i].first, s->snap_bytes[i], hipMemcpyDeviceToDevice, st));
+  }
+  GRD_HIP(hipStreamSynchronize(st));
+  s->snap_iteration = s->iteration;
+  s->snap_hi = s->hi;
+  return GRD_OK;
+}
+
+grd_status grd_restore(grd_state *s)
+{
+  if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_restore: bad state");
+  hipStream_t st = s->g->ctx->stream;
+  for (size_t i = 0; i < s->snap.size(); i++) {
+    if (!s->snap[i].second) return fail(GRD_ERR_STATE, "grd_restore: no snapshot taken");
+    GRD_HIP(hipMemcpyAsync(s->snap[i].first, s->snap[i].second, s->snap_bytes[i], hipMemcpyDeviceToDevice, st));
+  }
+  GRD_HIP(hipStreamSynchronize(st));
+  s->iteration = s->snap_iteration;
+  s->hi = s->snap_hi;
+  return GRD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,990 @@
+// bsp.hip -- the BSP superstep of the per-partition operator path on gfx950.
+//
+// One launch covers every partition: compute slots and memory proxies of all
+// partitions are laid out back to back (grd_internal.hpp), so each kernel
+// below is a single grid over the concatenation.  Wave64 throughout; the
+// bitvector protocol works on 64-bit words, one wave per word (lane = bit).
+//
+// Reference loops replaced (src/GraphAlgorithm.cpp):
+//   k_bcast_send / k_bcast_recv   compute pack + memory unpack       :199-343
+//   k_push / k_push_heavy / k_pull*  gen_updates (pr/cc/sssp.cpp)     :348-356
+//   k_reduce_send / k_reduce_recv memory pack + compute aggregate     :377-581
+//   k_update                      update_frontier + apply_updates +
+//                                 termination popcount               :643-662
+//   k_popc_blocks/k_scan_blocks/k_word_rank  DynamicBitSet::getOffsets ranks
+#include <algorithm>
+#include <cstring>
+
+#include "grd_device.hpp"
+#include "grd_internal.hpp"
+
+namespace grd {
+
+// ------------------------------------------------------------------------
+// operator policies
+// ------------------------------------------------------------------------
+
+// CC: label propagation, atomicMin (cc.cpp:65-78, DistributedGraph.hpp:103-112)
+struct OpCC {
+  using V = uint32_t;
+  static constexpr bool kMin = true;
+  __device__ static V edge(V v, uint32_t) { return v; }
+};
+// SSSP hop count: dist + 1 (sssp.cpp:69)
+struct OpHop {
+  using V = uint32_t;
+  static constexpr bool kMin = true;
+  __device__ static V edge(V v, uint32_t) { return v + 1u; }
+};
+// SSSP weighted (extension): saturating dist + w
+struct OpWeighted {
+  using V = uint32_t;
+  static constexpr bool kMin = true;
+  __device__ static V edge(V v, uint32_t w)
+  {
+    uint32_t t = v + w;
+    return t < v ? 0xFFFFFFFFu : t;
+  }
+};
+// Reference PageRank: push the vertex property, atomicAdd (pr.cpp:95-120)
+struct OpPR {
+  using V = float;
+  static constexpr bool kMin = false;
+  __device__ static V edge(V v, uint32_t) { return v; }
+};
+
+__device__ __forceinline__ void set_bit(uint64_t *bits, uint32_t i)
+{
+  uint64_t b = 1ull << (i & 63);
+  uint64_t *wd = bits + (i >> 6);
+  if (!(__atomic_load_n(wd, __ATOMIC_RELAXED) & b)) atomicOr((unsigned long long *)wd, (unsigned long long)b);
+}
+
+// minUpdate / addUpdate, marking the dirty bit (DistributedGraph.hpp:103-130).
+// minUpdate marks only on improvement.  addUpdate always marks: on the
+// memory side a proxy's update is zero until its first add of the iteration
+// (the previous reduce cleared it), so the first add (old == 0) marks; on the
+// compute side residuals below the threshold persist, so every add marks.
+template <class Op, bool kAlwaysMark = false>
+__device__ __forceinline__ void relax(typename Op::V *upd, uint64_t *dirty, uint32_t dst, typename Op::V cand)
+{
+  if constexpr (Op::kMin) {
+    uint32_t cur = __atomic_load_n(upd + dst, __ATOMIC_RELAXED);
+    if (cand < cur) {
+      uint32_t old = atomicMin(upd + dst, cand);
+      if (cand < old) set_bit(dirty, dst);
+    }
+  } else {
+    float old = atomicAdd(upd + dst, cand);
+    if (kAlwaysMark || old == 0.0f) set_bit(dirty, dst);
+  }
+}
+
+template <class Op>
+__device__ __forceinline__ typename Op::V identity()
+{
+  if constexpr (Op::kMin) return 0xFFFFFFFFu;
+  else return 0.0f;
+}
+
+// ------------------------------------------------------------------------
+// word popcount ranks (exclusive prefix of set bits per 64-bit word)
+// ------------------------------------------------------------------------
+
+constexpr int kWordsPerScanBlock = 1024;  // 256 threads x 4 words
+
+__global__ void __launch_bounds__(kBlock) k_popc_blocks(const uint64_t *__restrict__ w, uint32_t nw, uint32_t *blk)
+{
+  uint32_t base = blockIdx.x * kWordsPerScanBlock + threadIdx.x * 4;
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    if (base + i < nw) c += __popcll(w[base + i]);
+  c = block_sum(c);
+  if (threadIdx.x == 0) blk[blockIdx.x] = c;
+}
+
+// single block of 1024 threads: exclusive scan of nb partials, total -> *total
+__global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t *blk, uint32_t nb, uint32_t *total)
+{
+  __shared__ uint32_t s_w[16];
+  uint32_t per = (nb + 1023) / 1024;
+  uint32_t b0 = threadIdx.x * per;
+  uint32_t local = 0;
+  for (uint32_t i = 0; i < per; i++)
+    if (b0 + i < nb) local += blk[b0 + i];
+  uint32_t inc = wave_incl_scan(local);
+  if (lane_id() == kWave - 1) s_w[threadIdx.x / kWave] = inc;
+  __syncthreads();
+  uint32_t wb = 0, tot = 0;
+  for (int i = 0; i < 16; i++) {
+    if (i < (int)(threadIdx.x / kWave)) wb += s_w[i];
+    tot += s_w[i];
+  }
+  uint32_t run = wb + inc - local;
+  for (uint32_t i = 0; i < per; i++)
+    if (b0 + i < nb) {
+      uint32_t x = blk[b0 + i];
+      blk[b0 + i] = run;
+      run += x;
+    }
+  if (threadIdx.x == 0) *total = tot;
+}
+
+__global__ void __launch_bounds__(kBlock) k_word_rank(const uint64_t *__restrict__ w, uint32_t nw,
+                                                      const uint32_t *__restrict__ blk, uint32_t *__restrict__ wrank)
+{
+  uint32_t base = blockIdx.x * kWordsPerScanBlock + threadIdx.x * 4;
+  uint32_t c[4], s = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    c[i] = (base + i < nw) ? (uint32_t)__popcll(w[base + i]) : 0u;
+    s += c[i];
+  }
+  uint32_t tot;
+  uint32_t ex = block_excl_scan(s, &tot) + blk[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    if (base + i < nw) wrank[base + i] = ex;
+    ex += c[i];
+  }
+}
+
+static grd_status word_ranks(hipStream_t st, const uint64_t *w, uint32_t nw, uint32_t *blk, uint32_t *wrank,
+                             uint32_t *total)
+{
+  uint32_t nb = (nw + kWordsPerScanBlock - 1) / kWordsPerScanBlock;
+  if (nb == 0) nb = 1;
+  hipLaunchKernelGGL(k_popc_blocks, dim3(nb), dim3(kBlock), 0, st, w, nw, blk);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, blk, nb, total);
+  hipLaunchKernelGGL(k_word_rank, dim3(nb), dim3(kBlock), 0, st, w, nw, blk, wrank);
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+
+static inline uint32_t wave_grid(const grd_ctx *ctx, uint64_t nwords)
+{
+  uint64_t want = (nwords + kWavesPerBlock - 1) / kWavesPerBlock;
+  uint64_t cap = (uint64_t)ctx->num_cus * 16;
+  return (uint32_t)std::max<uint64_t>(1, std::min(want, cap));
+}
+
+// ------------------------------------------------------------------------
+// init
+// ------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
+{
+  for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) p[i] = v;
+}
+__global__ void __launch_bounds__(kBlock) k_fill_u64(uint64_t *p, uint64_t n, uint64_t v)
+{
+  for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) p[i] = v;
+}
+
+// compute-side init: pr.cpp:25-50, cc.cpp:24-36, sssp.cpp:22-46
+__global__ void __launch_bounds__(kBlock) k_init_compute(int algo, uint32_t nw, const uint32_t *__restrict__ slot_gid,
+                                                         const uint32_t *__restrict__ deg, void *prop_, void *upd_,
+                                                         float *pr, uint64_t *F, uint32_t source_slot)
+{
+  const int wpb = kWavesPerBlock;
+  for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
+    uint32_t s = w * 64 + lane_id();
+    uint32_t gid = slot_gid[s];
+    bool valid = gid != 0xFFFFFFFFu;
+    bool f = false;
+    if (algo == GRD_PR) {
+      float *prop = (float *)prop_, *upd = (float *)upd_;
+      uint32_t d = deg[s];
+      if (valid && d != 0) {
+        prop[s] = (float)(0.85 * (1.0 - 0.85) / (double)d);
+        pr[s] = (float)(1.0 - 0.85);
+        f = true;
+      } else {
+        prop[s] = 0.0f;
+        pr[s] = 0.0f;
+      }
+      upd[s] = 0.0f;
+    } else {
+      uint32_t *prop = (uint32_t *)prop_, *upd = (uint32_t *)upd_;
+      if (algo == GRD_CC) {
+        prop[s] = valid ? gid : 0xFFFFFFFFu;
+        upd[s] = 0xFFFFFFFFu;
+        f = valid;
+      } else {
+        bool src = (s == source_slot);
+        prop[s] = src ? 0u : 0xFFFFFFFFu;
+        upd[s] = src ? 0u : 0xFFFFFFFFu;
+        f = src;
+      }
+    }
+    uint64_t fm = __ballot(f);
+    if (lane_id() == 0) F[w] = fm;
+  }
+}
+
+// ------------------------------------------------------------------------
+// broadcast: compute -> memory (bitvector = frontier words of the pair)
+// ------------------------------------------------------------------------
+
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_bcast_send(uint32_t nw, const uint64_t *__restrict__ F,
+                                                       const V *__restrict__ prop, const uint32_t *__restrict__ wrank,
+                                                       int dense, uint64_t *__restrict__ bv, V *__restrict__ val)
+{
+  const int wpb = kWavesPerBlock;
+  for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
+    uint64_t word = F[w];
+    uint32_t s = w * 64 + lane_id();
+    if (lane_id() == 0) bv[w] = word;
+    if (dense) {
+      val[s] = prop[s];
+    } else if ((word >> lane_id()) & 1ull) {
+      val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())] = prop[s];
+    }
+  }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_bcast_recv(uint32_t nw, const uint64_t *__restrict__ bv,
+                                                       const V *__restrict__ val, const uint32_t *__restrict__ wrank,
+                                                       const uint32_t *__restrict__ word_dst, int dense, int want_list,
+                                                       V *__restrict__ mprop, uint64_t *__restrict__ Dp,
+                                                       uint32_t *__restrict__ AL)
+{
+  const int wpb = kWavesPerBlock;
+  for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
+    uint32_t pw = word_dst[w];
+    if (pw == 0xFFFFFFFFu) continue;
+    uint64_t word = bv[w];
+    if (lane_id() == 0) Dp[pw] = word;
+    if ((word >> lane_id()) & 1ull) {
+      uint32_t r = (dense && !want_list) ? 0u : wrank[w] + (uint32_t)__popcll(word & lanemask_lt());
+      V x = dense ? val[w * 64 + lane_id()] : val[r];
+      uint32_t p = pw * 64 + lane_id();
+      mprop[p] = x;
+      if (want_list) AL[r] = p;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// traversal: push over the active list (degree-binned, LDS-staged tiles)
+// ------------------------------------------------------------------------
+
+// A block stages 256 active vertices in LDS.  Vertices with out-degree below
+// H are processed together: the block scans their degrees and every thread
+// walks the concatenated edge range (coalesced col reads), locating its
+// source by binary search in the LDS prefix array.  Vertices with degree
+// >= H are appended to the heavy list and handled by k_push_heavy, where the
+// whole grid shares each heavy vertex.
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_push(const uint32_t *__restrict__ AL, const uint32_t *__restrict__ nact,
+                                                 const uint32_t *__restrict__ rp, const uint32_t *__restrict__ col,
+                                                 const uint32_t *__restrict__ wt, const typename Op::V *__restrict__ mprop,
+                                                 typename Op::V *upd, uint64_t *dirty, uint32_t *heavy, Counters *cnt,
+                                                 uint32_t H)
+{
+  using V = typename Op::V;
+  __shared__ uint32_t s_off[kBlock + 1];
+  __shared__ uint32_t s_beg[kBlock];
+  __shared__ V s_val[kBlock];
+  const uint32_t k = *nact;
+  uint64_t my_edges = 0;
+  for (uint32_t tile = blockIdx.x * kBlock; tile < k; tile += gridDim.x * kBlock) {
+    uint32_t i = tile + threadIdx.x;
+    uint32_t deg = 0, beg = 0;
+    V v = identity<Op>();
+    if (i < k) {
+      uint32_t p = AL[i];
+      beg = rp[p];
+      deg = rp[p + 1] - beg;
+      v = mprop[p];
+      if (deg >= H) {
+        unsigned long long slot = atomicAdd(&cnt->heavy, 1ull);
+        heavy[slot] = p;
+        my_edges += deg;
+        deg = 0;
+      }
+    }
+    uint32_t total;
+    uint32_t ex = block_excl_scan(deg, &total);
+    s_off[threadIdx.x] = ex;
+    s_beg[threadIdx.x] = beg;
+    s_val[threadIdx.x] = v;
+    if (threadIdx.x == 0) s_off[kBlock] = total;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < total; j += kBlock) {
+      // owner t: last index with s_off[t] <= j
+      uint32_t lo = 0, hi = kBlock;
+      while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (s_off[mid] <= j) lo = mid;
+        else hi = mid;
+      }
+      uint32_t e = s_beg[lo] + (j - s_off[lo]);
+      relax<Op>(upd, dirty, col[e], Op::edge(s_val[lo], wt ? wt[e] : 1u));
+    }
+    my_edges += total;
+    __syncthreads();
+  }
+  my_edges = block_sum(my_edges);
+  if (threadIdx.x == 0 && my_edges) atomicAdd(&cnt->edges, (unsigned long long)my_edges);
+}
+
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_push_heavy(const uint32_t *__restrict__ heavy, const Counters *cnt,
+                                                       const uint32_t *__restrict__ rp, const uint32_t *__restrict__ col,
+                                                       const uint32_t *__restrict__ wt,
+                                                       const typename Op::V *__restrict__ mprop, typename Op::V *upd,
+                                                       uint64_t *dirty)
+{
+  const uint32_t nh = (uint32_t)cnt->heavy;
+  for (uint32_t h = 0; h < nh; h++) {
+    uint32_t p = heavy[h];
+    uint32_t b = rp[p], e = rp[p + 1];
+    typename Op::V v = mprop[p];
+    for (uint32_t x = b + blockIdx.x * kBlock + threadIdx.x; x < e; x += gridDim.x * kBlock)
+      relax<Op>(upd, dirty, col[x], Op::edge(v, wt ? wt[x] : 1u));
+  }
+}
+
+// ------------------------------------------------------------------------
+// traversal: pull over the CSC of every proxy (G lanes per row)
+// ------------------------------------------------------------------------
+
+template <class Op>
+__device__ __forceinline__ typename Op::V combine(typename Op::V a, typename Op::V b)
+{
+  if constexpr (Op::kMin) return a < b ? a : b;
+  else return a + b;
+}
+
+template <class Op, int G>
+__global__ void __launch_bounds__(kBlock) k_pull(uint32_t P, const uint32_t *__restrict__ crp,
+                                                 const uint32_t *__restrict__ ccol, const uint32_t *__restrict__ cw,
+                                                 const uint64_t *__restrict__ Dp, const typename Op::V *__restrict__ mprop,
+                                                 typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty,
+                                                 uint32_t H, Counters *cnt)
+{
+  using V = typename Op::V;
+  constexpr int RPW = kWave / G;  // rows per wave
+  const int sub = lane_id() % G;
+  const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+  uint64_t my_edges = 0;
+  for (uint32_t wv = blockIdx.x * kWavesPerBlock + wave_in_block(); (uint64_t)wv * RPW < P; wv += nwaves) {
+    uint32_t r = wv * RPW + lane_id() / G;
+    uint32_t b = 0, e = 0;
+    if (r < P) {
+      b = crp[r];
+      e = crp[r + 1];
+      if (e - b >= H) e = b;  // heavy rows: k_pull_heavy
+    }
+    V acc = identity<Op>();
+    bool any = false;
+    for (uint32_t x = b + sub; x < e; x += G) {
+      uint32_t src = ccol[x];
+      if ((Dp[src >> 6] >> (src & 63)) & 1ull) {
+        acc = combine<Op>(acc, Op::edge(mprop[src], cw ? cw[x] : 1u));
+        any = true;
+      }
+    }
+    if (sub == 0) my_edges += e - b;
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+      V t = __shfl_xor(acc, o, kWave);
+      int ta = __shfl_xor((int)any, o, kWave);
+      acc = combine<Op>(acc, t);
+      any = any || ta;
+    }
+    bool mark = false;
+    if (sub == 0 && any && r < P) {
+      if constexpr (Op::kMin) {
+        if (acc < upd[r]) {
+          upd[r] = acc;
+          mark = true;
+        }
+      } else {
+        upd[r] = upd[r] + acc;
+        mark = true;
+      }
+    }
+    uint64_t mb = __ballot(mark);
+    if (lane_id() == 0 && mb) {
+      // compress the group-leader bits (every G-th lane) into row bits
+      uint64_t rows = 0;
+      for (int i = 0; i < RPW; i++)
+        if ((mb >> (i * G)) & 1ull) rows |= 1ull << i;
+      uint32_t r0 = wv * RPW;
+      atomicOr((unsigned long long *)(dirty + (r0 >> 6)), (unsigned long long)(rows << (r0 & 63)));
+    }
+  }
+  my_edges = block_sum(my_edges);
+  if (threadIdx.x == 0 && my_edges) atomicAdd(&cnt->edges, (unsigned long long)my_edges);
+}
+
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_pull_heavy(const uint32_t *__restrict__ rows, uint32_t nrows,
+                                                       const uint32_t *__restrict__ crp, const uint32_t *__restrict__ ccol,
+                                                       const uint32_t *__restrict__ cw, const uint64_t *__restrict__ Dp,
+                                                       const typename Op::V *__restrict__ mprop,
+                                                       typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty,
+                                                       Counters *cnt)
+{
+  using V = typename Op::V;
+  for (uint32_t i = blockIdx.x; i < nrows; i += gridDim.x) {
+    uint32_t r = rows[i];
+    uint32_t b = crp[r], e = crp[r + 1];
+    V acc = identity<Op>();
+    uint32_t any = 0;
+    for (uint32_t x = b + threadIdx.x; x < e; x += kBlock) {
+      uint32_t src = ccol[x];
+      if ((Dp[src >> 6] >> (src & 63)) & 1ull) {
+        acc = combine<Op>(acc, Op::edge(mprop[src], cw ? cw[x] : 1u));
+        any = 1;
+      }
+    }
+    if constexpr (Op::kMin) acc = block_min(acc);
+    else acc = block_sum(acc);
+    any = block_max(any);
+    if (threadIdx.x == 0) {
+      if (any) {
+        bool mark = false;
+        if constexpr (Op::kMin) {
+          if (acc < upd[r]) {
+            upd[r] = acc;
+            mark = true;
+          }
+        } else {
+          upd[r] = upd[r] + acc;
+          mark = true;
+        }
+        if (mark) atomicOr((unsigned long long *)(dirty + (r >> 6)), 1ull << (r & 63));
+      }
+      atomicAdd(&cnt->edges, (unsigned long long)(e - b));
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// reduce: memory -> compute (bitvector = dirty words of the pair)
+// ------------------------------------------------------------------------
+
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_reduce_send(uint32_t nw, uint64_t *__restrict__ Du, V *__restrict__ mupd,
+                                                        const uint32_t *__restrict__ wrank, int dense, int clear_vals,
+                                                        uint64_t *__restrict__ bv, V *__restrict__ val, Counters *cnt)
+{
+  const int wpb = kWavesPerBlock;
+  uint32_t mine = 0;
+  for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
+    uint64_t word = Du[w];
+    if (lane_id() == 0) {
+      bv[w] = word;
+      if (word) Du[w] = 0;
+      mine += (uint32_t)__popcll(word);
+    }
+    uint32_t p = w * 64 + lane_id();
+    bool bit = (word >> lane_id()) & 1ull;
+    if (dense) {
+      V x = mupd[p];
+      val[p] = x;
+    } else if (bit) {
+      val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())] = mupd[p];
+    }
+    if (clear_vals && bit) mupd[p] = V(0);  // PropertyList::clear (DistributedGraph.hpp:140-166)
+  }
+  uint32_t t = block_sum(mine);
+  if (threadIdx.x == 0 && t) atomicAdd(&cnt->dirty, (unsigned long long)t);
+}
+
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_reduce_recv(uint32_t nw, const uint64_t *__restrict__ bv,
+                                                        const typename Op::V *__restrict__ val,
+                                                        const uint32_t *__restrict__ wrank,
+                                                        const uint32_t *__restrict__ proxy_slot, int dense,
+                                                        typename Op::V *upd, uint64_t *Dc)
+{
+  const int wpb = kWavesPerBlock;
+  for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
+    uint64_t word = bv[w];
+    if (!word) continue;
+    if ((word >> lane_id()) & 1ull) {
+      uint32_t p = w * 64 + lane_id();
+      typename Op::V x = dense ? val[p] : val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())];
+      uint32_t s = proxy_slot[p];
+      if (s != 0xFFFFFFFFu) relax<Op, true>(upd, Dc, s, x);  // aggregate(): minUpdate / addUpdate
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// update: update_frontier + apply_updates + frontier popcount
+// ------------------------------------------------------------------------
+
+template <int ALGO>  // GRD_PR, GRD_CC or GRD_SSSP
+__global__ void __launch_bounds__(kBlock) k_update(uint32_t nw, uint64_t *__restrict__ Dc, void *prop_, void *upd_,
+                                                   float *__restrict__ pr, const uint32_t *__restrict__ deg,
+                                                   uint64_t *__restrict__ F, uint64_t *__restrict__ Pend,
+                                                   unsigned long long hi, int use_delta, Counters *cnt)
+{
+  const int wpb = kWavesPerBlock;
+  uint64_t f_cnt = 0, f_edges = 0, improved = 0;
+  uint32_t f_max = 0;
+  for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
+    uint64_t d = Dc[w];
+    bool f = false, pend = false, imp = false;
+    uint32_t s = w * 64 + lane_id();
+    if ((d >> lane_id()) & 1ull) {
+      if constexpr (ALGO == GRD_PR) {
+        float *prop = (float *)prop_, *upd = (float *)upd_;
+        float u = upd[s];
+        if ((double)u > 1e-6) {  // pr.cpp:133-142 and :76-83 (prev is always 0 here)
+          f = true;
+          pr[s] = pr[s] + u;
+          prop[s] = (float)(0.85 * (double)u / (double)deg[s]);
+          upd[s] = 0.0f;
+        }
+      } else {
+        uint32_t *prop = (uint32_t *)prop_, *upd = (uint32_t *)upd_;
+        uint32_t u = upd[s];
+        if (prop[s] > u) {  // cc.cpp:100-106, sssp.cpp:102-108
+          prop[s] = u;
+          imp = true;
+          if (use_delta && (unsigned long long)u >= hi) pend = true;
+          else f = true;
+        }
+      }
+    }
+    uint64_t fm = __ballot(f);
+    if (lane_id() == 0) {
+      F[w] = fm;
+      if (d) Dc[w] = 0;
+    }
+    if (use_delta) {
+      uint64_t pm = __ballot(pend);
+      if (lane_id() == 0 && (pm | fm)) Pend[w] = (Pend[w] | pm) & ~fm;
+    }
+    if (f) {
+      uint32_t dg = deg[s];
+      f_edges += dg;
+      f_max = dg > f_max ? dg : f_max;
+    }
+    improved += imp;
+    if (lane_id() == 0) f_cnt += (uint64_t)__popcll(fm);
+  }
+  uint64_t a = block_sum(f_cnt);
+  uint64_t b = block_sum(f_edges);
+  uint64_t c = block_sum(improved);
+  uint32_t mx = block_max(f_max);
+  if (threadIdx.x == 0) {
+    if (a) atomicAdd(&cnt->frontier, (unsigned long long)a);
+    if (b) atomicAdd(&cnt->frontier_edges, (unsigned long long)b);
+    if (c) atomicAdd(&cnt->improved, (unsigned long long)c);
+    if (mx) atomicMax(&cnt->frontier_maxdeg, mx);
+  }
+}
+
+// delta-stepping: smallest pending distance and pending count
+__global__ void __launch_bounds__(kBlock) k_pend_min(uint32_t nw, const uint64_t *__restrict__ Pend,
+                                                     const uint32_t *__restrict__ prop, Counters *cnt)
+{
+  const int wpb = kWavesPerBlock;
+  uint32_t mn = 0xFFFFFFFFu;
+  uint64_t c = 0;
+  for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
+    uint64_t pm = Pend[w];
+    if ((pm >> lane_id()) & 1ull) {
+      uint32_t v = prop[w * 64 + lane_id()];
+      mn = v < mn ? v : mn;
+      c++;
+    }
+  }
+  mn = block_min(mn);
+  c = block_sum(c);
+  if (threadIdx.x == 0 && c) {
+    atomicMin(&cnt->pend_min, mn);
+    atomicAdd(&cnt->pending, (unsigned long long)c);
+  }
+}
+
+// delta-stepping: move pending vertices below the new bucket bound into F
+__global__ void __launch_bounds__(kBlock) k_select(uint32_t nw, uint64_t *__restrict__ Pend,
+                                                   const uint32_t *__restrict__ prop, const uint32_t *__restrict__ deg,
+                                                   unsigned long long hi, uint64_t *__restrict__ F, Counters *cnt)
+{
+  const int wpb = kWavesPerBlock;
+  uint64_t f_cnt = 0, f_edges = 0;
+  uint32_t f_max = 0;
+  for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
+    uint64_t pm = Pend[w];
+    bool f = false;
+    uint32_t s = w * 64 + lane_id();
+    if ((pm >> lane_id()) & 1ull) f = (unsigned long long)prop[s] < hi;
+    uint64_t fm = __ballot(f);
+    if (lane_id() == 0) {
+      F[w] = fm;
+      if (fm) Pend[w] = pm & ~fm;
+      f_cnt += (uint64_t)__popcll(fm);
+    }
+    if (f) {
+      uint32_t dg = deg[s];
+      f_edges += dg;
+      f_max = dg > f_max ? dg : f_max;
+    }
+  }
+  uint64_t a = block_sum(f_cnt), b = block_sum(f_edges);
+  uint32_t mx = block_max(f_max);
+  if (threadIdx.x == 0) {
+    if (a) atomicAdd(&cnt->frontier, (unsigned long long)a);
+    if (b) atomicAdd(&cnt->frontier_edges, (unsigned long long)b);
+    if (mx) atomicMax(&cnt->frontier_maxdeg, mx);
+  }
+}
+
+__global__ void k_zero_counters(Counters *cnt)
+{
+  if (threadIdx.x == 0) {
+    cnt->frontier = 0;
+    cnt->frontier_edges = 0;
+    cnt->dirty = 0;
+    cnt->pending = 0;
+    cnt->improved = 0;
+    cnt->heavy = 0;
+    cnt->edges = 0;
+    cnt->frontier_maxdeg = 0;
+    cnt->pend_min = 0xFFFFFFFFu;
+  }
+}
+
+// single-element aggregate() (API compatibility path)
+template <class Op>
+__global__ void k_aggregate_one(typename Op::V *upd, uint64_t *Dc, uint32_t slot, typename Op::V x)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) relax<Op, true>(upd, Dc, slot, x);
+}
+
+// results by gid
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_gather_result(uint32_t S, const uint32_t *__restrict__ slot_gid,
+                                                          const V *__restrict__ src, V *__restrict__ out)
+{
+  for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < S; s += gridDim.x * kBlock) {
+    uint32_t g = slot_gid[s];
+    if (g != 0xFFFFFFFFu) out[g] = src[s];
+  }
+}
+
+// ------------------------------------------------------------------------
+// host orchestration
+// ------------------------------------------------------------------------
+
+grd_status op_zero_counters(grd_state *s)
+{
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s->g->ctx->stream, s->cnt);
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+
+grd_status op_read_counters(grd_state *s)
+{
+  GRD_HIP(hipMemcpyAsync(s->hcnt, s->cnt, sizeof(Counters), hipMemcpyDeviceToHost, s->g->ctx->stream));
+  GRD_HIP(hipStreamSynchronize(s->g->ctx->stream));
+  return GRD_OK;
+}
+
+grd_status state_alloc(grd_state *s)
+{
+  grd_graph *g = s->g;
+  const uint64_t S = g->S, P = g->P;
+  const uint32_t nwS = (uint32_t)(S / 64), nwP = (uint32_t)(P / 64);
+  const uint64_t V = 4;
+  DevMem &m = s->mem;
+  GRD_TRY(m.alloc(&s->prop, S * V));
+  GRD_TRY(m.alloc(&s->upd, S * V));
+  if (s->algo == GRD_PR || s->algo == GRD_PR_PULL) GRD_TRY(m.alloc_n(&s->pr, S));
+  GRD_TRY(m.alloc_n(&s->F, nwS));
+  GRD_TRY(m.alloc_n(&s->Dc, nwS));
+  GRD_TRY(m.alloc_n(&s->Pend, nwS));
+  GRD_TRY(m.alloc(&s->mprop, P * V));
+  GRD_TRY(m.alloc(&s->mupd, P * V));
+  GRD_TRY(m.alloc_n(&s->Dp, nwP));
+  GRD_TRY(m.alloc_n(&s->Du, nwP));
+  GRD_TRY(m.alloc_n(&s->bvB, nwS));
+  GRD_TRY(m.alloc_n(&s->bvR, nwP));
+  GRD_TRY(m.alloc(&s->valB, S * V));
+  GRD_TRY(m.alloc(&s->valR, P * V));
+  uint32_t nb = (std::max(nwS, nwP) + kWordsPerScanBlock - 1) / kWordsPerScanBlock + 1;
+  GRD_TRY(m.alloc_n(&s->blk, nb));
+  GRD_TRY(m.alloc_n(&s->wrankB, nwS));
+  GRD_TRY(m.alloc_n(&s->wrankR, nwP));
+  GRD_TRY(m.alloc_n(&s->totals, 64));
+  GRD_TRY(m.alloc_n(&s->AL, S));
+  GRD_TRY(m.alloc_n(&s->heavy, S));
+  GRD_TRY(m.alloc_n(&s->cnt, 1));
+  GRD_HIP(hipHostMalloc((void **)&s->hcnt, sizeof(Counters), hipHostMallocDefault));
+  memset(s->hcnt, 0, sizeof(Counters));
+  // snapshot list: every array whose content carries state between iterations
+  s->snap = {{s->prop, nullptr}, {s->upd, nullptr}, {s->F, nullptr},     {s->Dc, nullptr},
+             {s->Pend, nullptr}, {s->mprop, nullptr}, {s->mupd, nullptr}, {s->Dp, nullptr},
+             {s->Du, nullptr}};
+  s->snap_bytes = {S * V, S * V, nwS * 8ull, nwS * 8ull, nwS * 8ull, P * V, P * V, nwP * 8ull, nwP * 8ull};
+  if (s->pr) {
+    s->snap.push_back({s->pr, nullptr});
+    s->snap_bytes.push_back(S * 4);
+  }
+  return GRD_OK;
+}
+
+static grd_status fill_u32(hipStream_t st, void *p, uint64_t n, uint32_t v)
+{
+  hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)std::min<uint64_t>(4096, (n + kBlock - 1) / kBlock + 1)), dim3(kBlock), 0,
+                     st, (uint32_t *)p, n, v);
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+static grd_status fill_u64(hipStream_t st, void *p, uint64_t n, uint64_t v)
+{
+  hipLaunchKernelGGL(k_fill_u64, dim3((uint32_t)std::min<uint64_t>(4096, (n + kBlock - 1) / kBlock + 1)), dim3(kBlock), 0,
+                     st, (uint64_t *)p, n, v);
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+
+grd_status state_init(grd_state *s)
+{
+  grd_graph *g = s->g;
+  hipStream_t st = g->ctx->stream;
+  const uint32_t nwS = g->S / 64, nwP = g->P / 64;
+  GRD_TRY(fill_u64(st, s->Dc, nwS, 0));
+  GRD_TRY(fill_u64(st, s->Pend, nwS, 0));
+  GRD_TRY(fill_u64(st, s->Dp, nwP, 0));
+  GRD_TRY(fill_u64(st, s->Du, nwP, 0));
+  GRD_TRY(fill_u64(st, s->bvB, nwS, 0));
+  GRD_TRY(fill_u64(st, s->bvR, nwP, 0));
+  const uint32_t neutral = (s->algo == GRD_PR) ? 0u : 0xFFFFFFFFu;  // PR: 0.0f bit pattern
+  GRD_TRY(fill_u32(st, s->mprop, g->P, neutral));
+  GRD_TRY(fill_u32(st, s->mupd, g->P, neutral));
+  GRD_TRY(fill_u32(st, s->valR, g->P, neutral));
+  GRD_TRY(fill_u32(st, s->valB, g->S, neutral));
+  uint32_t source_slot = 0xFFFFFFFFu;
+  if (s->algo == GRD_SSSP) source_slot = g->h_gid_slot[s->prm.source];
+  hipLaunchKernelGGL(k_init_compute, dim3(wave_grid(g->ctx, nwS)), dim3(kBlock), 0, st, s->algo, nwS, g->slot_gid,
+                     g->slot_deg, s->prop, s->upd, s->pr, s->F, source_slot);
+  GRD_HIP(hipGetLastError());
+  GRD_TRY(op_zero_counters(s));
+  s->hi = (s->algo == GRD_SSSP && s->prm.delta) ? (uint64_t)s->prm.delta : ~0ull;
+  // initial frontier statistics for the first schedule decision
+  uint64_t fr = 0, fe = 0;
+  uint32_t mx = 0;
+  for (uint32_t v = 0; v < g->n; v++) {
+    bool in = false;
+    if (s->algo == GRD_PR) in = g->h_deg[v] != 0;
+    else if (s->algo == GRD_CC) in = true;
+    else in = (v == s->prm.source);
+    if (in) {
+      fr++;
+      fe += g->h_deg[v];
+      mx = std::max(mx, g->h_deg[v]);
+    }
+  }
+  s->last_frontier = fr;
+  s->prev_frontier = fr;
+  s->last_frontier_edges = fe;
+  s->last_maxdeg = mx;
+  s->last_dirty = 0;
+  s->iteration = 0;
+  s->bytes = 0;
+  s->edges = 0;
+  return GRD_OK;
+}
+
+// translation-table words of every pair (for the exchange byte count)
+static uint64_t bitvector_bytes(const grd_graph *g)
+{
+  uint64_t b = 0;
+  for (int c = 0; c < g->C; c++)
+    for (int m = 0; m < g->M; m++) {
+      uint64_t T = (uint64_t)g->ccnt[(size_t)c * g->M + m] + g->xcnt[(size_t)m * g->C + c];
+      b += 8 * ((T + 63) / 64);
+    }
+  return b;
+}
+
+grd_status op_broadcast(grd_state *s, int option)
+{
+  grd_graph *g = s->g;
+  hipStream_t st = g->ctx->stream;
+  const uint32_t nwS = g->S / 64;
+  const bool pull = option & 1, dense = option & 2;
+  const bool need_rank = !dense || !pull;
+  if (need_rank) GRD_TRY(word_ranks(st, s->F, nwS, s->blk, s->wrankB, s->totals + 0));
+  const uint32_t grid = wave_grid(g->ctx, nwS);
+  hipLaunchKernelGGL(k_bcast_send<uint32_t>, dim3(grid), dim3(kBlock), 0, st, nwS, s->F, (const uint32_t *)s->prop,
+                     s->wrankB, (int)dense, s->bvB, (uint32_t *)s->valB);
+  hipLaunchKernelGGL(k_bcast_recv<uint32_t>, dim3(grid), dim3(kBlock), 0, st, nwS, s->bvB, (const uint32_t *)s->valB,
+                     s->wrankB, g->word_dst, (int)dense, (int)!pull, (uint32_t *)s->mprop, s->Dp, s->AL);
+  GRD_HIP(hipGetLastError());
+  // exchange bytes (GraphAlgorithm.cpp:251-265): bitvector of every pair +
+  // packed values (sparse) or the pair's full own slice (dense)
+  uint64_t vals = dense ? (uint64_t)g->n : s->last_frontier;
+  s->bytes += bitvector_bytes(g) + vals * s->vbytes;
+  return GRD_OK;
+}
+
+template <class Op>
+static grd_status traverse_t(grd_state *s, int option)
+{
+  grd_graph *g = s->g;
+  grd_ctx *ctx = g->ctx;
+  hipStream_t st = ctx->stream;
+  using V = typename Op::V;
+  const uint32_t *wt = std::is_same<Op, OpWeighted>::value ? g->w : nullptr;
+  const uint32_t *cwt = std::is_same<Op, OpWeighted>::value ? g->cw : nullptr;
+  if (option & 1) {
+    const uint32_t H = g->heavy_in_threshold;
+    const uint32_t rows_per_block = kBlock / 8;
+    uint32_t want = (g->P + rows_per_block - 1) / rows_per_block;
+    uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(want, ctx->num_cus * 16));
+    hipLaunchKernelGGL((k_pull<Op, 8>), dim3(grid), dim3(kBlock), 0, st, g->P, g->crp, g->ccol, cwt, s->Dp,
+                       (const V *)s->mprop, (V *)s->mupd, s->Du, H, s->cnt);
+    if (g->n_heavy_in)
+      hipLaunchKernelGGL((k_pull_heavy<Op>), dim3(std::min<uint32_t>(g->n_heavy_in, ctx->num_cus * 8)), dim3(kBlock), 0,
+                         st, g->heavy_in, g->n_heavy_in, g->crp, g->ccol, cwt, s->Dp, (const V *)s->mprop,
+                         (V *)s->mupd, s->Du, s->cnt);
+  } else {
+    const uint32_t H = 4096;
+    uint32_t want = (uint32_t)std::min<uint64_t>((s->last_frontier + kBlock - 1) / kBlock, (uint64_t)ctx->num_cus * 8);
+    uint32_t grid = std::max<uint32_t>(1, want);
+    hipLaunchKernelGGL((k_push<Op>), dim3(grid), dim3(kBlock), 0, st, s->AL, s->totals + 0, g->rp, g->col, wt,
+                       (const V *)s->mprop, (V *)s->mupd, s->Du, s->heavy, s->cnt, H);
+    hipLaunchKernelGGL((k_push_heavy<Op>), dim3(ctx->num_cus * 4), dim3(kBlock), 0, st, s->heavy, s->cnt, g->rp, g->col,
+                       wt, (const V *)s->mprop, (V *)s->mupd, s->Du);
+  }
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+
+grd_status op_traverse(grd_state *s, int option)
+{
+  switch (s->algo) {
+  case GRD_PR: return traverse_t<OpPR>(s, option);
+  case GRD_CC: return traverse_t<OpCC>(s, option);
+  case GRD_SSSP: return s->prm.weighted ? traverse_t<OpWeighted>(s, option) : traverse_t<OpHop>(s, option);
+  default: return fail(GRD_ERR_STATE, "traverse: algorithm has no BSP traversal");
+  }
+}
+
+template <class Op>
+static grd_status reduce_t(grd_state *s, int option)
+{
+  grd_graph *g = s->g;
+  hipStream_t st = g->ctx->stream;
+  using V = typename Op::V;
+  const uint32_t nwP = g->P / 64;
+  const bool dense = option & 2;
+  if (!dense) GRD_TRY(word_ranks(st, s->Du, nwP, s->blk, s->wrankR, s->totals + 1));
+  const uint32_t grid = wave_grid(g->ctx, nwP);
+  hipLaunchKernelGGL(k_reduce_send<V>, dim3(grid), dim3(kBlock), 0, st, nwP, s->Du, (V *)s->mupd, s->wrankR, (int)dense,
+                     (int)(s->algo == GRD_PR), s->bvR, (V *)s->valR, s->cnt);
+  hipLaunchKernelGGL((k_reduce_recv<Op>), dim3(grid), dim3(kBlock), 0, st, nwP, s->bvR, (const V *)s->valR, s->wrankR,
+                     g->proxy_slot, (int)dense, (V *)s->upd, s->Dc);
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+
+grd_status op_reduce(grd_state *s, int option)
+{
+  switch (s->algo) {
+  case GRD_PR: return reduce_t<OpPR>(s, option);
+  case GRD_CC: return reduce_t<OpCC>(s, option);
+  case GRD_SSSP: return reduce_t<OpHop>(s, option);  // combine = min for both SSSP forms
+  default: return fail(GRD_ERR_STATE, "reduce: algorithm has no BSP exchange");
+  }
+}
+
+grd_status op_update(grd_state *s)
+{
+  grd_graph *g = s->g;
+  hipStream_t st = g->ctx->stream;
+  const uint32_t nwS = g->S / 64;
+  const uint32_t grid = wave_grid(g->ctx, nwS);
+  const int use_delta = (s->algo == GRD_SSSP && s->prm.delta) ? 1 : 0;
+  switch (s->algo) {
+  case GRD_PR:
+    hipLaunchKernelGGL(k_update<GRD_PR>, dim3(grid), dim3(kBlock), 0, st, nwS, s->Dc, s->prop, s->upd, s->pr,
+                       g->slot_deg, s->F, s->Pend, (unsigned long long)s->hi, 0, s->cnt);
+    break;
+  case GRD_CC:
+    hipLaunchKernelGGL(k_update<GRD_CC>, dim3(grid), dim3(kBlock), 0, st, nwS, s->Dc, s->prop, s->upd, s->pr,
+                       g->slot_deg, s->F, s->Pend, (unsigned long long)s->hi, 0, s->cnt);
+    break;
+  case GRD_SSSP:
+    hipLaunchKernelGGL(k_update<GRD_SSSP>, dim3(grid), dim3(kBlock), 0, st, nwS, s->Dc, s->prop, s->upd, s->pr,
+                       g->slot_deg, s->F, s->Pend, (unsigned long long)s->hi, use_delta, s->cnt);
+    break;
+  default: return fail(GRD_ERR_STATE, "update: algorithm has no BSP update");
+  }
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+
+// Delta-stepping bucket advance when the frontier came out empty.  Reads the
+// counters (synchronizes) and launches k_select.  *any_pending = frontier
+// non-empty afterwards.
+grd_status op_select_bucket(grd_state *s, bool *any_pending)
+{
+  grd_graph *g = s->g;
+  hipStream_t st = g->ctx->stream;
+  const uint32_t nwS = g->S / 64;
+  const uint32_t grid = wave_grid(g->ctx, nwS);
+  *any_pending = false;
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, s->cnt);
+  hipLaunchKernelGGL(k_pend_min, dim3(grid), dim3(kBlock), 0, st, nwS, s->Pend, (const uint32_t *)s->prop, s->cnt);
+  GRD_HIP(hipGetLastError());
+  GRD_TRY(op_read_counters(s));
+  if (s->hcnt->pending == 0) return GRD_OK;
+  uint64_t mn = s->hcnt->pend_min, d = s->prm.delta;
+  s->hi = (mn / d + 1) * d;
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, s->cnt);
+  hipLaunchKernelGGL(k_select, dim3(grid), dim3(kBlock), 0, st, nwS, s->Pend, (const uint32_t *)s->prop, g->slot_deg,
+                     (unsigned long long)s->hi, s->F, s->cnt);
+  GRD_HIP(hipGetLastError());
+  GRD_TRY(op_read_counters(s));
+  *any_pending = s->hcnt->frontier != 0;
+  return GRD_OK;
+}
+
+grd_status op_aggregate_one(grd_state *s, uint32_t slot, const void *value)
+{
+  hipStream_t st = s->g->ctx->stream;
+  if (s->algo == GRD_PR) {
+    float x;
+    memcpy(&x, value, 4);
+    hipLaunchKernelGGL(k_aggregate_one<OpPR>, dim3(1), dim3(64), 0, st, (float *)s->upd, s->Dc, slot, x);
+  } else {
+    uint32_t x;
+    memcpy(&x, value, 4);
+    hipLaunchKernelGGL(k_aggregate_one<OpCC>, dim3(1), dim3(64), 0, st, (uint32_t *)s->upd, s->Dc, slot, x);
+  }
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+
+grd_status op_download(grd_state *s, void *out)
+{
+  grd_graph *g = s->g;
+  hipStream_t st = g->ctx->stream;
+  void *dout = nullptr;
+  GRD_HIP(hipMallocAsync(&dout, (uint64_t)g->n * 4, st));
+  const void *src = (s->algo == GRD_PR || s->algo == GRD_PR_PULL) ? (const void *)s->pr : s->prop;
+  hipLaunchKernelGGL(k_gather_result<uint32_t>, dim3(std::min<uint32_t>(4096, g->S / kBlock + 1)), dim3(kBlock), 0, st,
+                     g->S, g->slot_gid, (const uint32_t *)src, (uint32_t *)dout);
+  GRD_HIP(hipGetLastError());
+  GRD_HIP(hipMemcpyAsync(out, dout, (uint64_t)g->n * 4, hipMemcpyDeviceToHost, st));
+  GRD_HIP(hipFreeAsync(dout, st));
+  GRD_HIP(hipStreamSynchronize(st));
+  return GRD_OK;
+}
+
+}  // namespace grd

@@ -1,0 +1,193 @@
+// grd_internal.hpp -- host-side structures of libgrudon_hip.so.
+//
+// Device layout (all partitions of one graph resident in one GPU's HBM):
+//
+//   compute slots  [0, S): compute partition c, then mirror group m, each
+//                  (c, m) group padded to a multiple of 64 slots.  Within a
+//                  group vertices are in ascending gid, so the group is
+//                  exactly the "own-on-m" part of the reference's compute
+//                  sTranslationTable[m] (DistributedGraph.cpp:701-717) and the
+//                  wire index of a slot is slot - cbase(c, m).
+//   memory proxies [0, P): memory partition m, then master c, then the own
+//                  region (ascending gid) and the cross-mirror region
+//                  (ascending gid), each padded to 64.  The (m, c) own+cross
+//                  range is the reference's memory-side translation table for
+//                  master c in the same order (DistributedGraph.cpp:644-660).
+//
+// Because every pair region starts on a 64-bit word, a pair's bitvector is a
+// word-aligned slice of the frontier / dirty bitmaps; bitvector sync copies
+// words and packs values by popcount prefix.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/grudon_hip.h"
+
+namespace grd {
+
+void set_error(const std::string &msg);
+grd_status fail(grd_status st, const std::string &msg);
+
+#define GRD_HIP(expr)                                                                            \
+  do {                                                                                           \
+    hipError_t _e = (expr);                                                                      \
+    if (_e != hipSuccess)                                                                        \
+      return ::grd::fail(GRD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));       \
+  } while (0)
+
+#define GRD_TRY(expr)                 \
+  do {                                \
+    grd_status _s = (expr);           \
+    if (_s != GRD_OK) return _s;      \
+  } while (0)
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+inline uint64_t pad64(uint64_t x) { return (x + 63) & ~uint64_t(63); }
+
+// Device allocation list owned by a graph or a state.
+struct DevMem {
+  std::vector<void *> ptrs;
+  uint64_t bytes = 0;
+  grd_status alloc(void **p, uint64_t nbytes);
+  template <typename T>
+  grd_status alloc_n(T **p, uint64_t n) { return alloc(reinterpret_cast<void **>(p), n * sizeof(T)); }
+  void release();
+  ~DevMem() { release(); }
+};
+
+// Counters written by device kernels and read back once per iteration.
+struct Counters {
+  unsigned long long frontier;        // popcount of the new frontier
+  unsigned long long frontier_edges;  // out-degree sum of the new frontier
+  unsigned long long dirty;           // dirty memory proxies at reduce
+  unsigned long long pending;         // SSSP delta-stepping: pending vertices
+  unsigned long long improved;        // compute vertices improved by update
+  unsigned long long heavy;           // push: heavy vertices appended this iteration
+  unsigned long long edges;           // edges examined by the traversal
+  unsigned int frontier_maxdeg;
+  unsigned int pend_min;
+  double residual;                    // PR pull: L1 change
+  double dangling[2];                 // PR pull: FP64 dangling mass (ping-pong)
+  unsigned long long pad[4];
+};
+
+}  // namespace grd
+
+struct grd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int host_threads = 0;
+  int num_cus = 256;
+  // learned selector weights (BF16, device) and threshold
+  uint16_t *sel_w = nullptr;
+  bool sel_loaded = false;
+  float sel_margin = 0.0f;
+  float *sel_out = nullptr;       // device: 4 logits + chosen option
+  float *sel_out_host = nullptr;  // pinned
+};
+
+struct grd_graph {
+  grd_ctx *ctx = nullptr;
+  uint32_t n = 0;
+  uint64_t m = 0;
+  int C = 1, M = 1;
+  bool weighted = false;
+  uint32_t S = 0;       // compute slots (padded)
+  uint32_t P = 0;       // memory proxies (padded)
+  uint64_t X = 0;       // cross mirrors
+  // host metadata
+  std::vector<uint32_t> cbase, ccnt;            // [C*M] pair (c,m): slot base and own count
+  std::vector<uint32_t> obase, ocnt, xbase, xcnt;  // [M*C] pair (m,c): proxy bases and counts
+  std::vector<uint32_t> h_gid_slot;             // gid -> compute slot
+  std::vector<uint32_t> h_deg;                  // gid -> out-degree
+  std::vector<uint32_t> h_proxy_gid;            // proxy -> gid (kNone for padding)
+  double avg_degree = 0.0;
+  // device arrays
+  grd::DevMem mem;
+  uint32_t *slot_gid = nullptr;    // [S]
+  uint32_t *slot_deg = nullptr;    // [S] global out-degree
+  uint32_t *word_dst = nullptr;    // [S/64] proxy word receiving each slot word
+  uint32_t *proxy_slot = nullptr;  // [P]
+  uint32_t *rp = nullptr, *col = nullptr, *w = nullptr;     // push CSR over proxies
+  uint32_t *crp = nullptr, *ccol = nullptr, *cw = nullptr;  // pull CSC over proxies
+  uint32_t *xptr = nullptr, *xlist = nullptr;               // [S+1], [X] cross proxies per slot
+  uint32_t *heavy_in = nullptr;  // pull rows with in-degree >= heavy_in_threshold
+  uint32_t n_heavy_in = 0;
+  uint32_t heavy_in_threshold = 0;
+  std::vector<uint32_t> h_in_deg;  // per proxy (host), to rebuild the heavy list
+};
+
+struct grd_state {
+  grd_graph *g = nullptr;
+  int algo = GRD_CC;
+  grd_params prm{};
+  bool initialized = false;
+  uint32_t iteration = 0;
+  uint64_t hi = ~0ull;  // SSSP delta-stepping bucket bound
+  int last_option = 0;
+  uint64_t last_frontier = 0, prev_frontier = 0, last_frontier_edges = 0, last_dirty = 0;
+  uint32_t last_maxdeg = 0;
+  uint64_t bytes = 0, edges = 0;
+  uint32_t vbytes = 4;  // sizeof(VP)
+  grd::DevMem mem;
+  // compute side [S]
+  void *prop = nullptr, *upd = nullptr;
+  float *pr = nullptr;
+  uint64_t *F = nullptr, *Dc = nullptr, *Pend = nullptr;  // [S/64]
+  // memory side [P]
+  void *mprop = nullptr, *mupd = nullptr;
+  uint64_t *Dp = nullptr, *Du = nullptr;  // [P/64]
+  // wire
+  uint64_t *bvB = nullptr, *bvR = nullptr;
+  void *valB = nullptr, *valR = nullptr;
+  // scan scratch
+  uint32_t *blk = nullptr, *wrankB = nullptr, *wrankR = nullptr, *totals = nullptr;
+  uint32_t *AL = nullptr, *heavy = nullptr;
+  grd::Counters *cnt = nullptr;   // device
+  grd::Counters *hcnt = nullptr;  // pinned host
+  // snapshot copies
+  grd::DevMem snap_mem;
+  std::vector<std::pair<void *, void *>> snap;  // (live, copy)
+  std::vector<uint64_t> snap_bytes;
+  uint32_t snap_iteration = 0;
+  uint64_t snap_hi = 0;
+};
+
+namespace grd {
+// layout.cpp
+grd_status build_graph(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const uint32_t *col, const uint32_t *w,
+                       int C, int M, const uint32_t *mirror, grd_graph **out);
+grd_status rebuild_heavy_in(grd_graph *g, uint32_t threshold);
+void parallel_for(int nthreads, uint64_t n, uint64_t grain, const std::function<void(uint64_t, uint64_t)> &fn);
+int host_threads(const grd_ctx *ctx);
+// gen.hip
+grd_status generate_rmat(grd_ctx *ctx, int scale, int ef, double a, double b, double c, uint64_t seed, int sym,
+                         std::vector<uint32_t> &rp, std::vector<uint32_t> &col);
+grd_status generate_er(grd_ctx *ctx, uint32_t n, double deg, uint64_t seed, std::vector<uint32_t> &rp,
+                       std::vector<uint32_t> &col);
+void host_weights(uint32_t n, const uint32_t *rp, const uint32_t *col, uint64_t wseed, uint32_t *w, int nthreads);
+// bsp.hip
+grd_status state_alloc(grd_state *s);
+grd_status state_init(grd_state *s);
+grd_status op_broadcast(grd_state *s, int option);
+grd_status op_traverse(grd_state *s, int option);
+grd_status op_reduce(grd_state *s, int option);
+grd_status op_update(grd_state *s);
+grd_status op_select_bucket(grd_state *s, bool *any_pending);
+grd_status op_aggregate_one(grd_state *s, uint32_t slot, const void *value);
+grd_status op_download(grd_state *s, void *out);
+grd_status op_features(grd_state *s, float *out16);
+grd_status op_zero_counters(grd_state *s);
+grd_status op_read_counters(grd_state *s);
+// prpull.hip
+grd_status prp_init(grd_state *s);
+grd_status prp_iteration(grd_state *s);
+// mlp.hip
+grd_status mlp_select(grd_ctx *ctx, const float *feat16_host, int *option, float *margin);
+}  // namespace grd

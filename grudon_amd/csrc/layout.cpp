@@ -1,0 +1,353 @@
+// layout.cpp -- partitioned graph construction (host) and upload to HBM.
+//
+// Restates the reference's DistributedGraph constructor
+// (src/DistributedGraph.cpp:13-785) for all C compute and M memory partitions
+// at once, with dense arrays instead of its per-rank unordered_maps:
+//   master(g) = g % C                        Partitioner.hpp:32, :176
+//   mirror(g) = g % M, or mirror[g] from a partition file when M > 1
+//                                            Partitioner.hpp:33, :73-105, :177
+//   cross mirrors of m = {v : u -> v, mirror(u) = m, mirror(v) != m}
+//                                            DistributedGraph.cpp:345-349, :367-371
+//   memory lids: own (ascending gid) then cross (ascending gid)
+//                                            DistributedGraph.cpp:517-525, :579-588
+//   translation tables: compute side own-on-m then cross-on-m, memory side
+//   own then cross per master, both ascending  DistributedGraph.cpp:644-660, :701-734
+// The resulting slots/proxies are stored in the padded, pair-grouped order
+// described in grd_internal.hpp.  Local CSR rows keep the global row order
+// (DistributedGraph.cpp:590-611); the pull CSC rows are sorted by source.
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <functional>
+#include <new>
+#include <thread>
+
+#include "grd_internal.hpp"
+
+namespace grd {
+
+int host_threads(const grd_ctx *ctx)
+{
+  int t = ctx && ctx->host_threads > 0 ? ctx->host_threads : (int)std::thread::hardware_concurrency();
+  if (t < 1) t = 1;
+  if (ctx == nullptr || ctx->host_threads <= 0) t = std::min(t, 16);
+  return t;
+}
+
+void parallel_for(int nthreads, uint64_t n, uint64_t grain, const std::function<void(uint64_t, uint64_t)> &fn)
+{
+  if (n == 0) return;
+  if (grain == 0) grain = 1;
+  if (nthreads <= 1 || n <= grain) {
+    fn(0, n);
+    return;
+  }
+  std::atomic<uint64_t> next{0};
+  auto body = [&]() {
+    for (;;) {
+      uint64_t b = next.fetch_add(grain);
+      if (b >= n) break;
+      fn(b, std::min(n, b + grain));
+    }
+  };
+  std::vector<std::thread> th;
+  int nt = (int)std::min<uint64_t>((uint64_t)nthreads, (n + grain - 1) / grain);
+  for (int t = 1; t < nt; t++) th.emplace_back(body);
+  body();
+  for (auto &x : th) x.join();
+}
+
+grd_status DevMem::alloc(void **p, uint64_t nbytes)
+{
+  *p = nullptr;
+  uint64_t b = nbytes ? nbytes : 64;
+  hipError_t e = hipMalloc(p, b);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(GRD_ERR_OOM, "hipMalloc(" + std::to_string(b) + " bytes) failed: " + hipGetErrorString(e));
+  }
+  ptrs.push_back(*p);
+  bytes += b;
+  return GRD_OK;
+}
+
+void DevMem::release()
+{
+  for (void *p : ptrs) (void)hipFree(p);
+  ptrs.clear();
+  bytes = 0;
+}
+
+template <typename T>
+static grd_status upload(grd_graph *g, T **dst, const std::vector<T> &src)
+{
+  GRD_TRY(g->mem.alloc_n(dst, src.size()));
+  if (!src.empty()) GRD_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return GRD_OK;
+}
+
+grd_status rebuild_heavy_in(grd_graph *g, uint32_t threshold)
+{
+  std::vector<uint32_t> rows;
+  for (uint32_t p = 0; p < g->P; p++)
+    if (g->h_in_deg[p] >= threshold) rows.push_back(p);
+  if (g->heavy_in) {
+    auto it = std::find(g->mem.ptrs.begin(), g->mem.ptrs.end(), (void *)g->heavy_in);
+    if (it != g->mem.ptrs.end()) g->mem.ptrs.erase(it);
+    (void)hipFree(g->heavy_in);
+    g->heavy_in = nullptr;
+  }
+  g->heavy_in_threshold = threshold;
+  g->n_heavy_in = (uint32_t)rows.size();
+  if (rows.empty()) rows.push_back(kNone);
+  return upload(g, &g->heavy_in, rows);
+}
+
+static grd_status build_impl(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const uint32_t *col, const uint32_t *w,
+                             int C, int M, const uint32_t *mirror_in, grd_graph **out)
+{
+  if (n == 0 || C < 1 || M < 1 || C > 64 || M > 64)
+    return fail(GRD_ERR_ARG, "graph: need n >= 1 and 1 <= C, M <= 64");
+  const uint64_t m = rp[n];
+  if (m >= 0xFFFFFFF0ull) return fail(GRD_ERR_ARG, "graph: more than 2^32-16 arcs is beyond the u32 offset limit");
+  for (uint32_t v = 0; v < n; v++)
+    if (rp[v + 1] < rp[v]) return fail(GRD_ERR_ARG, "graph: row_ptr is not monotone");
+  const int T = host_threads(ctx);
+  const uint32_t NW = (n + 63) / 64;
+
+  grd_graph *g = new grd_graph();
+  g->ctx = ctx;
+  g->n = n;
+  g->m = m;
+  g->C = C;
+  g->M = M;
+  g->weighted = (w != nullptr);
+  g->avg_degree = (double)m / (double)n;
+
+  // master / mirror (Partitioner.hpp)
+  std::vector<uint8_t> mir(n);
+  for (uint32_t v = 0; v < n; v++) {
+    uint32_t mm = (M == 1 || mirror_in == nullptr) ? v % (uint32_t)M : mirror_in[v];
+    if (mm >= (uint32_t)M) {
+      delete g;
+      return fail(GRD_ERR_ARG, "graph: mirror partition id " + std::to_string(mm) + " >= M");
+    }
+    mir[v] = (uint8_t)mm;
+  }
+  for (uint32_t e = 0; e < m; e++)
+    if (col[e] >= n) {
+      delete g;
+      return fail(GRD_ERR_ARG, "graph: column index out of range");
+    }
+  g->h_deg.resize(n);
+  for (uint32_t v = 0; v < n; v++) g->h_deg[v] = rp[v + 1] - rp[v];
+
+  // cross-node mirrors: one bitmap per memory partition
+  std::vector<uint64_t> crossw((uint64_t)M * NW, 0);
+  parallel_for(T, n, 4096, [&](uint64_t b, uint64_t e) {
+    for (uint64_t u = b; u < e; u++) {
+      uint32_t mu = mir[u];
+      for (uint32_t x = rp[u]; x < rp[u + 1]; x++) {
+        uint32_t v = col[x];
+        if (mir[v] != mu) {
+          uint64_t *wd = &crossw[(uint64_t)mu * NW + (v >> 6)];
+          uint64_t bit = 1ull << (v & 63);
+          if (!(__atomic_load_n(wd, __ATOMIC_RELAXED) & bit)) __atomic_fetch_or(wd, bit, __ATOMIC_RELAXED);
+        }
+      }
+    }
+  });
+
+  // group sizes
+  g->ccnt.assign((size_t)C * M, 0);
+  g->xcnt.assign((size_t)M * C, 0);
+  for (uint32_t v = 0; v < n; v++) g->ccnt[(size_t)(v % C) * M + mir[v]]++;
+  for (int mm = 0; mm < M; mm++)
+    for (uint32_t wd = 0; wd < NW; wd++) {
+      uint64_t x = crossw[(uint64_t)mm * NW + wd];
+      while (x) {
+        uint32_t v = wd * 64 + (uint32_t)__builtin_ctzll(x);
+        x &= x - 1;
+        g->xcnt[(size_t)mm * C + (v % C)]++;
+      }
+    }
+  // bases
+  g->cbase.assign((size_t)C * M, 0);
+  uint64_t cur = 0;
+  for (int c = 0; c < C; c++)
+    for (int mm = 0; mm < M; mm++) {
+      g->cbase[(size_t)c * M + mm] = (uint32_t)cur;
+      cur += pad64(g->ccnt[(size_t)c * M + mm]);
+    }
+  if (cur >= 0xFFFFFF00ull) {
+    delete g;
+    return fail(GRD_ERR_ARG, "graph: compute slots exceed u32");
+  }
+  g->S = (uint32_t)std::max<uint64_t>(cur, 64);
+  g->obase.assign((size_t)M * C, 0);
+  g->ocnt.assign((size_t)M * C, 0);
+  g->xbase.assign((size_t)M * C, 0);
+  cur = 0;
+  uint64_t X = 0;
+  for (int mm = 0; mm < M; mm++)
+    for (int c = 0; c < C; c++) {
+      size_t k = (size_t)mm * C + c;
+      g->ocnt[k] = g->ccnt[(size_t)c * M + mm];
+      g->obase[k] = (uint32_t)cur;
+      cur += pad64(g->ocnt[k]);
+      g->xbase[k] = (uint32_t)cur;
+      cur += pad64(g->xcnt[k]);
+      X += g->xcnt[k];
+    }
+  if (cur >= 0xFFFFFF00ull) {
+    delete g;
+    return fail(GRD_ERR_ARG, "graph: memory proxies exceed u32");
+  }
+  g->P = (uint32_t)std::max<uint64_t>(cur, 64);
+  g->X = X;
+
+  // own slot / own proxy per gid
+  std::vector<uint32_t> own_proxy(n);
+  g->h_gid_slot.assign(n, kNone);
+  {
+    std::vector<uint32_t> ctr((size_t)C * M, 0);
+    for (uint32_t v = 0; v < n; v++) {
+      uint32_t c = v % C, mm = mir[v];
+      uint32_t r = ctr[(size_t)c * M + mm]++;
+      g->h_gid_slot[v] = g->cbase[(size_t)c * M + mm] + r;
+      own_proxy[v] = g->obase[(size_t)mm * C + c] + r;
+    }
+  }
+  // cross proxy per (m, gid)
+  std::vector<uint32_t> xp((uint64_t)M * n, kNone);
+  g->h_proxy_gid.assign(g->P, kNone);
+  for (uint32_t v = 0; v < n; v++) g->h_proxy_gid[own_proxy[v]] = v;
+  for (int mm = 0; mm < M; mm++) {
+    std::vector<uint32_t> ctr(C, 0);
+    for (uint32_t wd = 0; wd < NW; wd++) {
+      uint64_t x = crossw[(uint64_t)mm * NW + wd];
+      while (x) {
+        uint32_t v = wd * 64 + (uint32_t)__builtin_ctzll(x);
+        x &= x - 1;
+        uint32_t c = v % C;
+        uint32_t p = g->xbase[(size_t)mm * C + c] + ctr[c]++;
+        xp[(uint64_t)mm * n + v] = p;
+        g->h_proxy_gid[p] = v;
+      }
+    }
+  }
+  std::vector<uint64_t>().swap(crossw);
+
+  // per-slot and per-proxy arrays
+  std::vector<uint32_t> slot_gid(g->S, kNone), slot_deg(g->S, 0);
+  for (uint32_t v = 0; v < n; v++) {
+    slot_gid[g->h_gid_slot[v]] = v;
+    slot_deg[g->h_gid_slot[v]] = g->h_deg[v];
+  }
+  std::vector<uint32_t> proxy_slot(g->P, kNone);
+  for (uint32_t p = 0; p < g->P; p++)
+    if (g->h_proxy_gid[p] != kNone) proxy_slot[p] = g->h_gid_slot[g->h_proxy_gid[p]];
+  std::vector<uint32_t> word_dst(g->S / 64, kNone);
+  for (int c = 0; c < C; c++)
+    for (int mm = 0; mm < M; mm++) {
+      uint32_t sb = g->cbase[(size_t)c * M + mm] / 64;
+      uint32_t nwd = (uint32_t)(pad64(g->ccnt[(size_t)c * M + mm]) / 64);
+      uint32_t pb = g->obase[(size_t)mm * C + c] / 64;
+      for (uint32_t k = 0; k < nwd; k++) word_dst[sb + k] = pb + k;
+    }
+
+  // push CSR over proxies
+  std::vector<uint32_t> prp((uint64_t)g->P + 1, 0);
+  for (uint32_t v = 0; v < n; v++) prp[own_proxy[v] + 1] = g->h_deg[v];
+  for (uint32_t p = 0; p < g->P; p++) prp[p + 1] += prp[p];
+  std::vector<uint32_t> pcol(m), pw(w ? m : 0);
+  parallel_for(T, n, 2048, [&](uint64_t b, uint64_t e) {
+    for (uint64_t u = b; u < e; u++) {
+      uint32_t mm = mir[u];
+      uint32_t o = prp[own_proxy[u]];
+      for (uint32_t x = rp[u]; x < rp[u + 1]; x++, o++) {
+        uint32_t v = col[x];
+        pcol[o] = (mir[v] == mm) ? own_proxy[v] : xp[(uint64_t)mm * n + v];
+        if (w) pw[o] = w[x];
+      }
+    }
+  });
+  std::vector<uint32_t>().swap(xp);
+
+  // pull CSC over proxies: count, prefix, fill, sort rows by (src, weight)
+  std::vector<uint32_t> crp((uint64_t)g->P + 1, 0);
+  {
+    std::vector<uint32_t> indeg(g->P, 0);
+    for (uint64_t x = 0; x < m; x++) indeg[pcol[x]]++;
+    for (uint32_t p = 0; p < g->P; p++) crp[p + 1] = crp[p] + indeg[p];
+    g->h_in_deg.swap(indeg);
+  }
+  std::vector<uint32_t> ccol(m), cwv(w ? m : 0);
+  {
+    // sources are visited in ascending proxy order, so every CSC row comes
+    // out sorted by source (the graph has no duplicate arcs)
+    std::vector<uint32_t> cursor(crp.begin(), crp.end() - 1);
+    for (uint32_t p = 0; p < g->P; p++)
+      for (uint32_t x = prp[p]; x < prp[p + 1]; x++) {
+        uint32_t o = cursor[pcol[x]]++;
+        ccol[o] = p;
+        if (w) cwv[o] = pw[x];
+      }
+  }
+
+  // cross proxies per compute slot (PR pull aggregation), ascending proxy order
+  std::vector<uint32_t> xptr((uint64_t)g->S + 1, 0), xlist(X);
+  for (uint32_t p = 0; p < g->P; p++) {
+    uint32_t v = g->h_proxy_gid[p];
+    if (v != kNone && own_proxy[v] != p) xptr[g->h_gid_slot[v] + 1]++;
+  }
+  for (uint32_t s = 0; s < g->S; s++) xptr[s + 1] += xptr[s];
+  {
+    std::vector<uint32_t> cursor(xptr.begin(), xptr.end() - 1);
+    for (uint32_t p = 0; p < g->P; p++) {
+      uint32_t v = g->h_proxy_gid[p];
+      if (v != kNone && own_proxy[v] != p) xlist[cursor[g->h_gid_slot[v]]++] = p;
+    }
+  }
+
+  // upload
+  grd_status st = GRD_OK;
+  if ((st = upload(g, &g->slot_gid, slot_gid)) != GRD_OK ||
+      (st = upload(g, &g->slot_deg, slot_deg)) != GRD_OK ||
+      (st = upload(g, &g->word_dst, word_dst)) != GRD_OK ||
+      (st = upload(g, &g->proxy_slot, proxy_slot)) != GRD_OK ||
+      (st = upload(g, &g->rp, prp)) != GRD_OK ||
+      (st = upload(g, &g->col, pcol)) != GRD_OK ||
+      (st = upload(g, &g->crp, crp)) != GRD_OK ||
+      (st = upload(g, &g->ccol, ccol)) != GRD_OK ||
+      (st = upload(g, &g->xptr, xptr)) != GRD_OK ||
+      (st = upload(g, &g->xlist, xlist)) != GRD_OK) {
+    delete g;
+    return st;
+  }
+  if (w) {
+    if ((st = upload(g, &g->w, pw)) != GRD_OK || (st = upload(g, &g->cw, cwv)) != GRD_OK) {
+      delete g;
+      return st;
+    }
+  }
+  if ((st = rebuild_heavy_in(g, 2048)) != GRD_OK) {
+    delete g;
+    return st;
+  }
+  *out = g;
+  return GRD_OK;
+}
+
+grd_status build_graph(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const uint32_t *col, const uint32_t *w, int C,
+                       int M, const uint32_t *mirror, grd_graph **out)
+{
+  try {
+    return build_impl(ctx, n, rp, col, w, C, M, mirror, out);
+  } catch (const std::bad_alloc &) {
+    return fail(GRD_ERR_OOM, "graph: host allocation failed");
+  }
+}
+
+}  // namespace grd

@@ -1,0 +1,211 @@
+/*
+ * grudon_hip.h -- C-ABI of the MI355X (gfx950) graph operator library
+ * (libgrudon_hip.so).  Plain pointers and sizes only; every function returns
+ * a grd_status and never throws.  grd_last_error() gives the message of the
+ * calling thread's last failure.
+ *
+ * The reference drives its per-partition operator path through in-process
+ * C++ virtual calls (include/GraphAlgorithm.hpp:24-86, include/Workers.hpp:12-115)
+ * and MPI point-to-point messages (include/MPI.hpp:8-124).  Here one process
+ * owns every compute and memory partition on one GPU; the exchange between
+ * partitions is done by device kernels over HBM-resident wire buffers.  The
+ * entry points below are what those virtual calls bind to in bulk form.
+ *
+ * Threading: one host thread per grd_ctx.  All operations are ordered on the
+ * context's HIP stream.
+ */
+#ifndef GRUDON_HIP_H
+#define GRUDON_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  GRD_OK = 0,
+  GRD_ERR_ARG = 1,    /* invalid argument or size beyond a stated limit */
+  GRD_ERR_HIP = 2,    /* HIP runtime error (message in grd_last_error) */
+  GRD_ERR_OOM = 3,    /* device or host allocation failed */
+  GRD_ERR_STATE = 4,  /* call out of order (e.g. step before init) */
+  GRD_ERR_IO = 5,     /* graph / partition / weight file could not be read */
+} grd_status;
+
+typedef struct grd_ctx grd_ctx;
+typedef struct grd_graph grd_graph;
+typedef struct grd_state grd_state;
+
+/* Algorithms.  GRD_PR is the reference's delta-push PageRank
+ * (src/graph_algorithms/pr.cpp), GRD_CC src/graph_algorithms/cc.cpp,
+ * GRD_SSSP src/graph_algorithms/sssp.cpp (hop count, or edge weights and
+ * delta-stepping as extensions), GRD_PR_PULL the normalised pull PageRank
+ * (FP32 Kahan contributions, FP64 dangling mass and residual). */
+typedef enum { GRD_PR = 0, GRD_CC = 1, GRD_SSSP = 2, GRD_PR_PULL = 3 } grd_algo;
+
+/* Per-iteration schedule option: bit 0 = pull traversal (else push),
+ * bit 1 = dense sync (else sparse bitvector + packed values). */
+enum { GRD_OPT_PUSH_SPARSE = 0, GRD_OPT_PULL_SPARSE = 1, GRD_OPT_PUSH_DENSE = 2, GRD_OPT_PULL_DENSE = 3 };
+
+/* Schedule selection policy for grd_run. */
+typedef enum {
+  GRD_SCHED_PLAIN = 0,     /* the reference schedule: push + sparse every iteration */
+  GRD_SCHED_HEURISTIC = 1, /* direction-optimizing heuristic */
+  GRD_SCHED_LEARNED = 2,   /* MLP selector with the heuristic as fallback */
+  GRD_SCHED_FIXED = 3      /* the option in grd_params.fixed_option every iteration */
+} grd_sched;
+
+typedef struct {
+  uint32_t source;       /* SSSP source gid (reference: gid 0, sssp.cpp:39-46) */
+  uint32_t weighted;     /* SSSP: use edge weights (else +1 per edge) */
+  uint32_t delta;        /* SSSP delta-stepping bucket width; 0 = reference schedule */
+  uint32_t max_iter;     /* reference MAX_ITERATIONS = 1000 (GraphAlgorithm.hpp:10) */
+  double damping;        /* PR_PULL damping (0.85) */
+  double tolerance;      /* PR_PULL convergence bound on the L1 change per iteration */
+  uint32_t sched;        /* grd_sched */
+  uint32_t fixed_option; /* option for GRD_SCHED_FIXED */
+  uint32_t time_iters;   /* record per-iteration device time in the trace */
+  uint32_t reserved;
+} grd_params;
+
+/* One record per iteration (grd_run trace).  feat holds the selector's 16
+ * input features for the iteration. */
+typedef struct {
+  uint32_t iteration;
+  uint32_t option;          /* schedule option that ran */
+  uint64_t frontier;        /* vertices broadcast this iteration */
+  uint64_t frontier_edges;  /* their out-degree sum */
+  uint64_t dirty;           /* memory proxies sent back (reduce) */
+  uint64_t next_frontier;   /* frontier after the update */
+  double ms;                /* device time of the iteration (time_iters) */
+  float feat[16];
+} grd_iter_rec;
+
+typedef struct {
+  uint64_t iterations;
+  uint64_t bytes_exchanged;  /* bitvectors + packed values, both directions */
+  uint64_t edges_examined;
+  uint64_t hit_max_iter;
+  double ms;                 /* device time of the whole iteration loop */
+  double residual;           /* PR_PULL: last L1 change */
+  uint64_t option_counts[4];
+  uint64_t fallbacks;        /* learned mode: iterations decided by the heuristic */
+} grd_run_stats;
+
+/* ---- context: replaces MPICore (include/MPI.hpp:11-24) ---- */
+const char *grd_last_error(void);
+grd_status grd_create(int device, grd_ctx **out);
+grd_status grd_destroy(grd_ctx *ctx);
+/* Host threads used for graph construction (reference -t, src/main.cpp:25). */
+grd_status grd_set_host_threads(grd_ctx *ctx, int nthreads);
+
+/* ---- graphs ---- */
+
+/* Seeded device generators; fill caller host arrays with the global CSR
+ * (row_ptr[n+1], col[cap]); *m_out = arc count.  Self-loops and duplicates
+ * are removed.  RMAT: n = 2^scale, edge_factor * n raw edges. */
+grd_status grd_generate_rmat(grd_ctx *ctx, int scale, int edge_factor, double a, double b, double c,
+                             uint64_t seed, int symmetrize, uint32_t *row_ptr, uint32_t *col,
+                             uint64_t cap, uint64_t *m_out);
+grd_status grd_generate_er(grd_ctx *ctx, uint32_t n, double mean_degree, uint64_t seed, uint32_t *row_ptr,
+                           uint32_t *col, uint64_t cap, uint64_t *m_out);
+
+/* Partition a global CSR into C compute and M memory partitions resident in
+ * HBM (reference DistributedGraph ctor, src/DistributedGraph.cpp:13-785;
+ * master = gid % C, mirror = gid % M or mirror[gid], Partitioner.hpp:11-205).
+ * weights may be NULL; mirror may be NULL (naive). */
+grd_status grd_graph_from_csr(grd_ctx *ctx, uint32_t n, const uint32_t *row_ptr, const uint32_t *col,
+                              const uint32_t *weights, int num_compute, int num_memory,
+                              const uint32_t *mirror, grd_graph **out);
+/* Generate on the device and partition in one call.  wseed != 0 attaches
+ * weights uniform in [1,255] (a symmetric function of the endpoints). */
+grd_status grd_graph_rmat(grd_ctx *ctx, int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                          int symmetrize, uint64_t wseed, int num_compute, int num_memory, grd_graph **out);
+grd_status grd_graph_er(grd_ctx *ctx, uint32_t n, double mean_degree, uint64_t seed, uint64_t wseed,
+                        int num_compute, int num_memory, grd_graph **out);
+/* Read a Galois .gr (v1/v2) or MatrixMarket .mtx file; partition_file (may
+ * be NULL/empty) holds one memory-partition id per line (Partitioner.hpp:89-105). */
+grd_status grd_graph_load(grd_ctx *ctx, const char *path, const char *partition_file, int num_compute,
+                          int num_memory, grd_graph **out);
+grd_status grd_graph_free(grd_graph *g);
+/* info[0] n, [1] m, [2] C, [3] M, [4] compute slots, [5] memory proxies,
+ * [6] cross mirrors, [7] weighted, [8] device bytes held by the graph. */
+grd_status grd_graph_info(const grd_graph *g, uint64_t *info16);
+/* Translation table of pair (c, m) as gids in wire-index order: the
+ * reference sTranslationTable/rTranslationTable content
+ * (DistributedGraph.cpp:701-734).  Returns T in *count; out may be NULL. */
+grd_status grd_graph_table(const grd_graph *g, int c, int m, uint32_t *out_gids, uint64_t *count);
+/* Out-degree per gid (host copy). */
+grd_status grd_graph_out_degree(const grd_graph *g, uint32_t *out);
+
+/* ---- algorithm state: GraphAlgorithm<VP> (GraphAlgorithm.hpp:24-86) ---- */
+grd_status grd_state_create(grd_graph *g, int algo, grd_state **out);
+grd_status grd_state_free(grd_state *s);
+/* init(): pr.cpp:20-61 / cc.cpp:18-40 / sssp.cpp:18-47 */
+grd_status grd_init(grd_state *s, const grd_params *p);
+/* The superstep in bulk form (GraphAlgorithm.cpp:177-690):
+ *   grd_broadcast  compute -> memory: frontier bitvector + values   (:199-343)
+ *   grd_traverse   gen_updates on every memory partition           (:348-356)
+ *   grd_reduce     memory -> compute: dirty bitvector + values and
+ *                  aggregate() into vertex_updates                  (:377-581)
+ *   grd_update     update_frontier + apply_updates + termination    (:643-662)
+ * option = GRD_OPT_*; *frontier_out = next frontier size. */
+grd_status grd_broadcast(grd_state *s, int option);
+grd_status grd_traverse(grd_state *s, int option);
+grd_status grd_reduce(grd_state *s, int option);
+grd_status grd_update(grd_state *s, uint64_t *frontier_out);
+/* Per-element aggregate(lid, value) of the reference API
+ * (GraphAlgorithm.hpp:42) on a compute vertex given by gid; slow path kept
+ * for API compatibility.  value points to a uint32 or float. */
+grd_status grd_aggregate_one(grd_state *s, uint32_t gid, const void *value);
+/* Whole loop with the chosen schedule policy; trace (may be NULL) receives
+ * min(iterations, trace_cap) records. */
+grd_status grd_run(grd_state *s, grd_run_stats *stats, grd_iter_rec *trace, uint32_t trace_cap);
+/* Result by gid: CC labels / SSSP distances (uint32), PR values (float). */
+grd_status grd_download(grd_state *s, void *out_by_gid);
+/* Selector features of the current state (16 floats). */
+grd_status grd_features(grd_state *s, float *out16);
+/* Copy / restore every mutable array of the state (used for labelling). */
+grd_status grd_snapshot(grd_state *s);
+grd_status grd_restore(grd_state *s);
+
+/* ---- schedule selector (learned MLP 16 -> 64 -> 64 -> 4) ---- */
+/* Load BF16 weights (5,508 values: W1[64][16] b1[64] W2[64][64] b2[64]
+ * W3[4][64] b3[4]) into the
+ * context; the learned policy falls back to the heuristic when none is loaded. */
+grd_status grd_selector_load(grd_ctx *ctx, const uint16_t *bf16_weights, uint64_t count, float min_margin);
+/* Train on the GPU with the project's forward/backward kernels: feat[n*16],
+ * label[n] in 0..3, cost[n*4] (per-option times, used to weight the loss).
+ * Writes the trained BF16 weights to out (may be NULL) and loads them. */
+grd_status grd_selector_train(grd_ctx *ctx, const float *feat, const uint32_t *label, const float *cost,
+                              uint64_t n, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
+                              uint16_t *out_bf16, float *final_loss);
+/* Forward pass on the GPU: logits[n*4]. */
+grd_status grd_selector_infer(grd_ctx *ctx, const float *feat, uint64_t n, float *logits);
+/* One forward+backward on the GPU from FP32 weights w[5508] (no BF16
+ * rounding); sample_weight[n] may be NULL (all 1).  Writes the summed
+ * weighted cross-entropy and its gradient (for the CPU gradient check). */
+grd_status grd_selector_grad(grd_ctx *ctx, const float *w, const float *feat, const uint32_t *label,
+                             const float *sample_weight, uint64_t n, float *loss, float *grad);
+
+/* ---- batched form: many graphs in one persistent launch ---- */
+typedef struct grd_batch grd_batch;
+grd_status grd_batch_create(grd_ctx *ctx, uint32_t count, const uint32_t *n, const uint64_t *m,
+                            const uint32_t *const *row_ptr, const uint32_t *const *col, grd_batch **out);
+grd_status grd_batch_free(grd_batch *b);
+/* Tunables: rows per tile, heavy-row threshold, lanes per row. 0 = keep. */
+grd_status grd_batch_tune(grd_batch *b, uint32_t tile_rows, uint32_t heavy_threshold, uint32_t lanes);
+/* Autotune the tunables by timing candidate settings on this batch. */
+grd_status grd_batch_autotune(grd_batch *b, uint32_t *chosen3);
+/* PageRank pull (Kahan FP32, FP64 dangling/residual) on every graph, with
+ * iterations captured in a hipGraph; iters_out[count], ms_out may be NULL. */
+grd_status grd_batch_pagerank(grd_batch *b, double damping, double tolerance, uint32_t max_iter,
+                              uint32_t *iters_out, double *ms_out);
+grd_status grd_batch_cc(grd_batch *b, uint32_t max_iter, uint32_t *iters_out, double *ms_out);
+/* Results of graph i: PR (float[n]) or CC labels (uint32[n]). */
+grd_status grd_batch_download(grd_batch *b, uint32_t i, int algo, void *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

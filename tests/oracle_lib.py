@@ -1,0 +1,190 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module.  The oracle is the checker, never the
+thing measured as the product.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_P32 = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_P64 = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_PF64 = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+ORC_PR, ORC_CC, ORC_SSSP = 0, 1, 2
+RMAT_ABC = (0.57, 0.19, 0.19)
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library missing: run __graft_entry__.build() first (%s)" % LIB_PATH)
+        L = ct.CDLL(LIB_PATH)
+        L.orc_hash.restype = ct.c_uint64
+        L.orc_hash.argtypes = [ct.c_uint64, ct.c_uint64]
+        L.orc_rmat.restype = ct.c_int64
+        L.orc_rmat.argtypes = [ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_double, ct.c_uint64, ct.c_int,
+                               _P32, _P32, ct.c_int64]
+        L.orc_er.restype = ct.c_int64
+        L.orc_er.argtypes = [ct.c_uint32, ct.c_double, ct.c_uint64, _P32, _P32, ct.c_int64]
+        L.orc_weights.restype = None
+        L.orc_weights.argtypes = [ct.c_uint32, _P32, _P32, ct.c_uint64, _P32]
+        L.orc_csr_from_pairs.restype = ct.c_int64
+        L.orc_csr_from_pairs.argtypes = [ct.c_uint32, _P64, ct.c_int64, _P32, _P32]
+        L.orc_part_build.restype = ct.c_void_p
+        L.orc_part_build.argtypes = [ct.c_uint32, _P32, _P32, ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p]
+        L.orc_part_free.restype = None
+        L.orc_part_free.argtypes = [ct.c_void_p]
+        L.orc_part_query.restype = ct.c_int64
+        L.orc_part_query.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p]
+        L.orc_bsp_u32.restype = ct.c_int
+        L.orc_bsp_u32.argtypes = [ct.c_void_p, ct.c_int, ct.c_uint32, ct.c_int, ct.c_uint32, ct.c_int, ct.c_int,
+                                  _P32, _P64, ct.c_void_p]
+        L.orc_bsp_pr.restype = ct.c_int
+        L.orc_bsp_pr.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, _PF64, _P64, ct.c_void_p]
+        L.orc_bfs.restype = None
+        L.orc_bfs.argtypes = [ct.c_uint32, _P32, _P32, ct.c_uint32, _P32]
+        L.orc_dijkstra.restype = None
+        L.orc_dijkstra.argtypes = [ct.c_uint32, _P32, _P32, _P32, ct.c_uint32, _P32]
+        L.orc_cc_unionfind.restype = None
+        L.orc_cc_unionfind.argtypes = [ct.c_uint32, _P32, _P32, _P32]
+        L.orc_pr_pull.restype = ct.c_int
+        L.orc_pr_pull.argtypes = [ct.c_uint32, _P32, _P32, ct.c_double, ct.c_int, ct.c_double, _PF64,
+                                  ct.POINTER(ct.c_double)]
+        _lib = L
+    return _lib
+
+
+class Graph:
+    """Global CSR (row_ptr u32[n+1], col u32[m], optional weights u32[m])."""
+
+    def __init__(self, rp, col, w=None):
+        self.rp = np.ascontiguousarray(rp, dtype=np.uint32)
+        self.col = np.ascontiguousarray(col, dtype=np.uint32)
+        self.w = None if w is None else np.ascontiguousarray(w, dtype=np.uint32)
+        self.n = len(self.rp) - 1
+        self.m = len(self.col)
+
+    def with_weights(self, wseed):
+        w = np.zeros(self.m, dtype=np.uint32)
+        lib().orc_weights(self.n, self.rp, self.col, wseed, w)
+        return Graph(self.rp, self.col, w)
+
+    def out_degree(self):
+        return np.diff(self.rp.astype(np.int64))
+
+
+def rmat(scale, ef=16, seed=1, symmetrize=False, abc=RMAT_ABC):
+    n = 1 << scale
+    raw = (ef << scale) * (2 if symmetrize else 1)
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    col = np.zeros(max(raw, 1), dtype=np.uint32)
+    m = lib().orc_rmat(scale, ef, abc[0], abc[1], abc[2], seed, 1 if symmetrize else 0, rp, col, raw)
+    assert m >= 0
+    return Graph(rp, col[:m].copy())
+
+
+def er(n, mean_degree, seed):
+    cap = int(n * mean_degree / 2.0) * 2
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    col = np.zeros(max(cap, 1), dtype=np.uint32)
+    m = lib().orc_er(n, mean_degree, seed, rp, col, cap)
+    assert m >= 0
+    return Graph(rp, col[:m].copy())
+
+
+def from_edges(n, edges):
+    """CSR from an (src, dst) edge list: sorted, self-loops and duplicates dropped."""
+    keys = np.array([(int(s) << 32) | int(d) for s, d in edges] or [0], dtype=np.uint64)
+    cnt = len(edges)
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    col = np.zeros(max(cnt, 1), dtype=np.uint32)
+    m = lib().orc_csr_from_pairs(n, keys, cnt, rp, col)
+    return Graph(rp, col[:m].copy())
+
+
+class Partition:
+    """Restated DistributedGraph layout for C compute and M memory partitions."""
+
+    def __init__(self, g, C, M, mirror=None):
+        self.g = g
+        self.C, self.M = C, M
+        self._mirror = None if mirror is None else np.ascontiguousarray(mirror, dtype=np.uint32)
+        wptr = None if g.w is None else g.w.ctypes.data
+        mptr = None if self._mirror is None else self._mirror.ctypes.data
+        self.h = lib().orc_part_build(g.n, g.rp, g.col, wptr, C, M, mptr)
+        if not self.h:
+            raise ValueError("invalid partition")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_part_free(self.h)
+            self.h = None
+
+    def query(self, what, i=0, j=0, fetch=True):
+        cnt = lib().orc_part_query(self.h, what, i, j, None)
+        if not fetch:
+            return cnt
+        out = np.zeros(max(cnt, 1), dtype=np.uint32)
+        lib().orc_part_query(self.h, what, i, j, out.ctypes.data)
+        return out[:cnt]
+
+    def cc(self, max_iter=1000, nthreads=1, trace=False):
+        out = np.zeros(self.g.n, dtype=np.uint32)
+        st = np.zeros(4, dtype=np.uint64)
+        tr = np.zeros(max_iter + 1, dtype=np.uint32) if trace else None
+        rc = lib().orc_bsp_u32(self.h, ORC_CC, 0, 0, 0, max_iter, nthreads, out, st,
+                               None if tr is None else tr.ctypes.data)
+        assert rc == 0
+        return (out, st, tr[: int(st[0])]) if trace else (out, st)
+
+    def sssp(self, source=0, weighted=False, delta=0, max_iter=100000, nthreads=1, trace=False):
+        out = np.zeros(self.g.n, dtype=np.uint32)
+        st = np.zeros(4, dtype=np.uint64)
+        tr = np.zeros(max_iter + 1, dtype=np.uint32) if trace else None
+        rc = lib().orc_bsp_u32(self.h, ORC_SSSP, source, 1 if weighted else 0, delta, max_iter, nthreads, out, st,
+                               None if tr is None else tr.ctypes.data)
+        assert rc == 0
+        return (out, st, tr[: int(st[0])]) if trace else (out, st)
+
+    def pr(self, fp64=False, max_iter=1000, nthreads=1, trace=False):
+        out = np.zeros(self.g.n, dtype=np.float64)
+        st = np.zeros(4, dtype=np.uint64)
+        tr = np.zeros(max_iter + 1, dtype=np.uint32) if trace else None
+        rc = lib().orc_bsp_pr(self.h, 1 if fp64 else 0, max_iter, nthreads, out, st,
+                              None if tr is None else tr.ctypes.data)
+        assert rc == 0
+        return (out, st, tr[: int(st[0])]) if trace else (out, st)
+
+
+def bfs(g, src):
+    out = np.zeros(g.n, dtype=np.uint32)
+    lib().orc_bfs(g.n, g.rp, g.col, src, out)
+    return out
+
+
+def dijkstra(g, src):
+    out = np.zeros(g.n, dtype=np.uint32)
+    lib().orc_dijkstra(g.n, g.rp, g.col, g.w, src, out)
+    return out
+
+
+def cc_unionfind(g):
+    out = np.zeros(g.n, dtype=np.uint32)
+    lib().orc_cc_unionfind(g.n, g.rp, g.col, out)
+    return out
+
+
+def pr_pull(g, damping=0.85, iters=0, tol=1e-10):
+    r = np.zeros(g.n, dtype=np.float64)
+    res = ct.c_double(0.0)
+    it = lib().orc_pr_pull(g.n, g.rp, g.col, damping, iters, tol, r, ct.byref(res))
+    return r, it, res.value
