@@ -34,7 +34,7 @@ class Params(ct.Structure):
     _fields_ = [("source", ct.c_uint32), ("weighted", ct.c_uint32), ("delta", ct.c_uint32),
                 ("max_iter", ct.c_uint32), ("damping", ct.c_double), ("tolerance", ct.c_double),
                 ("sched", ct.c_uint32), ("fixed_option", ct.c_uint32), ("time_iters", ct.c_uint32),
-                ("reserved", ct.c_uint32)]
+                ("time_kernel", ct.c_uint32)]
 
 
 class IterRec(ct.Structure):
@@ -46,13 +46,16 @@ class IterRec(ct.Structure):
 class RunStats(ct.Structure):
     _fields_ = [("iterations", ct.c_uint64), ("bytes_exchanged", ct.c_uint64), ("edges_examined", ct.c_uint64),
                 ("hit_max_iter", ct.c_uint64), ("ms", ct.c_double), ("residual", ct.c_double),
-                ("option_counts", ct.c_uint64 * 4), ("fallbacks", ct.c_uint64)]
+                ("option_counts", ct.c_uint64 * 4), ("fallbacks", ct.c_uint64), ("kernel_ms", ct.c_double),
+                ("kernel_launches", ct.c_uint64), ("kernel_bytes", ct.c_uint64)]
 
     def as_dict(self):
         return {"iterations": int(self.iterations), "bytes_exchanged": int(self.bytes_exchanged),
                 "edges_examined": int(self.edges_examined), "hit_max_iter": int(self.hit_max_iter),
                 "ms": float(self.ms), "residual": float(self.residual),
-                "option_counts": [int(x) for x in self.option_counts], "fallbacks": int(self.fallbacks)}
+                "option_counts": [int(x) for x in self.option_counts], "fallbacks": int(self.fallbacks),
+                "kernel_ms": float(self.kernel_ms), "kernel_launches": int(self.kernel_launches),
+                "kernel_bytes": int(self.kernel_bytes)}
 
 
 _lib = None
@@ -268,9 +271,9 @@ class State:
         self.h = h
 
     def init(self, source=0, weighted=False, delta=0, max_iter=1000, damping=0.85, tolerance=1e-6,
-             sched=SCHED_PLAIN, fixed_option=0, time_iters=False):
+             sched=SCHED_PLAIN, fixed_option=0, time_iters=False, time_kernel=False):
         p = Params(source, int(weighted), delta, max_iter, damping, tolerance, sched, fixed_option,
-                   int(time_iters), 0)
+                   int(time_iters), int(time_kernel))
         _check(lib().grd_init(self.h, ct.byref(p)), "grd_init")
 
     def run(self, trace_cap=0):

@@ -72,7 +72,7 @@ def build_host():
     """C++ drop-in operator API + CLI, linked against libgrudon_hip.so."""
     if not os.path.isdir(HOST):
         return None
-    srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp"))
+    srcs = sorted(os.path.join(d, f) for d, _, fs in os.walk(HOST) for f in fs if f.endswith(".cpp"))
     if not srcs:
         return None
     bindir = os.path.join(PKG, "bin")

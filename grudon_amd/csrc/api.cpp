@@ -19,10 +19,6 @@ grd_status fail(grd_status st, const std::string &msg)
   return st;
 }
 
-grd_status read_graph(const std::string &path, uint32_t &n, std::vector<uint32_t> &rp, std::vector<uint32_t> &col,
-                      std::vector<uint32_t> &w, bool &has_w);
-grd_status read_partition_file(const std::string &path, uint32_t n, int M, std::vector<uint32_t> &mirror);
-
 // ------------------------------------------------------------------------
 // schedule policies
 // ------------------------------------------------------------------------
@@ -76,6 +72,22 @@ static uint64_t bitvector_bytes(const grd_graph *g)
   return b;
 }
 
+// after the iteration's counters are read: accumulate the dominant kernel's
+// event time and algorithmic bytes (frontier_k = vertices pushed)
+static grd_status finish_kernel_timing(grd_state *s, uint64_t frontier_k)
+{
+  if (!s->kern_pending) return GRD_OK;
+  float ms = 0.0f;
+  GRD_HIP(hipEventSynchronize(s->kev[1]));
+  GRD_HIP(hipEventElapsedTime(&ms, s->kev[0], s->kev[1]));
+  s->kern_ms += ms;
+  s->kern_launches++;
+  if (s->kern_pending == 2) s->kern_bytes += 12 * s->hcnt->edges + 16 * frontier_k;
+  else s->kern_bytes += s->kern_pending_bytes;
+  s->kern_pending = 0;
+  return GRD_OK;
+}
+
 // one BSP superstep with the given option; reads the counters back
 static grd_status superstep(grd_state *s, int option)
 {
@@ -85,13 +97,14 @@ static grd_status superstep(grd_state *s, int option)
   GRD_TRY(op_reduce(s, option));
   GRD_TRY(op_update(s));
   GRD_TRY(op_read_counters(s));
+  GRD_TRY(finish_kernel_timing(s, s->last_frontier));
   const Counters &c = *s->hcnt;
   const grd_graph *g = s->g;
   // reduce bytes (GraphAlgorithm.cpp:409-423): bitvector + dirty values
   // (sparse) or every translation-table entry (dense)
   uint64_t vals = (option & 2) ? (uint64_t)g->n + g->X : c.dirty;
   s->bytes += bitvector_bytes(g) + vals * s->vbytes;
-  s->edges += c.edges;
+  s->edges += c.edges + c.heavy_edges;
   s->prev_frontier = s->last_frontier;
   s->last_dirty = c.dirty;
   s->last_frontier = c.frontier;
@@ -199,6 +212,14 @@ grd_status grd_generate_er(grd_ctx *c, uint32_t n, double deg, uint64_t seed, ui
   }
 }
 
+grd_status grd_edge_weights(grd_ctx *c, uint32_t n, const uint32_t *rp, const uint32_t *col, uint64_t wseed,
+                            uint32_t *w)
+{
+  if (!c || !rp || !w || (!col && rp[n])) return fail(GRD_ERR_ARG, "grd_edge_weights: NULL argument");
+  host_weights(n, rp, col, wseed, w, host_threads(c));
+  return GRD_OK;
+}
+
 grd_status grd_graph_from_csr(grd_ctx *c, uint32_t n, const uint32_t *rp, const uint32_t *col, const uint32_t *w,
                               int C, int M, const uint32_t *mirror, grd_graph **out)
 {
@@ -280,8 +301,8 @@ grd_status grd_graph_info(const grd_graph *g, uint64_t *info)
   info[6] = g->X;
   info[7] = g->weighted ? 1 : 0;
   info[8] = g->mem.bytes;
-  info[9] = g->n_heavy_in;
-  info[10] = g->heavy_in_threshold;
+  info[9] = g->n_pblk;
+  info[10] = g->n_plong;
   return GRD_OK;
 }
 
@@ -315,8 +336,12 @@ grd_status grd_state_create(grd_graph *g, int algo, grd_state **out)
   s->g = g;
   s->algo = algo;
   grd_status st = state_alloc(s);
+  if (st == GRD_OK && (hipEventCreate(&s->kev[0]) != hipSuccess || hipEventCreate(&s->kev[1]) != hipSuccess))
+    st = fail(GRD_ERR_HIP, "grd_state_create: hipEventCreate failed");
   if (st != GRD_OK) {
     if (s->hcnt) (void)hipHostFree(s->hcnt);
+    for (auto &e : s->kev)
+      if (e) (void)hipEventDestroy(e);
     delete s;
     return st;
   }
@@ -329,6 +354,8 @@ grd_status grd_state_free(grd_state *s)
   if (!s) return GRD_OK;
   (void)hipStreamSynchronize(s->g->ctx->stream);
   if (s->hcnt) (void)hipHostFree(s->hcnt);
+  for (auto &e : s->kev)
+    if (e) (void)hipEventDestroy(e);
   delete s;
   return GRD_OK;
 }
@@ -341,6 +368,9 @@ grd_status grd_init(grd_state *s, const grd_params *p)
     return fail(GRD_ERR_ARG, "grd_init: weighted SSSP on a graph without weights");
   if (p->sched > GRD_SCHED_FIXED || p->fixed_option > 3) return fail(GRD_ERR_ARG, "grd_init: bad schedule");
   s->prm = *p;
+  s->kern_ms = 0.0;
+  s->kern_launches = s->kern_bytes = 0;
+  s->kern_pending = 0;
   if (s->prm.max_iter == 0) s->prm.max_iter = 1000;
   if (s->prm.damping == 0.0) s->prm.damping = 0.85;
   if (s->prm.tolerance == 0.0) s->prm.tolerance = 1e-6;
@@ -354,8 +384,8 @@ grd_status grd_init(grd_state *s, const grd_params *p)
 grd_status grd_broadcast(grd_state *s, int option)
 {
   if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_broadcast: state not initialized");
-  if (s->algo == GRD_PR_PULL) return fail(GRD_ERR_STATE, "grd_broadcast: PR_PULL runs whole iterations");
   if (option < 0 || option > 3) return fail(GRD_ERR_ARG, "grd_broadcast: bad option");
+  if (s->algo == GRD_PR_PULL) return prp_broadcast(s);
   GRD_TRY(op_zero_counters(s));
   return op_broadcast(s, option);
 }
@@ -364,6 +394,7 @@ grd_status grd_traverse(grd_state *s, int option)
 {
   if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_traverse: state not initialized");
   if (option < 0 || option > 3) return fail(GRD_ERR_ARG, "grd_traverse: bad option");
+  if (s->algo == GRD_PR_PULL) return prp_traverse(s);
   return op_traverse(s, option);
 }
 
@@ -371,6 +402,9 @@ grd_status grd_reduce(grd_state *s, int option)
 {
   if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_reduce: state not initialized");
   if (option < 0 || option > 3) return fail(GRD_ERR_ARG, "grd_reduce: bad option");
+  // PR_PULL: the pull already wrote every proxy's partial sum into the
+  // dense reduce buffer; the compute side combines them in grd_update
+  if (s->algo == GRD_PR_PULL) return GRD_OK;
   s->last_option = option;
   return op_reduce(s, option);
 }
@@ -378,13 +412,22 @@ grd_status grd_reduce(grd_state *s, int option)
 grd_status grd_update(grd_state *s, uint64_t *frontier_out)
 {
   if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_update: state not initialized");
-  if (s->algo == GRD_PR_PULL) return fail(GRD_ERR_STATE, "grd_update: PR_PULL runs whole iterations");
+  if (s->algo == GRD_PR_PULL) {
+    GRD_TRY(prp_update(s));
+    GRD_TRY(op_read_counters(s));
+    GRD_TRY(finish_kernel_timing(s, 0));
+    s->iteration++;
+    const bool conv = s->hcnt->residual < s->prm.tolerance;
+    if (frontier_out) *frontier_out = conv ? 0 : s->g->n;
+    return GRD_OK;
+  }
   GRD_TRY(op_update(s));
   GRD_TRY(op_read_counters(s));
+  GRD_TRY(finish_kernel_timing(s, s->last_frontier));
   const Counters &c = *s->hcnt;
   uint64_t vals = (s->last_option & 2) ? (uint64_t)s->g->n + s->g->X : c.dirty;
   s->bytes += bitvector_bytes(s->g) + vals * s->vbytes;
-  s->edges += c.edges;
+  s->edges += c.edges + c.heavy_edges;
   s->prev_frontier = s->last_frontier;
   s->last_dirty = c.dirty;
   s->last_frontier = c.frontier;
@@ -433,6 +476,7 @@ grd_status grd_run(grd_state *s, grd_run_stats *stats, grd_iter_rec *trace, uint
       GRD_TRY(prp_iteration(s));
       if (timing) GRD_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
       GRD_TRY(op_read_counters(s));
+      GRD_TRY(finish_kernel_timing(s, 0));
       s->iteration++;
       st.residual = s->hcnt->residual;
       rec.option = GRD_OPT_PULL_DENSE;
@@ -497,8 +541,83 @@ grd_status grd_run(grd_state *s, grd_run_stats *stats, grd_iter_rec *trace, uint
   st.iterations = s->iteration;
   st.bytes_exchanged = s->bytes;
   st.edges_examined = s->edges;
+  st.kernel_ms = s->kern_ms;
+  st.kernel_launches = s->kern_launches;
+  st.kernel_bytes = s->kern_bytes;
   if (stats) *stats = st;
   return GRD_OK;
+}
+
+grd_status grd_select_option(grd_state *s, int *option)
+{
+  if (!s || !option || !s->initialized) return fail(GRD_ERR_STATE, "grd_select_option: bad state");
+  if (s->algo == GRD_PR_PULL) {
+    *option = GRD_OPT_PULL_DENSE;
+    return GRD_OK;
+  }
+  grd_ctx *ctx = s->g->ctx;
+  switch (s->prm.sched) {
+  case GRD_SCHED_PLAIN: *option = GRD_OPT_PUSH_SPARSE; return GRD_OK;
+  case GRD_SCHED_FIXED: *option = (int)s->prm.fixed_option; return GRD_OK;
+  case GRD_SCHED_HEURISTIC: *option = heuristic_option(s); return GRD_OK;
+  default: {
+    int h = heuristic_option(s), o = h;
+    if (ctx->sel_loaded) {
+      float feat[16], margin = 0.0f;
+      make_features(s, feat);
+      GRD_TRY(mlp_select(ctx, feat, &o, &margin));
+      if (o != h && margin < ctx->sel_margin) o = h;
+    }
+    *option = o;
+    return GRD_OK;
+  }
+  }
+}
+
+grd_status grd_state_stats(const grd_state *s, grd_run_stats *out)
+{
+  if (!s || !out) return fail(GRD_ERR_ARG, "grd_state_stats: NULL argument");
+  memset(out, 0, sizeof(*out));
+  out->iterations = s->iteration;
+  out->bytes_exchanged = s->bytes;
+  out->edges_examined = s->edges;
+  out->residual = s->hcnt ? s->hcnt->residual : 0.0;
+  out->kernel_ms = s->kern_ms;
+  out->kernel_launches = s->kern_launches;
+  out->kernel_bytes = s->kern_bytes;
+  return GRD_OK;
+}
+
+grd_status grd_graph_read(const char *path, uint32_t *n, uint64_t *m, uint32_t *row_ptr, uint32_t *col,
+                          uint32_t *weights, int *has_weights)
+{
+  if (!path || !n || !m) return fail(GRD_ERR_ARG, "grd_graph_read: NULL argument");
+  try {
+    uint32_t nn = 0;
+    std::vector<uint32_t> rp, cl, w;
+    bool hw = false;
+    GRD_TRY(read_graph(path, nn, rp, cl, w, hw));
+    const bool fill = row_ptr != nullptr;
+    if (fill && (*n != nn || *m != cl.size())) return fail(GRD_ERR_ARG, "grd_graph_read: buffer sizes do not match");
+    *n = nn;
+    *m = cl.size();
+    if (has_weights) *has_weights = hw ? 1 : 0;
+    if (fill) {
+      memcpy(row_ptr, rp.data(), rp.size() * 4);
+      if (col && !cl.empty()) memcpy(col, cl.data(), cl.size() * 4);
+      if (weights && hw && !w.empty()) memcpy(weights, w.data(), w.size() * 4);
+    }
+    return GRD_OK;
+  } catch (const std::bad_alloc &) {
+    return fail(GRD_ERR_OOM, "grd_graph_read: host allocation failed");
+  }
+}
+
+grd_status grd_graph_write_gr(const char *path, uint32_t n, const uint32_t *row_ptr, const uint32_t *col,
+                              const uint32_t *weights)
+{
+  if (!path || !row_ptr || (!col && row_ptr[n])) return fail(GRD_ERR_ARG, "grd_graph_write_gr: NULL argument");
+  return write_gr(path, n, row_ptr, col, weights);
 }
 
 grd_status grd_download(grd_state *s, void *out)

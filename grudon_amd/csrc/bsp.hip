@@ -7,7 +7,7 @@
 //
 // Reference loops replaced (src/GraphAlgorithm.cpp):
 //   k_bcast_send / k_bcast_recv   compute pack + memory unpack       :199-343
-//   k_push / k_push_heavy / k_pull*  gen_updates (pr/cc/sssp.cpp)     :348-356
+//   k_push / k_push_heavy / (pull.hip)  gen_updates (pr/cc/sssp.cpp)  :348-356
 //   k_reduce_send / k_reduce_recv memory pack + compute aggregate     :377-581
 //   k_update                      update_frontier + apply_updates +
 //                                 termination popcount               :643-662
@@ -17,75 +17,9 @@
 
 #include "grd_device.hpp"
 #include "grd_internal.hpp"
+#include "grd_ops.hpp"
 
 namespace grd {
-
-// ------------------------------------------------------------------------
-// operator policies
-// ------------------------------------------------------------------------
-
-// CC: label propagation, atomicMin (cc.cpp:65-78, DistributedGraph.hpp:103-112)
-struct OpCC {
-  using V = uint32_t;
-  static constexpr bool kMin = true;
-  __device__ static V edge(V v, uint32_t) { return v; }
-};
-// SSSP hop count: dist + 1 (sssp.cpp:69)
-struct OpHop {
-  using V = uint32_t;
-  static constexpr bool kMin = true;
-  __device__ static V edge(V v, uint32_t) { return v + 1u; }
-};
-// SSSP weighted (extension): saturating dist + w
-struct OpWeighted {
-  using V = uint32_t;
-  static constexpr bool kMin = true;
-  __device__ static V edge(V v, uint32_t w)
-  {
-    uint32_t t = v + w;
-    return t < v ? 0xFFFFFFFFu : t;
-  }
-};
-// Reference PageRank: push the vertex property, atomicAdd (pr.cpp:95-120)
-struct OpPR {
-  using V = float;
-  static constexpr bool kMin = false;
-  __device__ static V edge(V v, uint32_t) { return v; }
-};
-
-__device__ __forceinline__ void set_bit(uint64_t *bits, uint32_t i)
-{
-  uint64_t b = 1ull << (i & 63);
-  uint64_t *wd = bits + (i >> 6);
-  if (!(__atomic_load_n(wd, __ATOMIC_RELAXED) & b)) atomicOr((unsigned long long *)wd, (unsigned long long)b);
-}
-
-// minUpdate / addUpdate, marking the dirty bit (DistributedGraph.hpp:103-130).
-// minUpdate marks only on improvement.  addUpdate always marks: on the
-// memory side a proxy's update is zero until its first add of the iteration
-// (the previous reduce cleared it), so the first add (old == 0) marks; on the
-// compute side residuals below the threshold persist, so every add marks.
-template <class Op, bool kAlwaysMark = false>
-__device__ __forceinline__ void relax(typename Op::V *upd, uint64_t *dirty, uint32_t dst, typename Op::V cand)
-{
-  if constexpr (Op::kMin) {
-    uint32_t cur = __atomic_load_n(upd + dst, __ATOMIC_RELAXED);
-    if (cand < cur) {
-      uint32_t old = atomicMin(upd + dst, cand);
-      if (cand < old) set_bit(dirty, dst);
-    }
-  } else {
-    float old = atomicAdd(upd + dst, cand);
-    if (kAlwaysMark || old == 0.0f) set_bit(dirty, dst);
-  }
-}
-
-template <class Op>
-__device__ __forceinline__ typename Op::V identity()
-{
-  if constexpr (Op::kMin) return 0xFFFFFFFFu;
-  else return 0.0f;
-}
 
 // ------------------------------------------------------------------------
 // word popcount ranks (exclusive prefix of set bits per 64-bit word)
@@ -294,7 +228,7 @@ __global__ void __launch_bounds__(kBlock) k_push(const uint32_t *__restrict__ AL
   for (uint32_t tile = blockIdx.x * kBlock; tile < k; tile += gridDim.x * kBlock) {
     uint32_t i = tile + threadIdx.x;
     uint32_t deg = 0, beg = 0;
-    V v = identity<Op>();
+    V v = op_identity<Op>();
     if (i < k) {
       uint32_t p = AL[i];
       beg = rp[p];
@@ -303,7 +237,7 @@ __global__ void __launch_bounds__(kBlock) k_push(const uint32_t *__restrict__ AL
       if (deg >= H) {
         unsigned long long slot = atomicAdd(&cnt->heavy, 1ull);
         heavy[slot] = p;
-        my_edges += deg;
+        atomicAdd(&cnt->heavy_edges, (unsigned long long)deg);
         deg = 0;
       }
     }
@@ -325,7 +259,7 @@ __global__ void __launch_bounds__(kBlock) k_push(const uint32_t *__restrict__ AL
       uint32_t e = s_beg[lo] + (j - s_off[lo]);
       relax<Op>(upd, dirty, col[e], Op::edge(s_val[lo], wt ? wt[e] : 1u));
     }
-    my_edges += total;
+    if (threadIdx.x == 0) my_edges += total;
     __syncthreads();
   }
   my_edges = block_sum(my_edges);
@@ -346,123 +280,6 @@ __global__ void __launch_bounds__(kBlock) k_push_heavy(const uint32_t *__restric
     typename Op::V v = mprop[p];
     for (uint32_t x = b + blockIdx.x * kBlock + threadIdx.x; x < e; x += gridDim.x * kBlock)
       relax<Op>(upd, dirty, col[x], Op::edge(v, wt ? wt[x] : 1u));
-  }
-}
-
-// ------------------------------------------------------------------------
-// traversal: pull over the CSC of every proxy (G lanes per row)
-// ------------------------------------------------------------------------
-
-template <class Op>
-__device__ __forceinline__ typename Op::V combine(typename Op::V a, typename Op::V b)
-{
-  if constexpr (Op::kMin) return a < b ? a : b;
-  else return a + b;
-}
-
-template <class Op, int G>
-__global__ void __launch_bounds__(kBlock) k_pull(uint32_t P, const uint32_t *__restrict__ crp,
-                                                 const uint32_t *__restrict__ ccol, const uint32_t *__restrict__ cw,
-                                                 const uint64_t *__restrict__ Dp, const typename Op::V *__restrict__ mprop,
-                                                 typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty,
-                                                 uint32_t H, Counters *cnt)
-{
-  using V = typename Op::V;
-  constexpr int RPW = kWave / G;  // rows per wave
-  const int sub = lane_id() % G;
-  const uint32_t nwaves = gridDim.x * kWavesPerBlock;
-  uint64_t my_edges = 0;
-  for (uint32_t wv = blockIdx.x * kWavesPerBlock + wave_in_block(); (uint64_t)wv * RPW < P; wv += nwaves) {
-    uint32_t r = wv * RPW + lane_id() / G;
-    uint32_t b = 0, e = 0;
-    if (r < P) {
-      b = crp[r];
-      e = crp[r + 1];
-      if (e - b >= H) e = b;  // heavy rows: k_pull_heavy
-    }
-    V acc = identity<Op>();
-    bool any = false;
-    for (uint32_t x = b + sub; x < e; x += G) {
-      uint32_t src = ccol[x];
-      if ((Dp[src >> 6] >> (src & 63)) & 1ull) {
-        acc = combine<Op>(acc, Op::edge(mprop[src], cw ? cw[x] : 1u));
-        any = true;
-      }
-    }
-    if (sub == 0) my_edges += e - b;
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) {
-      V t = __shfl_xor(acc, o, kWave);
-      int ta = __shfl_xor((int)any, o, kWave);
-      acc = combine<Op>(acc, t);
-      any = any || ta;
-    }
-    bool mark = false;
-    if (sub == 0 && any && r < P) {
-      if constexpr (Op::kMin) {
-        if (acc < upd[r]) {
-          upd[r] = acc;
-          mark = true;
-        }
-      } else {
-        upd[r] = upd[r] + acc;
-        mark = true;
-      }
-    }
-    uint64_t mb = __ballot(mark);
-    if (lane_id() == 0 && mb) {
-      // compress the group-leader bits (every G-th lane) into row bits
-      uint64_t rows = 0;
-      for (int i = 0; i < RPW; i++)
-        if ((mb >> (i * G)) & 1ull) rows |= 1ull << i;
-      uint32_t r0 = wv * RPW;
-      atomicOr((unsigned long long *)(dirty + (r0 >> 6)), (unsigned long long)(rows << (r0 & 63)));
-    }
-  }
-  my_edges = block_sum(my_edges);
-  if (threadIdx.x == 0 && my_edges) atomicAdd(&cnt->edges, (unsigned long long)my_edges);
-}
-
-template <class Op>
-__global__ void __launch_bounds__(kBlock) k_pull_heavy(const uint32_t *__restrict__ rows, uint32_t nrows,
-                                                       const uint32_t *__restrict__ crp, const uint32_t *__restrict__ ccol,
-                                                       const uint32_t *__restrict__ cw, const uint64_t *__restrict__ Dp,
-                                                       const typename Op::V *__restrict__ mprop,
-                                                       typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty,
-                                                       Counters *cnt)
-{
-  using V = typename Op::V;
-  for (uint32_t i = blockIdx.x; i < nrows; i += gridDim.x) {
-    uint32_t r = rows[i];
-    uint32_t b = crp[r], e = crp[r + 1];
-    V acc = identity<Op>();
-    uint32_t any = 0;
-    for (uint32_t x = b + threadIdx.x; x < e; x += kBlock) {
-      uint32_t src = ccol[x];
-      if ((Dp[src >> 6] >> (src & 63)) & 1ull) {
-        acc = combine<Op>(acc, Op::edge(mprop[src], cw ? cw[x] : 1u));
-        any = 1;
-      }
-    }
-    if constexpr (Op::kMin) acc = block_min(acc);
-    else acc = block_sum(acc);
-    any = block_max(any);
-    if (threadIdx.x == 0) {
-      if (any) {
-        bool mark = false;
-        if constexpr (Op::kMin) {
-          if (acc < upd[r]) {
-            upd[r] = acc;
-            mark = true;
-          }
-        } else {
-          upd[r] = upd[r] + acc;
-          mark = true;
-        }
-        if (mark) atomicOr((unsigned long long *)(dirty + (r >> 6)), 1ull << (r & 63));
-      }
-      atomicAdd(&cnt->edges, (unsigned long long)(e - b));
-    }
   }
 }
 
@@ -651,6 +468,7 @@ __global__ void k_zero_counters(Counters *cnt)
     cnt->pending = 0;
     cnt->improved = 0;
     cnt->heavy = 0;
+    cnt->heavy_edges = 0;
     cnt->edges = 0;
     cnt->frontier_maxdeg = 0;
     cnt->pend_min = 0xFFFFFFFFu;
@@ -721,6 +539,7 @@ grd_status state_alloc(grd_state *s)
   GRD_TRY(m.alloc_n(&s->totals, 64));
   GRD_TRY(m.alloc_n(&s->AL, S));
   GRD_TRY(m.alloc_n(&s->heavy, S));
+  GRD_TRY(m.alloc(&s->partial, (uint64_t)std::max<uint32_t>(1, g->n_pblk) * 8));
   GRD_TRY(m.alloc_n(&s->cnt, 1));
   GRD_HIP(hipHostMalloc((void **)&s->hcnt, sizeof(Counters), hipHostMallocDefault));
   memset(s->hcnt, 0, sizeof(Counters));
@@ -841,23 +660,28 @@ static grd_status traverse_t(grd_state *s, int option)
   using V = typename Op::V;
   const uint32_t *wt = std::is_same<Op, OpWeighted>::value ? g->w : nullptr;
   const uint32_t *cwt = std::is_same<Op, OpWeighted>::value ? g->cw : nullptr;
+  const bool timed = s->prm.time_kernel != 0;
   if (option & 1) {
-    const uint32_t H = g->heavy_in_threshold;
-    const uint32_t rows_per_block = kBlock / 8;
-    uint32_t want = (g->P + rows_per_block - 1) / rows_per_block;
-    uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(want, ctx->num_cus * 16));
-    hipLaunchKernelGGL((k_pull<Op, 8>), dim3(grid), dim3(kBlock), 0, st, g->P, g->crp, g->ccol, cwt, s->Dp,
-                       (const V *)s->mprop, (V *)s->mupd, s->Du, H, s->cnt);
-    if (g->n_heavy_in)
-      hipLaunchKernelGGL((k_pull_heavy<Op>), dim3(std::min<uint32_t>(g->n_heavy_in, ctx->num_cus * 8)), dim3(kBlock), 0,
-                         st, g->heavy_in, g->n_heavy_in, g->crp, g->ccol, cwt, s->Dp, (const V *)s->mprop,
-                         (V *)s->mupd, s->Du, s->cnt);
+    if (timed) GRD_HIP(hipEventRecord(s->kev[0], st));
+    GRD_TRY(bsp_pull_launch<Op>(s, cwt));
+    if (timed) {
+      GRD_HIP(hipEventRecord(s->kev[1], st));
+      s->kern_pending = 1;
+      s->kern_pending_bytes = 8 * g->m + 8ull * g->P;
+    }
+    GRD_TRY(bsp_pull_fixup<Op>(s));
+    s->edges += g->m;  // a pull examines every arc of the memory partitions
   } else {
     const uint32_t H = 4096;
     uint32_t want = (uint32_t)std::min<uint64_t>((s->last_frontier + kBlock - 1) / kBlock, (uint64_t)ctx->num_cus * 8);
     uint32_t grid = std::max<uint32_t>(1, want);
+    if (timed) GRD_HIP(hipEventRecord(s->kev[0], st));
     hipLaunchKernelGGL((k_push<Op>), dim3(grid), dim3(kBlock), 0, st, s->AL, s->totals + 0, g->rp, g->col, wt,
                        (const V *)s->mprop, (V *)s->mupd, s->Du, s->heavy, s->cnt, H);
+    if (timed) {
+      GRD_HIP(hipEventRecord(s->kev[1], st));
+      s->kern_pending = 2;  // bytes from the edge counter after the iteration
+    }
     hipLaunchKernelGGL((k_push_heavy<Op>), dim3(ctx->num_cus * 4), dim3(kBlock), 0, st, s->heavy, s->cnt, g->rp, g->col,
                        wt, (const V *)s->mprop, (V *)s->mupd, s->Du);
   }

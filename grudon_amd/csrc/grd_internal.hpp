@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -73,8 +74,20 @@ struct Counters {
   unsigned int pend_min;
   double residual;                    // PR pull: L1 change
   double dangling[2];                 // PR pull: FP64 dangling mass (ping-pong)
-  unsigned long long pad[4];
+  unsigned long long heavy_edges;     // push: edges of heavy vertices (k_push_heavy)
+  unsigned long long pad[3];
 };
+
+// pull row block: rows [r0, r0 + nrows) with arcs [e0, e1); nrows == 0 marks
+// one segment [e0, e1) of the long row r0
+struct PullBlock {
+  uint32_t r0, nrows, e0, e1;
+};
+struct PullLong {
+  uint32_t row, first, nseg, pad;
+};
+constexpr uint32_t kPullNnz = 2048;     // arcs per block (8 per thread)
+constexpr uint32_t kPullMaxRows = 1024;  // rows per block
 
 }  // namespace grd
 
@@ -117,10 +130,12 @@ struct grd_graph {
   uint32_t *rp = nullptr, *col = nullptr, *w = nullptr;     // push CSR over proxies
   uint32_t *crp = nullptr, *ccol = nullptr, *cw = nullptr;  // pull CSC over proxies
   uint32_t *xptr = nullptr, *xlist = nullptr;               // [S+1], [X] cross proxies per slot
-  uint32_t *heavy_in = nullptr;  // pull rows with in-degree >= heavy_in_threshold
-  uint32_t n_heavy_in = 0;
-  uint32_t heavy_in_threshold = 0;
-  std::vector<uint32_t> h_in_deg;  // per proxy (host), to rebuild the heavy list
+  // pull row blocks (pull.hip): at most kPullNnz arcs per block, longer rows
+  // split into segments merged by a fix-up pass
+  grd::PullBlock *pblk = nullptr;
+  uint32_t n_pblk = 0;
+  grd::PullLong *plong = nullptr;
+  uint32_t n_plong = 0;
 };
 
 struct grd_state {
@@ -134,6 +149,12 @@ struct grd_state {
   uint64_t last_frontier = 0, prev_frontier = 0, last_frontier_edges = 0, last_dirty = 0;
   uint32_t last_maxdeg = 0;
   uint64_t bytes = 0, edges = 0;
+  // dominant-kernel timing (grd_params.time_kernel)
+  hipEvent_t kev[2] = {nullptr, nullptr};
+  int kern_pending = 0;   // 1 = pull-type launch recorded, 2 = push launch recorded
+  uint64_t kern_pending_bytes = 0;
+  double kern_ms = 0.0;
+  uint64_t kern_launches = 0, kern_bytes = 0;
   uint32_t vbytes = 4;  // sizeof(VP)
   grd::DevMem mem;
   // compute side [S]
@@ -149,6 +170,7 @@ struct grd_state {
   // scan scratch
   uint32_t *blk = nullptr, *wrankB = nullptr, *wrankR = nullptr, *totals = nullptr;
   uint32_t *AL = nullptr, *heavy = nullptr;
+  void *partial = nullptr;  // pull: one 8-byte partial per row block (long-row segments)
   grd::Counters *cnt = nullptr;   // device
   grd::Counters *hcnt = nullptr;  // pinned host
   // snapshot copies
@@ -163,7 +185,7 @@ namespace grd {
 // layout.cpp
 grd_status build_graph(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const uint32_t *col, const uint32_t *w,
                        int C, int M, const uint32_t *mirror, grd_graph **out);
-grd_status rebuild_heavy_in(grd_graph *g, uint32_t threshold);
+
 void parallel_for(int nthreads, uint64_t n, uint64_t grain, const std::function<void(uint64_t, uint64_t)> &fn);
 int host_threads(const grd_ctx *ctx);
 // gen.hip
@@ -188,6 +210,17 @@ grd_status op_read_counters(grd_state *s);
 // prpull.hip
 grd_status prp_init(grd_state *s);
 grd_status prp_iteration(grd_state *s);
+grd_status prp_broadcast(grd_state *s);
+grd_status prp_traverse(grd_state *s);
+grd_status prp_update(grd_state *s);
+// graph_io.cpp
+grd_status read_graph(const std::string &path, uint32_t &n, std::vector<uint32_t> &rp, std::vector<uint32_t> &col,
+                      std::vector<uint32_t> &w, bool &has_w);
+grd_status read_partition_file(const std::string &path, uint32_t n, int M, std::vector<uint32_t> &mirror);
+grd_status write_gr(const std::string &path, uint32_t n, const uint32_t *rp, const uint32_t *col, const uint32_t *w);
+// api.cpp
+int heuristic_option(const grd_state *s);
+void make_features(const grd_state *s, float *f);
 // mlp.hip
 grd_status mlp_select(grd_ctx *ctx, const float *feat16_host, int *option, float *margin);
 }  // namespace grd

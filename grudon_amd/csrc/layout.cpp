@@ -87,21 +87,38 @@ static grd_status upload(grd_graph *g, T **dst, const std::vector<T> &src)
   return GRD_OK;
 }
 
-grd_status rebuild_heavy_in(grd_graph *g, uint32_t threshold)
+// Cut the CSC rows into pull row blocks: consecutive rows while the block
+// holds at most kPullNnz arcs and kPullMaxRows rows; a row with more than
+// kPullNnz arcs becomes segments of kPullNnz arcs (one block each) plus a
+// PullLong entry that merges them in order.
+static grd_status build_row_blocks(grd_graph *g, const std::vector<uint32_t> &crp)
 {
-  std::vector<uint32_t> rows;
-  for (uint32_t p = 0; p < g->P; p++)
-    if (g->h_in_deg[p] >= threshold) rows.push_back(p);
-  if (g->heavy_in) {
-    auto it = std::find(g->mem.ptrs.begin(), g->mem.ptrs.end(), (void *)g->heavy_in);
-    if (it != g->mem.ptrs.end()) g->mem.ptrs.erase(it);
-    (void)hipFree(g->heavy_in);
-    g->heavy_in = nullptr;
+  std::vector<PullBlock> blks;
+  std::vector<PullLong> longs;
+  const uint32_t P = g->P;
+  uint32_t r = 0;
+  while (r < P) {
+    uint32_t deg = crp[r + 1] - crp[r];
+    if (deg > kPullNnz) {
+      PullLong L{r, (uint32_t)blks.size(), 0, 0};
+      for (uint32_t e = crp[r]; e < crp[r + 1]; e += kPullNnz) {
+        blks.push_back(PullBlock{r, 0, e, std::min(crp[r + 1], e + kPullNnz)});
+        L.nseg++;
+      }
+      longs.push_back(L);
+      r++;
+      continue;
+    }
+    const uint32_t r0 = r, e0 = crp[r];
+    while (r < P && r - r0 < kPullMaxRows && crp[r + 1] - crp[r] <= kPullNnz && crp[r + 1] - e0 <= kPullNnz) r++;
+    blks.push_back(PullBlock{r0, r - r0, e0, crp[r]});
   }
-  g->heavy_in_threshold = threshold;
-  g->n_heavy_in = (uint32_t)rows.size();
-  if (rows.empty()) rows.push_back(kNone);
-  return upload(g, &g->heavy_in, rows);
+  if (blks.size() >= 0x7FFFFFFFull) return fail(GRD_ERR_ARG, "graph: too many pull row blocks");
+  g->n_pblk = (uint32_t)blks.size();
+  g->n_plong = (uint32_t)longs.size();
+  if (longs.empty()) longs.push_back(PullLong{0, 0, 0, 0});
+  GRD_TRY(upload(g, &g->pblk, blks));
+  return upload(g, &g->plong, longs);
 }
 
 static grd_status build_impl(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const uint32_t *col, const uint32_t *w,
@@ -275,13 +292,12 @@ static grd_status build_impl(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const
   });
   std::vector<uint32_t>().swap(xp);
 
-  // pull CSC over proxies: count, prefix, fill, sort rows by (src, weight)
+  // pull CSC over proxies: count, prefix, fill (rows sorted by source)
   std::vector<uint32_t> crp((uint64_t)g->P + 1, 0);
   {
     std::vector<uint32_t> indeg(g->P, 0);
     for (uint64_t x = 0; x < m; x++) indeg[pcol[x]]++;
     for (uint32_t p = 0; p < g->P; p++) crp[p + 1] = crp[p] + indeg[p];
-    g->h_in_deg.swap(indeg);
   }
   std::vector<uint32_t> ccol(m), cwv(w ? m : 0);
   {
@@ -332,7 +348,7 @@ static grd_status build_impl(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const
       return st;
     }
   }
-  if ((st = rebuild_heavy_in(g, 2048)) != GRD_OK) {
+  if ((st = build_row_blocks(g, crp)) != GRD_OK) {
     delete g;
     return st;
   }

@@ -17,6 +17,9 @@
 
 namespace grd {
 
+grd_status prp_pull_launch(grd_state *s);
+grd_status prp_pull_fixup(grd_state *s);
+
 // dense receive: memory own proxies <- compute contributions
 __global__ void __launch_bounds__(kBlock) k_prp_recv(uint32_t nw, const uint32_t *__restrict__ word_dst,
                                                      const float *__restrict__ valB, float *__restrict__ mprop)
@@ -25,62 +28,6 @@ __global__ void __launch_bounds__(kBlock) k_prp_recv(uint32_t nw, const uint32_t
     uint32_t pw = word_dst[w];
     if (pw == 0xFFFFFFFFu) continue;
     mprop[pw * 64 + lane_id()] = valB[w * 64 + lane_id()];
-  }
-}
-
-// G lanes per CSC row, Kahan per lane, compensated merge across the group
-template <int G>
-__global__ void __launch_bounds__(kBlock) k_prp_pull(uint32_t P, const uint32_t *__restrict__ crp,
-                                                     const uint32_t *__restrict__ ccol, const float *__restrict__ mprop,
-                                                     float *__restrict__ out, uint32_t H)
-{
-  constexpr int RPW = kWave / G;
-  const int sub = lane_id() % G;
-  const uint32_t nwaves = gridDim.x * kWavesPerBlock;
-  for (uint32_t wv = blockIdx.x * kWavesPerBlock + wave_in_block(); (uint64_t)wv * RPW < P; wv += nwaves) {
-    uint32_t r = wv * RPW + lane_id() / G;
-    uint32_t b = 0, e = 0;
-    bool heavy = false;
-    if (r < P) {
-      b = crp[r];
-      e = crp[r + 1];
-      if (e - b >= H) {
-        heavy = true;
-        e = b;
-      }
-    }
-    Kahan k;
-    for (uint32_t x = b + sub; x < e; x += G) k.add(mprop[ccol[x]]);
-    kahan_wave_merge(k, G);
-    if (sub == 0 && r < P && !heavy) out[r] = k.value();
-  }
-}
-
-__global__ void __launch_bounds__(kBlock) k_prp_pull_heavy(const uint32_t *__restrict__ rows, uint32_t nrows,
-                                                           const uint32_t *__restrict__ crp,
-                                                           const uint32_t *__restrict__ ccol,
-                                                           const float *__restrict__ mprop, float *__restrict__ out)
-{
-  __shared__ float s_s[kWavesPerBlock], s_c[kWavesPerBlock];
-  for (uint32_t i = blockIdx.x; i < nrows; i += gridDim.x) {
-    uint32_t r = rows[i];
-    uint32_t b = crp[r], e = crp[r + 1];
-    Kahan k;
-    for (uint32_t x = b + threadIdx.x; x < e; x += kBlock) k.add(mprop[ccol[x]]);
-    kahan_wave_merge(k, kWave);
-    if (lane_id() == 0) {
-      s_s[wave_in_block()] = k.s;
-      s_c[wave_in_block()] = k.c;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      Kahan t;
-      t.s = s_s[0];
-      t.c = s_c[0];
-      for (int j = 1; j < kWavesPerBlock; j++) t.merge(s_s[j], s_c[j]);
-      out[r] = t.value();
-    }
-    __syncthreads();
   }
 }
 
@@ -163,32 +110,63 @@ grd_status prp_init(grd_state *s)
   return GRD_OK;
 }
 
-grd_status prp_iteration(grd_state *s)
+static uint32_t prp_grid(const grd_graph *g)
+{
+  const uint32_t nwS = g->S / 64;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((nwS + 3) / 4, (uint64_t)g->ctx->num_cus * 16));
+}
+
+// broadcast step: dense put of r/deg into the memory partitions' own proxies
+grd_status prp_broadcast(grd_state *s)
 {
   grd_graph *g = s->g;
-  grd_ctx *ctx = g->ctx;
-  hipStream_t st = ctx->stream;
-  const uint32_t nwS = g->S / 64;
+  hipStream_t st = g->ctx->stream;
   const int parity = (int)(s->iteration & 1);
   hipLaunchKernelGGL(k_prp_counters, dim3(1), dim3(64), 0, st, s->cnt, parity, -1.0);
-  uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((nwS + 3) / 4, (uint64_t)ctx->num_cus * 16));
-  hipLaunchKernelGGL(k_prp_recv, dim3(grid), dim3(kBlock), 0, st, nwS, g->word_dst, (const float *)s->valB,
-                     (float *)s->mprop);
-  const uint32_t rows_per_block = kBlock / 8;
-  uint32_t pgrid = std::max<uint32_t>(1, std::min<uint32_t>((g->P + rows_per_block - 1) / rows_per_block,
-                                                            ctx->num_cus * 16));
-  hipLaunchKernelGGL(k_prp_pull<8>, dim3(pgrid), dim3(kBlock), 0, st, g->P, g->crp, g->ccol, (const float *)s->mprop,
-                     (float *)s->valR, g->heavy_in_threshold);
-  if (g->n_heavy_in)
-    hipLaunchKernelGGL(k_prp_pull_heavy, dim3(std::min<uint32_t>(g->n_heavy_in, ctx->num_cus * 8)), dim3(kBlock), 0, st,
-                       g->heavy_in, g->n_heavy_in, g->crp, g->ccol, (const float *)s->mprop, (float *)s->valR);
-  hipLaunchKernelGGL(k_prp_update, dim3(grid), dim3(kBlock), 0, st, nwS, g->slot_gid, g->slot_deg, g->word_dst,
-                     g->xptr, g->xlist, (const float *)s->valR, s->pr, (float *)s->valB, (float)s->prm.damping,
-                     1.0 / (double)g->n, parity, s->cnt);
+  hipLaunchKernelGGL(k_prp_recv, dim3(prp_grid(g)), dim3(kBlock), 0, st, g->S / 64, g->word_dst,
+                     (const float *)s->valB, (float *)s->mprop);
   GRD_HIP(hipGetLastError());
-  s->bytes += (uint64_t)g->n * 4 + ((uint64_t)g->n + g->X) * 4;
-  s->edges += g->m;
+  s->bytes += (uint64_t)g->n * 4;
   return GRD_OK;
+}
+
+// traverse step: the pull writes each proxy's partial sum straight into the
+// dense reduce buffer (the memory partition's outgoing message)
+grd_status prp_traverse(grd_state *s)
+{
+  grd_graph *g = s->g;
+  hipStream_t st = g->ctx->stream;
+  const bool timed = s->prm.time_kernel != 0;
+  if (timed) GRD_HIP(hipEventRecord(s->kev[0], st));
+  GRD_TRY(prp_pull_launch(s));
+  if (timed) {
+    GRD_HIP(hipEventRecord(s->kev[1], st));
+    s->kern_pending = 1;
+    s->kern_pending_bytes = 8 * g->m + 8ull * g->P;
+  }
+  GRD_TRY(prp_pull_fixup(s));
+  s->edges += g->m;
+  s->bytes += ((uint64_t)g->n + g->X) * 4;
+  return GRD_OK;
+}
+
+// update step: aggregate the proxies of every vertex, new ranks, residual
+grd_status prp_update(grd_state *s)
+{
+  grd_graph *g = s->g;
+  const int parity = (int)(s->iteration & 1);
+  hipLaunchKernelGGL(k_prp_update, dim3(prp_grid(g)), dim3(kBlock), 0, g->ctx->stream, g->S / 64, g->slot_gid,
+                     g->slot_deg, g->word_dst, g->xptr, g->xlist, (const float *)s->valR, s->pr, (float *)s->valB,
+                     (float)s->prm.damping, 1.0 / (double)g->n, parity, s->cnt);
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+
+grd_status prp_iteration(grd_state *s)
+{
+  GRD_TRY(prp_broadcast(s));
+  GRD_TRY(prp_traverse(s));
+  return prp_update(s);
 }
 
 }  // namespace grd

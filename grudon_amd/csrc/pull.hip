@@ -1,0 +1,251 @@
+// pull.hip -- edge-balanced pull over the memory partitions' CSC.
+//
+// The CSC rows (one per memory proxy: own vertices and cross mirrors) are cut
+// at graph-build time into row blocks holding at most kPullNnz arcs
+// (adaptive CSR row blocks).  A 256-thread workgroup takes one block:
+//   1. stage: every thread walks the block's arcs with stride 256 (coalesced
+//      column reads), gathers the source value and stores the per-arc
+//      contribution in LDS;
+//   2. reduce: the block's rows are assigned to groups of G lanes, G being the
+//      largest power of two with G * rows <= 256, and each group reduces its
+//      rows from LDS.
+// A row with more than kPullNnz arcs is split into segments of kPullNnz arcs,
+// one workgroup each; segment partials are merged in segment order by a
+// fix-up pass, so results do not depend on scheduling.
+//
+// PageRank pull sums in FP32 with Kahan compensation (per lane, merged with
+// TwoSum); the BSP pull (CC / SSSP / reference PageRank) combines the
+// contributions of the sources active this iteration with min or +.
+#include <algorithm>
+
+#include "grd_device.hpp"
+#include "grd_internal.hpp"
+#include "grd_ops.hpp"
+
+namespace grd {
+
+// per-block LDS bitmap of dirty rows (rows of one block span <= 17 words)
+constexpr int kDirtyWords = kPullMaxRows / 64 + 2;
+
+__device__ __forceinline__ int group_size(uint32_t nrows)
+{
+  int g = 64;
+  while (g > 1 && (uint32_t)g * nrows > (uint32_t)kBlock) g >>= 1;
+  return g;
+}
+
+// ---- PageRank pull: Kahan FP32 ----
+
+__global__ void __launch_bounds__(kBlock) k_prp_pull_rb(const PullBlock *__restrict__ blocks,
+                                                        const uint32_t *__restrict__ crp,
+                                                        const uint32_t *__restrict__ ccol,
+                                                        const float *__restrict__ mprop, float *__restrict__ out,
+                                                        float2 *__restrict__ partial)
+{
+  __shared__ float s_v[kPullNnz];
+  __shared__ float s_ws[kWavesPerBlock], s_wc[kWavesPerBlock];
+  const PullBlock d = blocks[blockIdx.x];
+  if (d.nrows == 0) {  // segment of a long row
+    Kahan k;
+    for (uint32_t e = d.e0 + threadIdx.x; e < d.e1; e += kBlock) k.add(mprop[ccol[e]]);
+    kahan_wave_merge(k, kWave);
+    if (lane_id() == 0) {
+      s_ws[wave_in_block()] = k.s;
+      s_wc[wave_in_block()] = k.c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      Kahan t;
+      t.s = s_ws[0];
+      t.c = s_wc[0];
+      for (int j = 1; j < kWavesPerBlock; j++) t.merge(s_ws[j], s_wc[j]);
+      partial[blockIdx.x] = make_float2(t.s, t.c);
+    }
+    return;
+  }
+  const uint32_t ne = d.e1 - d.e0;
+  for (uint32_t i = threadIdx.x; i < ne; i += kBlock) s_v[i] = mprop[ccol[d.e0 + i]];
+  __syncthreads();
+  const int G = group_size(d.nrows);
+  const int sub = threadIdx.x % G;
+  const uint32_t ngroups = kBlock / G;
+  // every group runs the same number of rounds so that the group shuffles
+  // stay in step
+  const uint32_t rounds = (d.nrows + ngroups - 1) / ngroups;
+  for (uint32_t k = 0; k < rounds; k++) {
+    uint32_t rl = k * ngroups + threadIdx.x / G;
+    Kahan acc;
+    uint32_t r = d.r0 + rl;
+    if (rl < d.nrows) {
+      uint32_t b = crp[r] - d.e0, e = crp[r + 1] - d.e0;
+      for (uint32_t x = b + sub; x < e; x += G) acc.add(s_v[x]);
+    }
+    kahan_wave_merge(acc, G);
+    if (sub == 0 && rl < d.nrows) out[r] = acc.value();
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_prp_pull_fixup(const PullLong *__restrict__ longs, uint32_t nlong,
+                                                           const float2 *__restrict__ partial, float *__restrict__ out)
+{
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nlong; i += gridDim.x * kBlock) {
+    PullLong L = longs[i];
+    Kahan t;
+    for (uint32_t j = 0; j < L.nseg; j++) {
+      float2 p = partial[L.first + j];
+      t.merge(p.x, p.y);
+    }
+    out[L.row] = t.value();
+  }
+}
+
+// ---- BSP pull: min / + over the active sources ----
+
+template <class Op>
+__device__ __forceinline__ typename Op::V pull_val(uint32_t e, const uint32_t *__restrict__ ccol,
+                                                   const uint32_t *__restrict__ cw, const uint64_t *__restrict__ Dp,
+                                                   const typename Op::V *__restrict__ mprop)
+{
+  uint32_t src = ccol[e];
+  if ((Dp[src >> 6] >> (src & 63)) & 1ull) return Op::edge(mprop[src], cw ? cw[e] : 1u);
+  return op_identity<Op>();
+}
+
+// apply the pulled value to the proxy's running update; true = dirty
+template <class Op>
+__device__ __forceinline__ bool pull_apply(typename Op::V *upd, uint32_t r, typename Op::V acc)
+{
+  if constexpr (Op::kMin) {
+    if (acc < upd[r]) {
+      upd[r] = acc;
+      return true;
+    }
+    return false;
+  } else {
+    // contributions of active sources are strictly positive (pr.cpp:80)
+    if (acc > 0.0f) {
+      upd[r] = upd[r] + acc;
+      return true;
+    }
+    return false;
+  }
+}
+
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restrict__ blocks,
+                                                        const uint32_t *__restrict__ crp,
+                                                        const uint32_t *__restrict__ ccol,
+                                                        const uint32_t *__restrict__ cw, const uint64_t *__restrict__ Dp,
+                                                        const typename Op::V *__restrict__ mprop,
+                                                        typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty,
+                                                        typename Op::V *__restrict__ partial)
+{
+  using V = typename Op::V;
+  __shared__ V s_v[kPullNnz];
+  __shared__ unsigned long long s_dirty[kDirtyWords];
+  const PullBlock d = blocks[blockIdx.x];
+  if (d.nrows == 0) {
+    V acc = op_identity<Op>();
+    for (uint32_t e = d.e0 + threadIdx.x; e < d.e1; e += kBlock)
+      acc = op_combine<Op>(acc, pull_val<Op>(e, ccol, cw, Dp, mprop));
+    if constexpr (Op::kMin) acc = block_min(acc);
+    else acc = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+    return;
+  }
+  for (int i = threadIdx.x; i < kDirtyWords; i += kBlock) s_dirty[i] = 0ull;
+  const uint32_t ne = d.e1 - d.e0;
+  for (uint32_t i = threadIdx.x; i < ne; i += kBlock) s_v[i] = pull_val<Op>(d.e0 + i, ccol, cw, Dp, mprop);
+  __syncthreads();
+  const int G = group_size(d.nrows);
+  const int sub = threadIdx.x % G;
+  const uint32_t ngroups = kBlock / G;
+  const uint32_t rounds = (d.nrows + ngroups - 1) / ngroups;
+  const uint32_t w0 = d.r0 >> 6;
+  for (uint32_t k = 0; k < rounds; k++) {
+    uint32_t rl = k * ngroups + threadIdx.x / G;
+    V acc = op_identity<Op>();
+    uint32_t r = d.r0 + rl;
+    if (rl < d.nrows) {
+      uint32_t b = crp[r] - d.e0, e = crp[r + 1] - d.e0;
+      for (uint32_t x = b + sub; x < e; x += G) acc = op_combine<Op>(acc, s_v[x]);
+    }
+    for (int o = G / 2; o > 0; o >>= 1) acc = op_combine<Op>(acc, __shfl_xor(acc, o, kWave));
+    if (sub == 0 && rl < d.nrows && pull_apply<Op>(upd, r, acc))
+      atomicOr(&s_dirty[(r >> 6) - w0], 1ull << (r & 63));
+  }
+  __syncthreads();
+  const uint32_t nw = ((d.r0 + d.nrows - 1) >> 6) - w0 + 1;
+  for (uint32_t i = threadIdx.x; i < nw; i += kBlock)
+    if (s_dirty[i]) atomicOr((unsigned long long *)(dirty + w0 + i), s_dirty[i]);
+}
+
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_bsp_pull_fixup(const PullLong *__restrict__ longs, uint32_t nlong,
+                                                           const typename Op::V *__restrict__ partial,
+                                                           typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty)
+{
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nlong; i += gridDim.x * kBlock) {
+    PullLong L = longs[i];
+    typename Op::V acc = op_identity<Op>();
+    for (uint32_t j = 0; j < L.nseg; j++) acc = op_combine<Op>(acc, partial[L.first + j]);
+    if (pull_apply<Op>(upd, L.row, acc))
+      atomicOr((unsigned long long *)(dirty + (L.row >> 6)), 1ull << (L.row & 63));
+  }
+}
+
+// ------------------------------------------------------------------------
+
+grd_status prp_pull_launch(grd_state *s)
+{
+  grd_graph *g = s->g;
+  hipStream_t st = g->ctx->stream;
+  hipLaunchKernelGGL(k_prp_pull_rb, dim3(g->n_pblk), dim3(kBlock), 0, st, g->pblk, g->crp, g->ccol,
+                     (const float *)s->mprop, (float *)s->valR, (float2 *)s->partial);
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+
+grd_status prp_pull_fixup(grd_state *s)
+{
+  grd_graph *g = s->g;
+  if (!g->n_plong) return GRD_OK;
+  hipLaunchKernelGGL(k_prp_pull_fixup, dim3((g->n_plong + kBlock - 1) / kBlock), dim3(kBlock), 0, g->ctx->stream,
+                     g->plong, g->n_plong, (const float2 *)s->partial, (float *)s->valR);
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+
+template <class Op>
+grd_status bsp_pull_launch(grd_state *s, const uint32_t *cw)
+{
+  grd_graph *g = s->g;
+  using V = typename Op::V;
+  hipLaunchKernelGGL(k_bsp_pull_rb<Op>, dim3(g->n_pblk), dim3(kBlock), 0, g->ctx->stream, g->pblk, g->crp, g->ccol, cw,
+                     s->Dp, (const V *)s->mprop, (V *)s->mupd, s->Du, (V *)s->partial);
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+
+template <class Op>
+grd_status bsp_pull_fixup(grd_state *s)
+{
+  grd_graph *g = s->g;
+  using V = typename Op::V;
+  if (!g->n_plong) return GRD_OK;
+  hipLaunchKernelGGL(k_bsp_pull_fixup<Op>, dim3((g->n_plong + kBlock - 1) / kBlock), dim3(kBlock), 0, g->ctx->stream,
+                     g->plong, g->n_plong, (const V *)s->partial, (V *)s->mupd, s->Du);
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+
+template grd_status bsp_pull_launch<OpCC>(grd_state *, const uint32_t *);
+template grd_status bsp_pull_launch<OpHop>(grd_state *, const uint32_t *);
+template grd_status bsp_pull_launch<OpWeighted>(grd_state *, const uint32_t *);
+template grd_status bsp_pull_launch<OpPR>(grd_state *, const uint32_t *);
+template grd_status bsp_pull_fixup<OpCC>(grd_state *);
+template grd_status bsp_pull_fixup<OpHop>(grd_state *);
+template grd_status bsp_pull_fixup<OpWeighted>(grd_state *);
+template grd_status bsp_pull_fixup<OpPR>(grd_state *);
+
+}  // namespace grd

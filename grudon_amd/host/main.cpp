@@ -1,0 +1,218 @@
+// main.cpp -- command line of the operator path (reference src/main.cpp:16-148).
+//
+// Same flags as the reference (-g -c -m -t -p -h) plus the -a flag its README
+// documents (pr | cc | sssp; prpull = normalised pull PageRank).  One process
+// owns all C compute and M memory partitions on one GPU, so there is no
+// mpirun and the node id is 0.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "Exchange.hpp"
+#include "GraphAlgorithm.hpp"
+#include "graph_algorithms/cc.hpp"
+#include "graph_algorithms/pr.hpp"
+#include "graph_algorithms/sssp.hpp"
+
+namespace {
+
+const char *kHelp =
+    "A Graph Processing Framework (MI355X, one process for every partition)\n"
+    "Usage:\n"
+    "  grudon [OPTION...]\n\n"
+    "  -g, --graph-path arg           Input graph file (.gr, .mtx) or generator spec\n"
+    "                                 rmat:SCALE[:EF[:SEED[:sym[:WSEED]]]] | er:N:DEG[:SEED[:WSEED]]\n"
+    "  -c, --num-compute arg          Number of compute nodes\n"
+    "  -m, --num-memory arg           Number of memory nodes\n"
+    "  -t, --threads arg              Number of host threads for graph construction (default: 1)\n"
+    "  -p, --partitioning-scheme arg  Partitioning scheme file (one memory partition per line)\n"
+    "  -a, --algorithm arg            pr | cc | sssp | prpull (default: pr)\n"
+    "  -h, --help                     Print usage\n"
+    "      --sched arg                plain | heuristic | learned | fixed:N (default: plain)\n"
+    "      --selector arg             BF16 selector weights file for --sched learned\n"
+    "      --source arg               SSSP source gid (default: 0)\n"
+    "      --weighted                 SSSP: use the edge weights\n"
+    "      --delta arg                SSSP: delta-stepping bucket width (default: 0 = off)\n"
+    "      --max-iter arg             iteration cap (default: 1000)\n"
+    "      --print-state              print every vertex value\n"
+    "      --verify                   run the algorithm's invariant check\n"
+    "      --write-gr arg             also write the input graph as a Galois .gr file\n"
+    "      --result arg               write the result by gid (raw 32-bit values)\n";
+
+struct Options {
+  std::string graph_path, pfile, algo = "pr", sched = "plain", selector, write_gr, result;
+  size_t num_compute = 0, num_memory = 0, threads = 1;
+  uint32_t source = 0, delta = 0, max_iter = 1000;
+  bool weighted = false, print_state = false, verify = false, help = false;
+  bool has_g = false, has_c = false, has_m = false;
+};
+
+Options parse(int argc, char **argv)
+{
+  Options o;
+  auto need = [&](int &i, const std::string &name) -> std::string {
+    if (i + 1 >= argc) throw std::runtime_error("Option '" + name + "' is missing an argument");
+    return argv[++i];
+  };
+  auto num = [](const std::string &s, const std::string &name) -> unsigned long long {
+    char *end = nullptr;
+    unsigned long long v = std::strtoull(s.c_str(), &end, 10);
+    if (s.empty() || *end) throw std::runtime_error("Argument '" + s + "' failed to parse for option " + name);
+    return v;
+  };
+  for (int i = 1; i < argc; i++) {
+    std::string a = argv[i];
+    if (a == "-g" || a == "--graph-path") o.graph_path = need(i, a), o.has_g = true;
+    else if (a == "-c" || a == "--num-compute") o.num_compute = num(need(i, a), a), o.has_c = true;
+    else if (a == "-m" || a == "--num-memory") o.num_memory = num(need(i, a), a), o.has_m = true;
+    else if (a == "-t" || a == "--threads") o.threads = num(need(i, a), a);
+    else if (a == "-p" || a == "--partitioning-scheme") o.pfile = need(i, a);
+    else if (a == "-a" || a == "--algorithm") o.algo = need(i, a);
+    else if (a == "-h" || a == "--help") o.help = true;
+    else if (a == "--sched") o.sched = need(i, a);
+    else if (a == "--selector") o.selector = need(i, a);
+    else if (a == "--source") o.source = (uint32_t)num(need(i, a), a);
+    else if (a == "--weighted") o.weighted = true;
+    else if (a == "--delta") o.delta = (uint32_t)num(need(i, a), a);
+    else if (a == "--max-iter") o.max_iter = (uint32_t)num(need(i, a), a);
+    else if (a == "--print-state") o.print_state = true;
+    else if (a == "--verify") o.verify = true;
+    else if (a == "--write-gr") o.write_gr = need(i, a);
+    else if (a == "--result") o.result = need(i, a);
+    else throw std::runtime_error("Option '" + a + "' does not exist");
+  }
+  return o;
+}
+
+template <typename VP>
+int run_algorithm(GraphAlgorithm<VP> &alg, const Options &o)
+{
+  alg.init();
+  alg.run();
+  if (o.print_state) alg.printState();
+  if (o.verify) alg.verify();
+  if (!o.result.empty()) {
+    std::vector<VP> r = alg.result();
+    std::ofstream out(o.result, std::ios::binary);
+    out.write(reinterpret_cast<const char *>(r.data()), (std::streamsize)(r.size() * sizeof(VP)));
+    if (!out) throw std::runtime_error("cannot write " + o.result);
+  }
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+  Options o;
+  try {
+    o = parse(argc, argv);
+    if (o.help) {
+      std::cout << kHelp << std::endl;
+      return 0;
+    }
+    if (!o.has_g) {
+      std::cerr << "Please provide a graph path" << std::endl;
+      return 1;
+    }
+    if (!o.has_c) {
+      std::cerr << "Please provide the number of master nodes" << std::endl;
+      return 1;
+    }
+    if (!o.has_m) {
+      std::cerr << "Please provide the number of mirror nodes" << std::endl;
+      return 1;
+    }
+  } catch (const std::exception &e) {
+    std::cerr << "Error parsing options: " << e.what() << std::endl;
+    return 1;
+  }
+  if (o.algo != "pr" && o.algo != "cc" && o.algo != "sssp" && o.algo != "prpull") {
+    std::cerr << "Error parsing options: unknown algorithm '" << o.algo << "' (pr | cc | sssp | prpull)" << std::endl;
+    return 1;
+  }
+  if (o.num_compute < 1 || o.num_memory < 1) {
+    std::cerr << "Error parsing options: -c and -m must be at least 1" << std::endl;
+    return 1;
+  }
+  grd_sched sched = GRD_SCHED_PLAIN;
+  int fixed = 0;
+  if (o.sched == "heuristic") sched = GRD_SCHED_HEURISTIC;
+  else if (o.sched == "learned") sched = GRD_SCHED_LEARNED;
+  else if (o.sched.rfind("fixed:", 0) == 0) {
+    sched = GRD_SCHED_FIXED;
+    fixed = std::atoi(o.sched.c_str() + 6);
+  } else if (o.sched != "plain") {
+    std::cerr << "Error parsing options: unknown schedule '" << o.sched << "'" << std::endl;
+    return 1;
+  }
+
+  try {
+    Exchange net(0);
+    grd_set_host_threads(net.context(), (int)o.threads);
+    std::printf("[%s] Graph Path: %s\n[%s] Number of Compute Nodes: %zu\n[%s] Number of Memory Nodes: %zu\n"
+                "[%s] Number of Threads: %zu\n", "info", o.graph_path.c_str(), "info", o.num_compute, "info",
+                o.num_memory, "info", o.threads);
+    if (!o.selector.empty()) {
+      std::ifstream in(o.selector, std::ios::binary);
+      std::vector<uint16_t> w((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+      if (!in.good() && !in.eof()) throw std::runtime_error("cannot read selector weights " + o.selector);
+      std::vector<uint16_t> ww(w.size() / 2);
+      std::memcpy(ww.data(), w.data(), ww.size() * 2);
+      if (grd_selector_load(net.context(), ww.data(), ww.size(), 0.0f) != GRD_OK)
+        throw std::runtime_error(std::string("selector: ") + grd_last_error());
+    }
+    const uint32_t node_id = 0;
+    std::string pfile = o.pfile;
+    if (o.algo == "cc") {
+      CC<uint32_t> alg(COMPUTE_NODE, "", o.graph_path, o.num_compute, o.num_memory, node_id, net, pfile);
+      alg.setSchedule(sched, fixed);
+      alg.setMaxIterations(o.max_iter);
+      if (!o.write_gr.empty()) {
+        const HostGraph &g = alg.graph().host();
+        if (grd_graph_write_gr(o.write_gr.c_str(), g.n, g.row_ptr.data(), g.col.data(),
+                               g.weights.empty() ? nullptr : g.weights.data()) != GRD_OK)
+          throw std::runtime_error(grd_last_error());
+      }
+      return run_algorithm(alg, o);
+    }
+    if (o.algo == "sssp") {
+      SSSP<uint32_t> alg(COMPUTE_NODE, "", o.graph_path, o.num_compute, o.num_memory, node_id, net, pfile);
+      alg.setSchedule(sched, fixed);
+      alg.setMaxIterations(o.max_iter);
+      alg.setSource(o.source);
+      alg.setWeighted(o.weighted);
+      alg.setDelta(o.delta);
+      if (!o.write_gr.empty()) {
+        const HostGraph &g = alg.graph().host();
+        if (grd_graph_write_gr(o.write_gr.c_str(), g.n, g.row_ptr.data(), g.col.data(),
+                               g.weights.empty() ? nullptr : g.weights.data()) != GRD_OK)
+          throw std::runtime_error(grd_last_error());
+      }
+      return run_algorithm(alg, o);
+    }
+    std::unique_ptr<PageRank<float>> alg;
+    if (o.algo == "prpull")
+      alg.reset(new PageRankPull<float>(COMPUTE_NODE, "", o.graph_path, o.num_compute, o.num_memory, node_id, net, pfile));
+    else
+      alg.reset(new PageRank<float>(COMPUTE_NODE, "", o.graph_path, o.num_compute, o.num_memory, node_id, net, pfile));
+    alg->setSchedule(sched, fixed);
+    alg->setMaxIterations(o.max_iter);
+    if (!o.write_gr.empty()) {
+      const HostGraph &g = alg->graph().host();
+      if (grd_graph_write_gr(o.write_gr.c_str(), g.n, g.row_ptr.data(), g.col.data(),
+                             g.weights.empty() ? nullptr : g.weights.data()) != GRD_OK)
+        throw std::runtime_error(grd_last_error());
+    }
+    return run_algorithm(*alg, o);
+  } catch (const std::exception &e) {
+    std::cerr << "Error: " << e.what() << std::endl;
+    return 2;
+  }
+}

@@ -65,7 +65,7 @@ typedef struct {
   uint32_t sched;        /* grd_sched */
   uint32_t fixed_option; /* option for GRD_SCHED_FIXED */
   uint32_t time_iters;   /* record per-iteration device time in the trace */
-  uint32_t reserved;
+  uint32_t time_kernel;  /* bracket the dominant traversal kernel with HIP events */
 } grd_params;
 
 /* One record per iteration (grd_run trace).  feat holds the selector's 16
@@ -90,6 +90,12 @@ typedef struct {
   double residual;           /* PR_PULL: last L1 change */
   uint64_t option_counts[4];
   uint64_t fallbacks;        /* learned mode: iterations decided by the heuristic */
+  /* time_kernel: the dominant traversal kernel (pull: k_pull / k_prp_pull,
+   * push: k_push), summed over launches, with its algorithmic HBM bytes:
+   * pull 8 B per edge + 8 B per row, push 12 B per edge + 16 B per vertex */
+  double kernel_ms;
+  uint64_t kernel_launches;
+  uint64_t kernel_bytes;
 } grd_run_stats;
 
 /* ---- context: replaces MPICore (include/MPI.hpp:11-24) ---- */
@@ -110,6 +116,11 @@ grd_status grd_generate_rmat(grd_ctx *ctx, int scale, int edge_factor, double a,
 grd_status grd_generate_er(grd_ctx *ctx, uint32_t n, double mean_degree, uint64_t seed, uint32_t *row_ptr,
                            uint32_t *col, uint64_t cap, uint64_t *m_out);
 
+/* Seeded edge weights uniform in [1,255] for a host CSR: a symmetric function
+ * of the endpoints {u, v}, hash(wseed, min << 32 | max). */
+grd_status grd_edge_weights(grd_ctx *ctx, uint32_t n, const uint32_t *row_ptr, const uint32_t *col, uint64_t wseed,
+                            uint32_t *w);
+
 /* Partition a global CSR into C compute and M memory partitions resident in
  * HBM (reference DistributedGraph ctor, src/DistributedGraph.cpp:13-785;
  * master = gid % C, mirror = gid % M or mirror[gid], Partitioner.hpp:11-205).
@@ -128,6 +139,15 @@ grd_status grd_graph_er(grd_ctx *ctx, uint32_t n, double mean_degree, uint64_t s
 grd_status grd_graph_load(grd_ctx *ctx, const char *path, const char *partition_file, int num_compute,
                           int num_memory, grd_graph **out);
 grd_status grd_graph_free(grd_graph *g);
+/* Read a .gr / .mtx file into caller buffers: call once with row_ptr NULL to
+ * get *n and *m, then again with row_ptr[n+1], col[m] (and weights[m], may be
+ * NULL) allocated.  Duplicates and self-loops are kept as in the file. */
+grd_status grd_graph_read(const char *path, uint32_t *n, uint64_t *m, uint32_t *row_ptr, uint32_t *col,
+                          uint32_t *weights, int *has_weights);
+/* Write a Galois .gr version 1 file (u32 edge data when weights != NULL);
+ * replaces the graph-convert step of scripts/convert_and_partition.sh. */
+grd_status grd_graph_write_gr(const char *path, uint32_t n, const uint32_t *row_ptr, const uint32_t *col,
+                              const uint32_t *weights);
 /* info[0] n, [1] m, [2] C, [3] M, [4] compute slots, [5] memory proxies,
  * [6] cross mirrors, [7] weighted, [8] device bytes held by the graph. */
 grd_status grd_graph_info(const grd_graph *g, uint64_t *info16);
@@ -161,6 +181,11 @@ grd_status grd_aggregate_one(grd_state *s, uint32_t gid, const void *value);
 /* Whole loop with the chosen schedule policy; trace (may be NULL) receives
  * min(iterations, trace_cap) records. */
 grd_status grd_run(grd_state *s, grd_run_stats *stats, grd_iter_rec *trace, uint32_t trace_cap);
+/* The schedule option the state's policy picks for the next superstep (the
+ * offload decision point of GraphAlgorithm.cpp:187-195). */
+grd_status grd_select_option(grd_state *s, int *option);
+/* Counters of the iterations run so far (step-wise use). */
+grd_status grd_state_stats(const grd_state *s, grd_run_stats *out);
 /* Result by gid: CC labels / SSSP distances (uint32), PR values (float). */
 grd_status grd_download(grd_state *s, void *out_by_gid);
 /* Selector features of the current state (16 floats). */

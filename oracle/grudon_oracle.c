@@ -688,3 +688,75 @@ int orc_pr_pull(uint32_t n, const uint32_t *rp, const uint32_t *col, double d, i
   free(nx);
   return it;
 }
+
+/* ---- multithreaded FP64 pull PageRank (same recurrence as orc_pr_pull) ---- */
+
+typedef struct {
+  uint32_t n;
+  const uint32_t *trp, *tcol, *deg;
+  const double *r;
+  double *contrib, *nx;
+  double base, d;
+} prmt_ctx;
+
+static void prmt_contrib(void *vc, uint64_t b, uint64_t e)
+{
+  prmt_ctx *c = (prmt_ctx *)vc;
+  for (uint64_t u = b; u < e; u++) c->contrib[u] = c->deg[u] ? c->r[u] / (double)c->deg[u] : 0.0;
+}
+
+static void prmt_pull(void *vc, uint64_t b, uint64_t e)
+{
+  prmt_ctx *c = (prmt_ctx *)vc;
+  for (uint64_t v = b; v < e; v++) {
+    double s = 0.0;
+    for (uint32_t x = c->trp[v]; x < c->trp[v + 1]; x++) s += c->contrib[c->tcol[x]];
+    c->nx[v] = c->base + c->d * s;
+  }
+}
+
+int orc_pr_pull_mt(uint32_t n, const uint32_t *rp, const uint32_t *col, double d, int iters, double tol, double *r,
+                   double *last_residual, int nthreads)
+{
+  uint64_t m = rp[n];
+  uint32_t *trp = (uint32_t *)xcalloc((size_t)n + 1, sizeof(uint32_t));
+  uint32_t *tcol = (uint32_t *)xcalloc(m, sizeof(uint32_t));
+  uint32_t *deg = (uint32_t *)xcalloc(n, sizeof(uint32_t));
+  for (uint64_t x = 0; x < m; x++) trp[col[x] + 1]++;
+  for (uint32_t v = 0; v < n; v++) trp[v + 1] += trp[v];
+  uint32_t *cur = (uint32_t *)xcalloc(n, sizeof(uint32_t));
+  memcpy(cur, trp, sizeof(uint32_t) * n);
+  for (uint32_t u = 0; u < n; u++) {
+    deg[u] = rp[u + 1] - rp[u];
+    for (uint32_t x = rp[u]; x < rp[u + 1]; x++) tcol[cur[col[x]]++] = u;
+  }
+  free(cur);
+  double *contrib = (double *)xcalloc(n, sizeof(double));
+  double *nx = (double *)xcalloc(n, sizeof(double));
+  for (uint32_t i = 0; i < n; i++) r[i] = 1.0 / (double)n;
+  prmt_ctx c = {n, trp, tcol, deg, r, contrib, nx, 0.0, d};
+  int it = 0, cap = iters > 0 ? iters : 10000;
+  double res = 0.0;
+  while (it < cap) {
+    double D = 0.0;
+    for (uint32_t u = 0; u < n; u++)
+      if (!deg[u]) D += r[u];
+    c.base = (1.0 - d) / (double)n + d * D / (double)n;
+    parallel_for(nthreads, n, 65536, prmt_contrib, &c);
+    parallel_for(nthreads, n, 16384, prmt_pull, &c);
+    res = 0.0;
+    for (uint32_t v = 0; v < n; v++) {
+      res += fabs(nx[v] - r[v]);
+      r[v] = nx[v];
+    }
+    it++;
+    if (iters <= 0 && res < tol) break;
+  }
+  if (last_residual) *last_residual = res;
+  free(trp);
+  free(tcol);
+  free(deg);
+  free(contrib);
+  free(nx);
+  return it;
+}

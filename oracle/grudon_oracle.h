@@ -121,6 +121,11 @@ void orc_cc_unionfind(uint32_t n, const uint32_t *row_ptr, const uint32_t *col,
 int orc_pr_pull(uint32_t n, const uint32_t *row_ptr, const uint32_t *col,
                 double damping, int iters, double tol, double *r,
                 double *last_residual);
+/* Same recurrence, evaluated as a pull over the transposed graph with
+ * nthreads threads (per-vertex FP64 sums in ascending source order). */
+int orc_pr_pull_mt(uint32_t n, const uint32_t *row_ptr, const uint32_t *col,
+                   double damping, int iters, double tol, double *r,
+                   double *last_residual, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,9 @@ def lib():
         L.orc_pr_pull.restype = ct.c_int
         L.orc_pr_pull.argtypes = [ct.c_uint32, _P32, _P32, ct.c_double, ct.c_int, ct.c_double, _PF64,
                                   ct.POINTER(ct.c_double)]
+        L.orc_pr_pull_mt.restype = ct.c_int
+        L.orc_pr_pull_mt.argtypes = [ct.c_uint32, _P32, _P32, ct.c_double, ct.c_int, ct.c_double, _PF64,
+                                     ct.POINTER(ct.c_double), ct.c_int]
         _lib = L
     return _lib
 
@@ -183,8 +186,12 @@ def cc_unionfind(g):
     return out
 
 
-def pr_pull(g, damping=0.85, iters=0, tol=1e-10):
+def pr_pull(g, damping=0.85, iters=0, tol=1e-10, nthreads=0):
+    """FP64 power iteration; nthreads > 0 uses the threaded pull form."""
     r = np.zeros(g.n, dtype=np.float64)
     res = ct.c_double(0.0)
-    it = lib().orc_pr_pull(g.n, g.rp, g.col, damping, iters, tol, r, ct.byref(res))
+    if nthreads > 0:
+        it = lib().orc_pr_pull_mt(g.n, g.rp, g.col, damping, iters, tol, r, ct.byref(res), nthreads)
+    else:
+        it = lib().orc_pr_pull(g.n, g.rp, g.col, damping, iters, tol, r, ct.byref(res))
     return r, it, res.value
