@@ -66,6 +66,32 @@ def traffic_from_profiles(kernel_prefix):
     return None
 
 
+def calibrate_copy(mib):
+    """Device-to-device copy of a cold buffer of known size (the runtime's
+    copy kernel); run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE it gives
+    the counters' scale factor: bytes counted / bytes copied."""
+    import ctypes as ct
+
+    hip = ct.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ct.c_void_p(), ct.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ct.byref(p), ct.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ct.c_size_t(n))
+    # flush the caches' view of `a` with an unrelated buffer larger than the
+    # 256 MB last-level cache before the measured copy
+    c = ct.c_void_p()
+    hip.hipMalloc(ct.byref(c), ct.c_size_t(1 << 30))
+    hip.hipMemset(c, 2, ct.c_size_t(1 << 30))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, ct.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    for p in (a, b, c):
+        hip.hipFree(p)
+    log("calibration copy: %d MiB device-to-device" % mib)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,7 +105,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 / bit-exact check")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a cold device copy of MIB MiB (PMC counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU (partitions are resident in one HBM); "

@@ -20,10 +20,12 @@ MLP_PARAMS = 5508
 # every extern "C" symbol declared in include/grudon_hip.h
 EXPORTS = [
     "grd_last_error", "grd_create", "grd_destroy", "grd_set_host_threads",
-    "grd_generate_rmat", "grd_generate_er", "grd_graph_from_csr", "grd_graph_rmat", "grd_graph_er",
-    "grd_graph_load", "grd_graph_free", "grd_graph_info", "grd_graph_table", "grd_graph_out_degree",
+    "grd_generate_rmat", "grd_generate_er", "grd_edge_weights", "grd_graph_from_csr", "grd_graph_rmat",
+    "grd_graph_er", "grd_graph_load", "grd_graph_read", "grd_graph_write_gr", "grd_graph_free", "grd_graph_info",
+    "grd_graph_table", "grd_graph_out_degree",
     "grd_state_create", "grd_state_free", "grd_init", "grd_broadcast", "grd_traverse", "grd_reduce",
-    "grd_update", "grd_aggregate_one", "grd_run", "grd_download", "grd_features", "grd_snapshot",
+    "grd_update", "grd_aggregate_one", "grd_run", "grd_select_option", "grd_state_stats", "grd_download",
+    "grd_features", "grd_snapshot",
     "grd_restore", "grd_selector_load", "grd_selector_train", "grd_selector_infer", "grd_selector_grad",
     "grd_batch_create", "grd_batch_free", "grd_batch_tune", "grd_batch_autotune", "grd_batch_pagerank",
     "grd_batch_cc", "grd_batch_download",
@@ -88,6 +90,11 @@ def lib():
             "grd_graph_free": [vp], "grd_graph_info": [vp, vp], "grd_graph_table": [vp, ct.c_int, ct.c_int, vp,
                                                                                    ct.POINTER(ct.c_uint64)],
             "grd_graph_out_degree": [vp, vp],
+            "grd_edge_weights": [vp, ct.c_uint32, vp, vp, ct.c_uint64, vp],
+            "grd_graph_read": [ct.c_char_p, ct.POINTER(ct.c_uint32), ct.POINTER(ct.c_uint64), vp, vp, vp,
+                               ct.POINTER(ct.c_int)],
+            "grd_graph_write_gr": [ct.c_char_p, ct.c_uint32, vp, vp, vp],
+            "grd_select_option": [vp, ct.POINTER(ct.c_int)], "grd_state_stats": [vp, ct.POINTER(RunStats)],
             "grd_state_create": [vp, ct.c_int, pp], "grd_state_free": [vp], "grd_init": [vp, ct.POINTER(Params)],
             "grd_broadcast": [vp, ct.c_int], "grd_traverse": [vp, ct.c_int], "grd_reduce": [vp, ct.c_int],
             "grd_update": [vp, ct.POINTER(ct.c_uint64)], "grd_aggregate_one": [vp, ct.c_uint32, vp],
@@ -313,6 +320,64 @@ class State:
     def free(self):
         if self.h:
             lib().grd_state_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many graphs processed together: one persistent launch per phase,
+    iterations replayed from a captured hipGraph."""
+
+    def __init__(self, ctx, graphs):
+        """graphs: list of (row_ptr, col) numpy arrays."""
+        self.ctx = ctx
+        self._rp = [np.ascontiguousarray(g[0], dtype=np.uint32) for g in graphs]
+        self._col = [np.ascontiguousarray(g[1], dtype=np.uint32) for g in graphs]
+        cnt = len(graphs)
+        self.n = np.array([len(r) - 1 for r in self._rp], dtype=np.uint32)
+        self.m = np.array([len(c) for c in self._col], dtype=np.uint64)
+        rps = (ct.c_void_p * cnt)(*[r.ctypes.data for r in self._rp])
+        cols = (ct.c_void_p * cnt)(*[c.ctypes.data if len(c) else 0 for c in self._col])
+        h = ct.c_void_p()
+        _check(lib().grd_batch_create(ctx.h, cnt, _ptr(self.n), _ptr(self.m), rps, cols, ct.byref(h)),
+               "grd_batch_create")
+        self.h = h
+        self.count = cnt
+
+    def tune(self, tile=0, grid_per_cu=0, K=0):
+        _check(lib().grd_batch_tune(self.h, tile, grid_per_cu, K), "grd_batch_tune")
+
+    def autotune(self):
+        out = np.zeros(3, dtype=np.uint32)
+        _check(lib().grd_batch_autotune(self.h, _ptr(out)), "grd_batch_autotune")
+        return {"tile": int(out[0]), "grid_per_cu": int(out[1]), "K": int(out[2])}
+
+    def pagerank(self, damping=0.85, tolerance=1e-6, max_iter=1000):
+        it = np.zeros(self.count, dtype=np.uint32)
+        ms = ct.c_double()
+        _check(lib().grd_batch_pagerank(self.h, damping, tolerance, max_iter, _ptr(it), ct.byref(ms)),
+               "grd_batch_pagerank")
+        return it, ms.value
+
+    def cc(self, max_iter=1000):
+        it = np.zeros(self.count, dtype=np.uint32)
+        ms = ct.c_double()
+        _check(lib().grd_batch_cc(self.h, max_iter, _ptr(it), ct.byref(ms)), "grd_batch_cc")
+        return it, ms.value
+
+    def result(self, i, algo):
+        out = np.zeros(int(self.n[i]), dtype=np.float32 if algo == PR_PULL else np.uint32)
+        _check(lib().grd_batch_download(self.h, i, algo, _ptr(out)), "grd_batch_download")
+        return out
+
+    def free(self):
+        if self.h:
+            lib().grd_batch_free(self.h)
             self.h = None
 
     def __del__(self):

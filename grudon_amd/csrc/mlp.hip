@@ -339,9 +339,9 @@ grd_status grd_selector_train(grd_ctx *ctx, const float *feat, const uint32_t *l
   for (int i = 0; i < kParams; i++) wb0[i] = f32_to_bf16(w0[i]);
   GRD_HIP(hipMemcpy(dw, w0.data(), kParams * 4, hipMemcpyHostToDevice));
   GRD_HIP(hipMemcpy(dwb, wb0.data(), kParams * 2, hipMemcpyHostToDevice));
-  GRD_HIP(hipMemset(dm1, 0, kParams * 4));
-  GRD_HIP(hipMemset(dm2, 0, kParams * 4));
-  GRD_HIP(hipMemset(dg, 0, kParams * 4));
+  GRD_HIP(hipMemsetAsync(dm1, 0, kParams * 4, st));
+  GRD_HIP(hipMemsetAsync(dm2, 0, kParams * 4, st));
+  GRD_HIP(hipMemsetAsync(dg, 0, kParams * 4, st));
   GRD_HIP(hipMemcpy(dfeat, feat, n * kIn * 4, hipMemcpyHostToDevice));
   GRD_HIP(hipMemcpy(dsw, sw.data(), n * 4, hipMemcpyHostToDevice));
   GRD_HIP(hipMemcpy(dlab, label, n * 4, hipMemcpyHostToDevice));
@@ -371,7 +371,8 @@ grd_status grd_selector_train(grd_ctx *ctx, const float *feat, const uint32_t *l
     last = ls / (float)n;
   }
   std::vector<uint16_t> wb(kParams);
-  GRD_HIP(hipMemcpy(wb.data(), dwb, kParams * 2, hipMemcpyDeviceToHost));
+  GRD_HIP(hipMemcpyAsync(wb.data(), dwb, kParams * 2, hipMemcpyDeviceToHost, st));
+  GRD_HIP(hipStreamSynchronize(st));
   if (out_bf16) memcpy(out_bf16, wb.data(), kParams * 2);
   if (final_loss) *final_loss = last;
   return grd_selector_load(ctx, wb.data(), kParams, ctx->sel_margin);
@@ -417,14 +418,15 @@ grd_status grd_selector_grad(grd_ctx *ctx, const float *w, const float *feat, co
     GRD_TRY(mem.alloc_n(&dsw, n));
     GRD_HIP(hipMemcpy(dsw, cost, n * 4, hipMemcpyHostToDevice));
   }
-  GRD_HIP(hipMemset(dg, 0, kParams * 4));
-  GRD_HIP(hipMemset(dloss, 0, 4));
+  GRD_HIP(hipMemsetAsync(dg, 0, kParams * 4, ctx->stream));
+  GRD_HIP(hipMemsetAsync(dloss, 0, 4, ctx->stream));
   uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(64, (n + 15) / 16));
   hipLaunchKernelGGL(k_mlp_grad<false>, dim3(grid), dim3(kBlock), 0, ctx->stream, dw, nullptr, df, dl, dsw, nullptr,
                      (uint32_t)n, dg, dloss);
   GRD_HIP(hipGetLastError());
-  GRD_HIP(hipMemcpy(grad, dg, kParams * 4, hipMemcpyDeviceToHost));
-  GRD_HIP(hipMemcpy(loss, dloss, 4, hipMemcpyDeviceToHost));
+  GRD_HIP(hipMemcpyAsync(grad, dg, kParams * 4, hipMemcpyDeviceToHost, ctx->stream));
+  GRD_HIP(hipMemcpyAsync(loss, dloss, 4, hipMemcpyDeviceToHost, ctx->stream));
+  GRD_HIP(hipStreamSynchronize(ctx->stream));
   return GRD_OK;
 }
 

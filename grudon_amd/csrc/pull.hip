@@ -44,7 +44,8 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_rb(const PullBlock *__restr
 {
   __shared__ float s_v[kPullNnz];
   __shared__ float s_ws[kWavesPerBlock], s_wc[kWavesPerBlock];
-  const PullBlock d = blocks[blockIdx.x];
+  const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+  const PullBlock d = blocks[bid];
   if (d.nrows == 0) {  // segment of a long row
     Kahan k;
     for (uint32_t e = d.e0 + threadIdx.x; e < d.e1; e += kBlock) k.add(mprop[ccol[e]]);
@@ -59,12 +60,33 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_rb(const PullBlock *__restr
       t.s = s_ws[0];
       t.c = s_wc[0];
       for (int j = 1; j < kWavesPerBlock; j++) t.merge(s_ws[j], s_wc[j]);
-      partial[blockIdx.x] = make_float2(t.s, t.c);
+      partial[bid] = make_float2(t.s, t.c);
     }
     return;
   }
   const uint32_t ne = d.e1 - d.e0;
-  for (uint32_t i = threadIdx.x; i < ne; i += kBlock) s_v[i] = mprop[ccol[d.e0 + i]];
+  {
+    // all column loads first, then all gathers: 8 independent requests in
+    // flight per thread for each stage
+    constexpr int kPer = kPullNnz / kBlock;
+    uint32_t src[kPer];
+    float v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      uint32_t i = threadIdx.x + k * kBlock;
+      src[k] = i < ne ? ccol[d.e0 + i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      uint32_t i = threadIdx.x + k * kBlock;
+      v[k] = i < ne ? mprop[src[k]] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      uint32_t i = threadIdx.x + k * kBlock;
+      if (i < ne) s_v[i] = v[k];
+    }
+  }
   __syncthreads();
   const int G = group_size(d.nrows);
   const int sub = threadIdx.x % G;
@@ -143,19 +165,47 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
   using V = typename Op::V;
   __shared__ V s_v[kPullNnz];
   __shared__ unsigned long long s_dirty[kDirtyWords];
-  const PullBlock d = blocks[blockIdx.x];
+  const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+  const PullBlock d = blocks[bid];
   if (d.nrows == 0) {
     V acc = op_identity<Op>();
     for (uint32_t e = d.e0 + threadIdx.x; e < d.e1; e += kBlock)
       acc = op_combine<Op>(acc, pull_val<Op>(e, ccol, cw, Dp, mprop));
     if constexpr (Op::kMin) acc = block_min(acc);
     else acc = block_sum(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+    if (threadIdx.x == 0) partial[bid] = acc;
     return;
   }
   for (int i = threadIdx.x; i < kDirtyWords; i += kBlock) s_dirty[i] = 0ull;
   const uint32_t ne = d.e1 - d.e0;
-  for (uint32_t i = threadIdx.x; i < ne; i += kBlock) s_v[i] = pull_val<Op>(d.e0 + i, ccol, cw, Dp, mprop);
+  {
+    // batched stages: column ids, active bits, then the gathers of the
+    // active sources
+    constexpr int kPer = kPullNnz / kBlock;
+    uint32_t src[kPer];
+    uint64_t act[kPer];
+    V v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      uint32_t i = threadIdx.x + k * kBlock;
+      src[k] = i < ne ? ccol[d.e0 + i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      uint32_t i = threadIdx.x + k * kBlock;
+      act[k] = i < ne ? (Dp[src[k] >> 6] >> (src[k] & 63)) & 1ull : 0ull;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      uint32_t i = threadIdx.x + k * kBlock;
+      v[k] = act[k] ? Op::edge(mprop[src[k]], cw ? cw[d.e0 + i] : 1u) : op_identity<Op>();
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      uint32_t i = threadIdx.x + k * kBlock;
+      if (i < ne) s_v[i] = v[k];
+    }
+  }
   __syncthreads();
   const int G = group_size(d.nrows);
   const int sub = threadIdx.x % G;
