@@ -92,6 +92,47 @@ def calibrate_copy(mib):
     log("calibration copy: %d MiB device-to-device" % mib)
 
 
+def schedule_speedup(api, ctx, scale, parts):
+    """Learned vs plain (and heuristic) schedule on RMAT-`scale`: CC on the
+    symmetrized graph, SSSP hop count from gid 0 and the reference delta
+    PageRank on the directed graph.  Speedups are time ratios of the same
+    work; GTEPS follow the definitions of the PR (arcs x plain iterations /
+    time), CC and SSSP (arcs / time) workloads."""
+    import numpy as np
+
+    wpath = os.path.join(ROOT, "grudon_amd", "selector_bf16.bin")
+    ctx.selector_load(np.fromfile(wpath, dtype="<u2"), 0.0)
+    out = {"graph": "RMAT-%d EF16 seed 1, 1 compute + %d memory partitions" % (scale, parts)}
+    scheds = [("plain", api.SCHED_PLAIN), ("heuristic", api.SCHED_HEURISTIC), ("learned", api.SCHED_LEARNED)]
+    for sym, algo, name, kw in [(True, api.CC, "cc", {}), (False, api.SSSP, "sssp", {"source": 0}),
+                                (False, api.PR, "pr", {"max_iter": 1000})]:
+        G = api.Graph.rmat(ctx, scale, 16, seed=1, symmetrize=sym, C=1, M=parts)
+        S = api.State(G, algo)
+        ms = {}
+        iters_plain = None
+        for label, sc in scheds:
+            S.init(sched=sc, **kw)
+            S.run()
+            best = None
+            for _ in range(3):
+                S.init(sched=sc, **kw)
+                t = time.perf_counter()
+                st = S.run()
+                dt = (time.perf_counter() - t) * 1e3
+                best = dt if best is None else min(best, dt)
+            ms[label] = best
+            if sc == api.SCHED_PLAIN:
+                iters_plain = st["iterations"]
+        work = G.m * (iters_plain if algo == api.PR else 1)
+        out[name] = {"ms": {k: round(v, 3) for k, v in ms.items()},
+                     "gteps": {k: round(work / (v * 1e-3) / 1e9, 2) for k, v in ms.items()},
+                     "learned_vs_plain": round(ms["plain"] / ms["learned"], 3),
+                     "learned_vs_heuristic": round(ms["heuristic"] / ms["learned"], 3)}
+        S.free()
+        G.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +146,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 / bit-exact check")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--speedup-scale", type=int, default=22,
+                    help="RMAT scale of the learned-vs-plain schedule leg (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a cold device copy of MIB MiB (PMC counter calibration)")
     args = ap.parse_args()
@@ -179,6 +222,13 @@ def main():
         res = oracle_check.pr_pull_l1(rp, col, S.result(), last["iterations"], args.cpu_threads)
         extra["fp64_l1"] = res
         log("FP64 oracle L1 = %.3e" % res)
+
+    # free the headline workload before the schedule-comparison leg
+    S.free()
+    G.free()
+    if args.speedup_scale:
+        extra["learned_vs_plain"] = schedule_speedup(api, ctx, args.speedup_scale, args.parts)
+        log("learned vs plain: %s" % json.dumps(extra["learned_vs_plain"]))
 
     cpu = None
     if not args.no_cpu:

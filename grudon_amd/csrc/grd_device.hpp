@@ -11,20 +11,6 @@ constexpr int kWave = 64;
 constexpr int kBlock = 256;  // 4 waves
 constexpr int kWavesPerBlock = kBlock / kWave;
 
-// MI355X has 8 XCDs, each with its own L2; consecutive workgroups of a launch
-// are dispatched round-robin over them.  xcd_block() maps the hardware block
-// index so that XCD x runs the contiguous logical range
-// [x*q + min(x, r), ...) of the grid (q = G / 8, r = G % 8): rows that are
-// adjacent in memory, and hence the same memory partition's gather working
-// set, stay in one L2.  It is a bijection on [0, G).
-constexpr uint32_t kXcds = 8;
-__device__ __forceinline__ uint32_t xcd_block(uint32_t p, uint32_t G)
-{
-  const uint32_t q = G / kXcds, r = G % kXcds;
-  const uint32_t x = p % kXcds, j = p / kXcds;
-  return x * q + (x < r ? x : r) + j;
-}
-
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 __device__ __forceinline__ int wave_in_block() { return (int)(threadIdx.x / kWave); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }

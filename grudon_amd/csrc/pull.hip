@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_rb(const PullBlock *__restr
 {
   __shared__ float s_v[kPullNnz];
   __shared__ float s_ws[kWavesPerBlock], s_wc[kWavesPerBlock];
-  const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t bid = blockIdx.x;
   const PullBlock d = blocks[bid];
   if (d.nrows == 0) {  // segment of a long row
     Kahan k;
@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
   using V = typename Op::V;
   __shared__ V s_v[kPullNnz];
   __shared__ unsigned long long s_dirty[kDirtyWords];
-  const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t bid = blockIdx.x;
   const PullBlock d = blocks[bid];
   if (d.nrows == 0) {
     V acc = op_identity<Op>();

@@ -1,0 +1,68 @@
+"""What the trained schedule selector gains over the schedule without it.
+
+Held-out seeded graphs (seeds outside the training draw), CC / SSSP / the
+reference PageRank run with the plain schedule (push + sparse sync every
+superstep, the reference's), the direction-optimizing heuristic, and the
+learned MLP (packaged BF16 weights, heuristic as fallback).  The schedule
+never changes results: CC labels and SSSP distances are compared bit for
+bit across schedules, PageRank within the FP32 bound.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from grudon_amd import api
+
+pytestmark = pytest.mark.gpu
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "grudon_amd", "selector_bf16.bin")
+
+
+def _run(S, sched, **kw):
+    S.init(sched=sched, **kw)
+    S.run()  # warm-up
+    S.init(sched=sched, **kw)
+    st = S.run()
+    return st, S.result()
+
+
+def test_learned_selector_gain(ctx):
+    w = np.fromfile(WEIGHTS, dtype="<u2")
+    assert w.size == api.MLP_PARAMS
+    ctx.selector_load(w, min_margin=0.0)
+    totals = {api.SCHED_PLAIN: 0.0, api.SCHED_HEURISTIC: 0.0, api.SCHED_LEARNED: 0.0}
+    rows = []
+    cases = [("rmat20", dict(scale=20, ef=16, seed=424242)), ("er", None)]
+    for name, spec in cases:
+        if spec:
+            Gd = api.Graph.rmat(ctx, spec["scale"], spec["ef"], seed=spec["seed"], C=1, M=4)
+            Gs = api.Graph.rmat(ctx, spec["scale"], spec["ef"], seed=spec["seed"], symmetrize=True, C=1, M=4)
+        else:
+            Gs = Gd = api.Graph.er(ctx, 1 << 19, 12.0, seed=434343, C=1, M=4)
+        jobs = [(Gs, api.CC, {}), (Gd, api.SSSP, {"source": int(np.argmax(Gd.out_degree() > 0))})]
+        if spec:
+            jobs.append((Gd, api.PR, {"max_iter": 60}))
+        for G, algo, kw in jobs:
+            S = api.State(G, algo)
+            res = {}
+            for sched in totals:
+                st, out = _run(S, sched, **kw)
+                res[sched] = (st, out)
+                totals[sched] += st["ms"]
+            base = res[api.SCHED_PLAIN][1]
+            for sched in totals:
+                if algo == api.PR:
+                    a = res[sched][1].astype(np.float64)
+                    assert np.abs(a / a.sum() - base / base.sum()).sum() < 1e-5
+                else:
+                    assert np.array_equal(res[sched][1], base)
+            rows.append((name, algo, {s: round(res[s][0]["ms"], 3) for s in totals},
+                         res[api.SCHED_LEARNED][0]["option_counts"], res[api.SCHED_LEARNED][0]["fallbacks"]))
+            S.free()
+    for r in rows:
+        print("selector gain", r, flush=True)
+    sp_plain = totals[api.SCHED_PLAIN] / totals[api.SCHED_LEARNED]
+    sp_heur = totals[api.SCHED_HEURISTIC] / totals[api.SCHED_LEARNED]
+    print("learned vs plain %.2fx, learned vs heuristic %.2fx" % (sp_plain, sp_heur), flush=True)
+    assert sp_plain >= 1.3
+    assert sp_heur >= 0.9
