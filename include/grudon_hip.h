@@ -218,9 +218,12 @@ typedef struct grd_batch grd_batch;
 grd_status grd_batch_create(grd_ctx *ctx, uint32_t count, const uint32_t *n, const uint64_t *m,
                             const uint32_t *const *row_ptr, const uint32_t *const *col, grd_batch **out);
 grd_status grd_batch_free(grd_batch *b);
-/* Tunables: rows per tile, heavy-row threshold, lanes per row. 0 = keep. */
-grd_status grd_batch_tune(grd_batch *b, uint32_t tile_rows, uint32_t heavy_threshold, uint32_t lanes);
-/* Autotune the tunables by timing candidate settings on this batch. */
+/* Tunables (0 = keep): tile = arcs per row block and long-row threshold
+ * (1024, 2048 or 4096), grid_per_cu = persistent workgroups per CU (<= 64),
+ * K = iterations per captured hipGraph (<= 64). */
+grd_status grd_batch_tune(grd_batch *b, uint32_t tile, uint32_t grid_per_cu, uint32_t K);
+/* Autotune the tunables by timing candidate settings on this batch
+ * (tile x grid_per_cu x K); chosen3 (may be NULL) receives the winner. */
 grd_status grd_batch_autotune(grd_batch *b, uint32_t *chosen3);
 /* PageRank pull (Kahan FP32, FP64 dangling/residual) on every graph, with
  * iterations captured in a hipGraph; iters_out[count], ms_out may be NULL. */

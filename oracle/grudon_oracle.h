@@ -18,7 +18,11 @@
  *   4. closed-form oracles used to pin (3): BFS, Dijkstra, union-find
  *      connected components, FP64 power-iteration PageRank.
  *
- * Pinning status of each function is recorded in oracle/README.md.
+ * Every function is pinned by a check that shares no code with it (golden
+ * fixtures from textbook definitions, scipy / networkx routines, closed forms,
+ * the reference's verify() invariants; the table per function is in
+ * oracle/README.md), with one exception: the exchanged-byte count that the BSP
+ * restatement reports in stats[1] is parity unpinned.
  */
 #ifndef GRUDON_ORACLE_H
 #define GRUDON_ORACLE_H
