@@ -11,24 +11,41 @@ the L1 change drops below 1e-6.
 
 A step is one complete PageRank solve (init + iterations to convergence) on
 the resident graph; GTEPS = arcs x iterations / time.  The dominant kernel
-(the pull over the memory partitions' CSC) is bracketed by HIP events on its
-own stream inside the timed steps for the roofline figure.
+(the pull over the memory partitions' CSC) is bracketed by HIP events on the
+context's stream inside the timed steps for the roofline figure.
 
-    python bench.py [--steps K] [--warmup W] [--scale S] [--parts M]
-                    [--workload pr_pull|cc|sssp|pr] [--no-cpu] [--no-check]
+Other workloads (one JSON line each, same fields):
+  cc       CC on RMAT-24 symmetrized; step = one solve; GTEPS = arcs / time;
+           labels checked bit for bit against scipy connected components
+  sssp     weighted SSSP (weights [1,255], seed 2) from 16 sources (seed 3,
+           degree >= 1) on RMAT-24 symmetrized, delta-stepping; step = the 16
+           solves; GTEPS = harmonic mean of arcs / time over the sources;
+           distances checked against the Bellman equations
+  pr       the reference's delta-push PageRank (FP32) on RMAT-24
+  batch    64 graphs of uneven size (32 RMAT scale 14-20 EF 8-32, 32
+           Erdos-Renyi n 2^14-2^20 mean degree 4-64, seed 4) in one batch,
+           autotuned; step = PageRank + CC on all of them
+  learned  learned vs heuristic vs plain schedule on RMAT-26 (weights
+           [1,255]): reference PageRank, CC, weighted SSSP; value = geometric
+           mean speedup of learned over plain
+
+    python bench.py [--steps K] [--warmup W] [--workload NAME] [--scale S]
+                    [--parts M] [--no-cpu] [--no-check]
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-sys.path.insert(1, os.path.join(ROOT, "tests"))  # oracle legs only (FP64 check, CPU baseline)
+sys.path.insert(1, os.path.join(ROOT, "tests"))  # the cpu_baseline leg only
 
 METRIC = "GTEPS on 1 MI355X, RMAT PR/CC/SSSP; learned-vs-plain schedule speedup"
 HBM_PEAK_GBS = 8000.0
+INF = 0xFFFFFFFF
 
 
 def log(*a):
@@ -50,6 +67,11 @@ def cpu_info():
     except AttributeError:
         cores = os.cpu_count() or 1
     return model, cores
+
+
+def cpu_threads(args):
+    model, cores = cpu_info()
+    return model, (args.cpu_threads or min(cores, 64))
 
 
 def traffic_from_profiles(kernel_prefix):
@@ -92,45 +114,423 @@ def calibrate_copy(mib):
     log("calibration copy: %d MiB device-to-device" % mib)
 
 
-def schedule_speedup(api, ctx, scale, parts):
-    """Learned vs plain (and heuristic) schedule on RMAT-`scale`: CC on the
-    symmetrized graph, SSSP hop count from gid 0 and the reference delta
-    PageRank on the directed graph.  Speedups are time ratios of the same
-    work; GTEPS follow the definitions of the PR (arcs x plain iterations /
-    time), CC and SSSP (arcs / time) workloads."""
+# ---------------------------------------------------------------------------
+# full-size checks (numpy / scipy, independent of the library)
+# ---------------------------------------------------------------------------
+
+def fp64_pagerank(rp, col, iters, d=0.85):
+    """FP64 power iteration r' = (1-d)/n + d (A^T (r/deg) + D/n), r0 = 1/n,
+    run for exactly `iters` iterations."""
+    import numpy as np
+    import scipy.sparse as sp
+
+    n = len(rp) - 1
+    rp64 = rp.astype(np.int64)
+    deg = np.diff(rp64).astype(np.float64)
+    sink = deg == 0
+    A = sp.csr_matrix((np.ones(len(col), dtype=np.float64), col, rp64), shape=(n, n))
+    AT = A.T  # CSC view: A^T x without an explicit transpose
+    r = np.full(n, 1.0 / n)
+    inv = np.where(sink, 0.0, 1.0 / np.where(sink, 1.0, deg))
+    for _ in range(int(iters)):
+        D = r[sink].sum()
+        r = (1.0 - d) / n + d * (AT @ (r * inv) + D / n)
+    return r
+
+
+def cc_labels_scipy(rp, col):
+    """Min gid of every vertex's connected component (symmetric input)."""
+    import numpy as np
+    import scipy.sparse as sp
+    import scipy.sparse.csgraph as cg
+
+    n = len(rp) - 1
+    A = sp.csr_matrix((np.ones(len(col), dtype=np.int8), col, rp.astype(np.int64)), shape=(n, n))
+    _, comp = cg.connected_components(A, directed=False)
+    mins = np.full(comp.max() + 1, np.iinfo(np.int64).max, dtype=np.int64)
+    np.minimum.at(mins, comp, np.arange(n, dtype=np.int64))
+    return mins[comp].astype(np.uint32)
+
+
+def sssp_bellman_ok(rp, col, w, dist, src):
+    """Exact check of shortest distances on a symmetric graph with positive
+    weights: d[src] = 0 and d[v] = min over arcs (v,u) of d[u] + w(u,v) for
+    every other v (INF when no neighbour is reached).  With positive weights
+    these equations have exactly one solution."""
+    import numpy as np
+
+    d = dist.astype(np.int64)
+    du = d[col]
+    cand = np.where(du == INF, INF, du + w.astype(np.int64))
+    del du
+    rp64 = rp.astype(np.int64)
+    nonempty = rp64[1:] > rp64[:-1]
+    mins = np.full(len(d), INF, dtype=np.int64)
+    mins[nonempty] = np.minimum.reduceat(cand, rp64[:-1][nonempty])
+    mins = np.minimum(mins, INF)
+    mins[src] = 0
+    return bool(np.array_equal(d, mins))
+
+
+# ---------------------------------------------------------------------------
+# common pieces
+# ---------------------------------------------------------------------------
+
+def roofline_of(kms, klaunch, kbytes, kname, traffic):
+    achieved = (kbytes / klaunch) / (kms / klaunch * 1e-3) / 1e9 if klaunch else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": kname, "launches": klaunch,
+            "avg_launch_us": round(kms / klaunch * 1e3, 2) if klaunch else None,
+            "algorithmic_bytes_per_launch": int(kbytes / klaunch) if klaunch else None}
+
+
+def line(args, value, unit, ms_per_step, dtype, config, roofline, cpu, extra=None, higher=True):
+    out = {"metric": METRIC, "value": round(value, 3), "unit": unit, "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+           "higher_is_better": higher, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic", "config": config, "roofline": roofline, "cpu_baseline": cpu}
+    if extra:
+        out.update(extra)
+    return out
+
+
+def layout_desc(args, scale, sym):
+    return ("RMAT-%d EF%d seed 1%s, %d compute + %d memory partitions resident in HBM, "
+            "on-device bitvector master/mirror sync" % (scale, args.ef, " symmetrized" if sym else "",
+                                                         args.compute, args.parts))
+
+
+def timed_solves(S, args, init_kw, sources=None):
+    """Warm-up, then exactly args.steps timed steps; a step is one solve (or
+    one solve per source).  Returns (seconds, iterations, kernel stats, last
+    run stats)."""
+    srcs = sources if sources is not None else [None]
+
+    def step(time_kernel):
+        acc = [0, 0.0, 0, 0]
+        st = None
+        for s in srcs:
+            kw = dict(init_kw)
+            if s is not None:
+                kw["source"] = int(s)
+            S.init(time_kernel=time_kernel, **kw)
+            st = S.run()
+            acc[0] += st["iterations"]
+            acc[1] += st["kernel_ms"]
+            acc[2] += st["kernel_launches"]
+            acc[3] += st["kernel_bytes"]
+        return acc, st
+
+    for _ in range(args.warmup):
+        step(False)
+    iters, kms, kl, kb = 0, 0.0, 0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        acc, last = step(True)
+        iters += acc[0]
+        kms += acc[1]
+        kl += acc[2]
+        kb += acc[3]
+    return time.perf_counter() - t0, iters, (kms, kl, kb), last
+
+
+def schedule_speedup(api, ctx, scale, parts, ef=16, weighted=False, reps=3, sym_cc=True):
+    """Learned vs plain (and heuristic) schedule on RMAT-`scale`: CC (on the
+    symmetrized graph when sym_cc), SSSP from gid 0 (weighted or hop count)
+    and the reference delta PageRank on the directed graph.  Speedups are
+    time ratios of the same work; GTEPS follow the definitions of the PR
+    (arcs x plain iterations / time), CC and SSSP (arcs / time) workloads.
+    Results are compared across the schedules (bit for bit for CC / SSSP)."""
     import numpy as np
 
     wpath = os.path.join(ROOT, "grudon_amd", "selector_bf16.bin")
     ctx.selector_load(np.fromfile(wpath, dtype="<u2"), 0.0)
-    out = {"graph": "RMAT-%d EF16 seed 1, 1 compute + %d memory partitions" % (scale, parts)}
+    out = {"graph": "RMAT-%d EF%d seed 1, 1 compute + %d memory partitions%s"
+                    % (scale, ef, parts, ", weights [1,255] seed 2" if weighted else "")}
     scheds = [("plain", api.SCHED_PLAIN), ("heuristic", api.SCHED_HEURISTIC), ("learned", api.SCHED_LEARNED)]
-    for sym, algo, name, kw in [(True, api.CC, "cc", {}), (False, api.SSSP, "sssp", {"source": 0}),
-                                (False, api.PR, "pr", {"max_iter": 1000})]:
-        G = api.Graph.rmat(ctx, scale, 16, seed=1, symmetrize=sym, C=1, M=parts)
-        S = api.State(G, algo)
+    jobs = [(sym_cc, api.CC, "cc", {}), (False, api.SSSP, "sssp", {"source": 0, "weighted": weighted}),
+            (False, api.PR, "pr", {"max_iter": 1000})]
+    G = None
+    G_sym = None
+    for sym, algo, name, kw in jobs:
+        if sym:
+            if G_sym is None:
+                G_sym = api.Graph.rmat(ctx, scale, ef, seed=1, symmetrize=True, wseed=2 if weighted else 0,
+                                       C=1, M=parts)
+            Gx = G_sym
+        else:
+            if G is None:
+                if G_sym is not None:
+                    G_sym.free()
+                    G_sym = None
+                G = api.Graph.rmat(ctx, scale, ef, seed=1, symmetrize=False, wseed=2 if weighted else 0,
+                                   C=1, M=parts)
+            Gx = G
+        S = api.State(Gx, algo)
         ms = {}
+        res = {}
         iters_plain = None
         for label, sc in scheds:
             S.init(sched=sc, **kw)
             S.run()
             best = None
-            for _ in range(3):
+            for _ in range(reps):
                 S.init(sched=sc, **kw)
                 t = time.perf_counter()
                 st = S.run()
                 dt = (time.perf_counter() - t) * 1e3
                 best = dt if best is None else min(best, dt)
             ms[label] = best
+            res[label] = S.result()
             if sc == api.SCHED_PLAIN:
                 iters_plain = st["iterations"]
-        work = G.m * (iters_plain if algo == api.PR else 1)
-        out[name] = {"ms": {k: round(v, 3) for k, v in ms.items()},
+        if algo == api.PR:
+            base = res["plain"].astype(np.float64)
+            same = all(np.abs(r.astype(np.float64) / r.sum() - base / base.sum()).sum() < 1e-5
+                       for r in res.values())
+        else:
+            same = all(np.array_equal(r, res["plain"]) for r in res.values())
+        work = Gx.m * (iters_plain if algo == api.PR else 1)
+        out[name] = {"arcs": Gx.m, "ms": {k: round(v, 3) for k, v in ms.items()},
                      "gteps": {k: round(work / (v * 1e-3) / 1e9, 2) for k, v in ms.items()},
                      "learned_vs_plain": round(ms["plain"] / ms["learned"], 3),
-                     "learned_vs_heuristic": round(ms["heuristic"] / ms["learned"], 3)}
+                     "learned_vs_heuristic": round(ms["heuristic"] / ms["learned"], 3),
+                     "results_identical": bool(same)}
+        log("schedule %s: %s" % (name, json.dumps(out[name])))
         S.free()
-        G.free()
+    for g in (G, G_sym):
+        if g is not None:
+            g.free()
     return out
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+
+def run_pagerank(api, ctx, args):
+    import numpy as np
+
+    pull = args.workload == "pr_pull"
+    t0 = time.perf_counter()
+    rp, col = ctx.generate_rmat(args.scale, args.ef, seed=1)
+    t_gen = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    G = api.Graph.from_csr(ctx, rp, col, C=args.compute, M=args.parts)
+    t_part = time.perf_counter() - t1
+    log("graph: n=%d m=%d proxies=%d cross=%d gen %.1fs partition+upload %.1fs device %.2f GB"
+        % (G.n, G.m, G.proxies, G.cross, t_gen, t_part, G.device_bytes / 1e9))
+    algo = api.PR_PULL if pull else api.PR
+    S = api.State(G, algo)
+    secs, iters, (kms, kl, kb), last = timed_solves(
+        S, args, {"max_iter": 1000, "tolerance": 1e-6, "sched": api.SCHED_HEURISTIC})
+    gteps = G.m * iters / secs / 1e9
+    log("timed %d steps: %.3f s, %d iterations, %.2f ms/step, %.1f GTEPS"
+        % (args.steps, secs, iters, secs * 1e3 / args.steps, gteps))
+    kname = "k_prp_pull" if pull else "k_bsp_traverse"
+    roof = roofline_of(kms, kl, kb, kname, traffic_from_profiles("%s_rmat%d" % (kname, args.scale)))
+    extra = {}
+    if pull and not args.no_check:
+        ref = fp64_pagerank(rp, col, last["iterations"])
+        extra["fp64_l1"] = float(np.abs(S.result().astype(np.float64) - ref).sum())
+        log("FP64 L1 = %.3e (bar 1e-6)" % extra["fp64_l1"])
+    n, m, bytes_x = G.n, G.m, last["bytes_exchanged"]
+    S.free()
+    G.free()
+    if pull and args.speedup_scale:
+        extra["learned_vs_plain"] = schedule_speedup(api, ctx, args.speedup_scale, args.parts)
+    cpu = None
+    if not args.no_cpu:
+        import oracle_check
+        model, nt = cpu_threads(args)
+        cpu = oracle_check.cpu_baseline(rp, col, args.compute, args.parts, nt, api.PR)
+        cpu.update(cores=nt, cpu_model=model)
+        log("cpu baseline: %s" % json.dumps(cpu))
+    wl = ("PageRank pull (FP32 Kahan, FP64 dangling+residual, d=0.85, tol 1e-6)" if pull
+          else "reference delta-push PageRank (FP32, d=0.85, per-vertex threshold 1e-6)")
+    config = {"workload": wl + ", " + layout_desc(args, args.scale, False), "n": n, "arcs": m,
+              "iterations_per_step": last["iterations"], "schedule": "heuristic",
+              "exchange_bytes_per_step": bytes_x}
+    return line(args, gteps, "GTEPS", secs * 1e3 / args.steps, "f32", config, roof, cpu, extra)
+
+
+def run_cc(api, ctx, args):
+    import numpy as np
+
+    rp, col = ctx.generate_rmat(args.scale, args.ef, seed=1, symmetrize=True)
+    t1 = time.perf_counter()
+    G = api.Graph.from_csr(ctx, rp, col, C=args.compute, M=args.parts)
+    log("graph: n=%d m=%d partition+upload %.1fs" % (G.n, G.m, time.perf_counter() - t1))
+    S = api.State(G, api.CC)
+    secs, iters, (kms, kl, kb), last = timed_solves(S, args, {"sched": api.SCHED_HEURISTIC})
+    gteps = G.m * args.steps / secs / 1e9
+    log("timed %d steps: %.3f s, %.1f GTEPS" % (args.steps, secs, gteps))
+    extra = {}
+    if not args.no_check:
+        extra["labels_bit_exact"] = bool(np.array_equal(S.result(), cc_labels_scipy(rp, col)))
+        log("labels bit-exact vs scipy components: %s" % extra["labels_bit_exact"])
+    roof = roofline_of(kms, kl, kb, "k_bsp_traverse", None)
+    n, m = G.n, G.m
+    S.free()
+    G.free()
+    cpu = None
+    if not args.no_cpu:
+        import oracle_check
+        model, nt = cpu_threads(args)
+        cpu = oracle_check.cpu_baseline(rp, col, args.compute, args.parts, nt, api.CC)
+        cpu.update(cores=nt, cpu_model=model)
+    config = {"workload": "CC label propagation (u32 labels), " + layout_desc(args, args.scale, True),
+              "n": n, "arcs": m, "iterations_per_step": last["iterations"], "schedule": "heuristic",
+              "gteps_definition": "arcs / time of one solve"}
+    return line(args, gteps, "GTEPS", secs * 1e3 / args.steps, "u32", config, roof, cpu, extra)
+
+
+def run_sssp(api, ctx, args):
+    import numpy as np
+
+    rp, col = ctx.generate_rmat(args.scale, args.ef, seed=1, symmetrize=True)
+    w = ctx.edge_weights(rp, col, 2)
+    deg = np.diff(rp.astype(np.int64))
+    rng = np.random.default_rng(3)
+    sources = rng.choice(np.flatnonzero(deg > 0), size=16, replace=False)
+    t1 = time.perf_counter()
+    G = api.Graph.from_csr(ctx, rp, col, w, C=args.compute, M=args.parts)
+    log("graph: n=%d m=%d partition+upload %.1fs" % (G.n, G.m, time.perf_counter() - t1))
+    S = api.State(G, api.SSSP)
+    kw = {"weighted": True, "delta": args.delta, "sched": api.SCHED_HEURISTIC}
+    secs, iters, (kms, kl, kb), last = timed_solves(S, args, kw, sources=sources)
+    # harmonic mean over sources of arcs / t_s = 16 arcs / sum t_s
+    gteps = G.m * len(sources) * args.steps / secs / 1e9
+    log("timed %d steps x %d sources: %.3f s, %.1f GTEPS" % (args.steps, len(sources), secs, gteps))
+    extra = {}
+    if not args.no_check:
+        ok = []
+        for s in sources:
+            S.init(source=int(s), **kw)
+            S.run()
+            ok.append(sssp_bellman_ok(rp, col, w, S.result(), int(s)))
+        extra["distances_exact"] = bool(all(ok))
+        log("Bellman check over %d sources: %s" % (len(ok), ok))
+    roof = roofline_of(kms, kl, kb, "k_bsp_traverse", None)
+    n, m = G.n, G.m
+    S.free()
+    G.free()
+    cpu = None
+    if not args.no_cpu:
+        import oracle_check
+        model, nt = cpu_threads(args)
+        cpu = oracle_check.cpu_baseline(rp, col, args.compute, args.parts, nt, api.SSSP, supersteps=12, w=w,
+                                        source=int(sources[0]))
+        cpu.update(cores=nt, cpu_model=model)
+    config = {"workload": "SSSP delta-stepping (u32 distances, delta %d), weights [1,255] seed 2, 16 sources "
+                          "seed 3 among degree >= 1, %s" % (args.delta, layout_desc(args, args.scale, True)),
+              "n": n, "arcs": m, "iterations_per_step": iters // max(1, args.steps), "schedule": "heuristic",
+              "sources": [int(s) for s in sources],
+              "gteps_definition": "harmonic mean over the 16 sources of arcs / time"}
+    return line(args, gteps, "GTEPS", secs * 1e3 / args.steps, "u32", config, roof, cpu, extra)
+
+
+def batch_graphs(ctx, seed=4):
+    """32 RMAT (scale 14-20, EF 8-32) and 32 Erdos-Renyi (n 2^14-2^20, mean
+    degree 4-64) graphs, drawn from `seed`."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    graphs, desc = [], []
+    for i in range(32):
+        s, ef = int(rng.integers(14, 21)), int(rng.integers(8, 33))
+        graphs.append(ctx.generate_rmat(s, ef, seed=1000 + i))
+        desc.append("rmat%d/ef%d" % (s, ef))
+    for i in range(32):
+        n = int(round(2.0 ** rng.uniform(14, 20)))
+        d = float(rng.uniform(4, 64))
+        graphs.append(ctx.generate_er(n, d, seed=2000 + i))
+        desc.append("er%d/%.1f" % (n, d))
+    return graphs, desc
+
+
+def run_batch(api, ctx, args):
+    import numpy as np
+
+    graphs, desc = batch_graphs(ctx)
+    B = api.Batch(ctx, graphs)
+    t0 = time.perf_counter()
+    chosen = B.autotune()
+    log("autotune %.1fs: %s" % (time.perf_counter() - t0, chosen))
+    arcs = [len(c) for _, c in graphs]
+
+    def step():
+        it_pr, _ = B.pagerank(tolerance=1e-6, max_iter=1000)
+        it_cc, _ = B.cc(max_iter=1000)
+        return it_pr, it_cc
+
+    for _ in range(args.warmup):
+        step()
+    work = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        it_pr, it_cc = step()
+        work += sum(a * int(i) for a, i in zip(arcs, it_pr)) + sum(a * int(i) for a, i in zip(arcs, it_cc))
+    secs = time.perf_counter() - t0
+    gteps = work / secs / 1e9
+    log("timed %d steps: %.3f s, %.1f GTEPS" % (args.steps, secs, gteps))
+    extra = {"autotuned": chosen}
+    if not args.no_check:
+        # PageRank: every graph within 1e-6 L1 of the FP64 power iteration;
+        # CC: the Erdos-Renyi half (symmetric) against scipy components, the
+        # RMAT half against the partitioned BSP path of this library
+        worst = 0.0
+        cc_ok = True
+        for i, (rp, col) in enumerate(graphs):
+            ref = fp64_pagerank(rp, col, int(it_pr[i]))
+            worst = max(worst, float(np.abs(B.result(i, api.PR_PULL).astype(np.float64) - ref).sum()))
+            lab = B.result(i, api.CC)
+            if desc[i].startswith("er"):
+                cc_ok &= bool(np.array_equal(lab, cc_labels_scipy(rp, col)))
+            else:
+                G = api.Graph.from_csr(ctx, rp, col, C=1, M=2)
+                S = api.State(G, api.CC)
+                S.init(sched=api.SCHED_HEURISTIC)
+                S.run()
+                cc_ok &= bool(np.array_equal(lab, S.result()))
+                S.free()
+                G.free()
+        extra["pr_fp64_l1_max"] = worst
+        extra["cc_labels_exact"] = bool(cc_ok)
+        log("batch checks: PR worst L1 %.3e, CC exact %s" % (worst, cc_ok))
+    B.free()
+    cpu = None
+    if not args.no_cpu:
+        import oracle_check
+        model, nt = cpu_threads(args)
+        cpu = oracle_check.cpu_baseline_pull(graphs, nt)
+        cpu.update(cores=nt, cpu_model=model)
+    config = {"workload": "batched PageRank pull (FP32 Kahan, FP64 residual, tol 1e-6) + CC over 64 graphs "
+                          "(32 RMAT scale 14-20 EF 8-32, 32 Erdos-Renyi n 2^14-2^20 degree 4-64, seed 4) in "
+                          "persistent launches with hipGraph-captured iterations",
+              "graphs": len(graphs), "arcs": int(sum(arcs)), "n": int(sum(len(r) - 1 for r, _ in graphs)),
+              "gteps_definition": "sum over graphs of arcs x iterations (PR + CC) / time"}
+    return line(args, gteps, "GTEPS", secs * 1e3 / args.steps, "f32", config, None, cpu, extra)
+
+
+def run_learned(api, ctx, args):
+    scale = args.scale if args.scale != 24 else 26
+    sp = schedule_speedup(api, ctx, scale, args.parts, ef=args.ef, weighted=True, reps=max(1, args.steps),
+                          sym_cc=False)
+    ratios = [sp[k]["learned_vs_plain"] for k in ("pr", "cc", "sssp")]
+    geo = math.exp(sum(math.log(r) for r in ratios) / len(ratios))
+    ms = sum(sp[k]["ms"]["learned"] for k in ("pr", "cc", "sssp"))
+    config = {"workload": "learned vs heuristic vs plain schedule: reference PageRank, CC, weighted SSSP from "
+                          "gid 0 on RMAT-%d EF%d seed 1 (weights [1,255] seed 2), 1 compute + %d memory "
+                          "partitions; best of %d solves per schedule" % (scale, args.ef, args.parts,
+                                                                         max(1, args.steps)),
+              "arcs": sp["pr"]["arcs"]}
+    extra = {"learned_vs_plain": sp}
+    cpu = {"value": None, "unit": "x", "cores": 0, "kind": "port",
+           "sample": "none: the value is a GPU-to-GPU schedule ratio"}
+    return line(args, geo, "x (learned / plain, geometric mean of PR, CC, SSSP)", ms, "f32+u32", config, None,
+                cpu, extra)
 
 
 def main():
@@ -142,12 +542,13 @@ def main():
     ap.add_argument("--ef", type=int, default=16)
     ap.add_argument("--parts", type=int, default=4)
     ap.add_argument("--compute", type=int, default=1)
-    ap.add_argument("--workload", default="pr_pull", choices=["pr_pull", "cc", "sssp", "pr"])
+    ap.add_argument("--delta", type=int, default=64, help="SSSP delta-stepping bucket width")
+    ap.add_argument("--workload", default="pr_pull", choices=["pr_pull", "cc", "sssp", "pr", "batch", "learned"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-check", action="store_true", help="skip the FP64 / bit-exact check")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 / exactness check")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--speedup-scale", type=int, default=22,
-                    help="RMAT scale of the learned-vs-plain schedule leg (0 = skip)")
+                    help="RMAT scale of the learned-vs-plain schedule leg of pr_pull (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a cold device copy of MIB MiB (PMC counter calibration)")
     args = ap.parse_args()
@@ -160,103 +561,12 @@ def main():
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     from grudon_amd import api
 
     ctx = api.Context(0)
-    t0 = time.perf_counter()
-    sym = args.workload == "cc"
-    rp, col = ctx.generate_rmat(args.scale, args.ef, seed=1, symmetrize=sym)
-    t_gen = time.perf_counter() - t0
-    wts = None
-    if args.workload == "sssp":
-        sym = True
-        rp, col = ctx.generate_rmat(args.scale, args.ef, seed=1, symmetrize=True)
-    t1 = time.perf_counter()
-    G = api.Graph.from_csr(ctx, rp, col, wts, C=args.compute, M=args.parts)
-    t_part = time.perf_counter() - t1
-    m = G.m
-    log("graph: n=%d m=%d proxies=%d cross=%d gen %.1fs partition+upload %.1fs device %.2f GB"
-        % (G.n, m, G.proxies, G.cross, t_gen, t_part, G.device_bytes / 1e9))
-
-    algo = {"pr_pull": api.PR_PULL, "cc": api.CC, "sssp": api.SSSP, "pr": api.PR}[args.workload]
-    S = api.State(G, algo)
-    sched = api.SCHED_HEURISTIC
-
-    def one_step(time_kernel):
-        S.init(max_iter=1000, tolerance=1e-6, sched=sched, time_kernel=time_kernel)
-        return S.run()
-
-    for _ in range(args.warmup):
-        one_step(False)
-    t_start = time.perf_counter()
-    iters = 0
-    kms = 0.0
-    klaunch = 0
-    kbytes = 0
-    for _ in range(args.steps):
-        st = one_step(True)
-        iters += st["iterations"]
-        kms += st["kernel_ms"]
-        klaunch += st["kernel_launches"]
-        kbytes += st["kernel_bytes"]
-    t_total = time.perf_counter() - t_start
-    last = st
-    gteps = m * iters / t_total / 1e9
-    ms_per_step = t_total * 1e3 / args.steps
-    log("timed %d steps: %.3f s, %d iterations total, %.2f ms/step, %.1f GTEPS" %
-        (args.steps, t_total, iters, ms_per_step, gteps))
-
-    achieved = (kbytes / klaunch) / (kms / klaunch * 1e-3) / 1e9 if klaunch else 0.0
-    kname = {"pr_pull": "k_prp_pull", "cc": "k_pull", "sssp": "k_pull", "pr": "k_pull"}[args.workload]
-    traffic = traffic_from_profiles(kname + "_rmat%d" % args.scale)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": kname, "launches": klaunch,
-                "avg_launch_us": round(kms / klaunch * 1e3, 2) if klaunch else None,
-                "algorithmic_bytes_per_launch": int(kbytes / klaunch) if klaunch else None}
-
-    extra = {}
-    if not args.no_check and args.workload == "pr_pull":
-        import oracle_check
-        res = oracle_check.pr_pull_l1(rp, col, S.result(), last["iterations"], args.cpu_threads)
-        extra["fp64_l1"] = res
-        log("FP64 oracle L1 = %.3e" % res)
-
-    # free the headline workload before the schedule-comparison leg
-    S.free()
-    G.free()
-    if args.speedup_scale:
-        extra["learned_vs_plain"] = schedule_speedup(api, ctx, args.speedup_scale, args.parts)
-        log("learned vs plain: %s" % json.dumps(extra["learned_vs_plain"]))
-
-    cpu = None
-    if not args.no_cpu:
-        import oracle_check
-        model, cores = cpu_info()
-        nthreads = args.cpu_threads or min(cores, 64)
-        cpu = oracle_check.cpu_baseline(rp, col, args.compute, args.parts, nthreads, algo)
-        cpu["cores"] = nthreads
-        cpu["cpu_model"] = model
-        log("cpu baseline: %s" % json.dumps(cpu))
-
-    out = {
-        "metric": METRIC, "value": round(gteps, 2), "unit": "GTEPS", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32" if algo in (0, 3) else "u32",
-        "data": "synthetic",
-        "config": {"workload": {"pr_pull": "PageRank pull (FP32 Kahan, FP64 dangling+residual, d=0.85, tol 1e-6)",
-                                "cc": "CC label propagation", "sssp": "SSSP hop count from gid 0",
-                                "pr": "reference delta-push PageRank"}[args.workload]
-                   + ", RMAT-%d EF%d seed 1%s, %d compute + %d memory partitions resident in HBM, "
-                     "on-device bitvector master/mirror sync" % (args.scale, args.ef, " symmetrized" if sym else "",
-                                                                   args.compute, args.parts),
-                   "n": G.n, "arcs": m, "iterations_per_step": last["iterations"],
-                   "schedule": "heuristic", "exchange_bytes_per_step": last["bytes_exchanged"]},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
-    out.update(extra)
+    run = {"pr_pull": run_pagerank, "pr": run_pagerank, "cc": run_cc, "sssp": run_sssp, "batch": run_batch,
+           "learned": run_learned}[args.workload]
+    out = run(api, ctx, args)
     print(json.dumps(out), flush=True)
 
 

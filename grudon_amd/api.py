@@ -169,6 +169,14 @@ class Context:
                "grd_generate_er")
         return rp, col[: m.value].copy()
 
+    def edge_weights(self, rp, col, wseed):
+        """Seeded arc weights in [1, 255], equal for (u, v) and (v, u)."""
+        rp = np.ascontiguousarray(rp, dtype=np.uint32)
+        col = np.ascontiguousarray(col, dtype=np.uint32)
+        w = np.zeros(max(len(col), 1), dtype=np.uint32)
+        _check(lib().grd_edge_weights(self.h, len(rp) - 1, _ptr(rp), _ptr(col), wseed, _ptr(w)), "grd_edge_weights")
+        return w[: len(col)]
+
     # selector
     def selector_load(self, w_bf16, min_margin=0.0):
         w = np.ascontiguousarray(w_bf16, dtype=np.uint16)

@@ -3,16 +3,22 @@
 // Device layout (all partitions of one graph resident in one GPU's HBM):
 //
 //   compute slots  [0, S): compute partition c, then mirror group m, each
-//                  (c, m) group padded to a multiple of 64 slots.  Within a
-//                  group vertices are in ascending gid, so the group is
-//                  exactly the "own-on-m" part of the reference's compute
-//                  sTranslationTable[m] (DistributedGraph.cpp:701-717) and the
-//                  wire index of a slot is slot - cbase(c, m).
+//                  (c, m) group padded to a multiple of 64 slots.  A group
+//                  holds the "own-on-m" vertices of the reference's compute
+//                  sTranslationTable[m] (DistributedGraph.cpp:701-717),
+//                  ordered by out-degree, highest first, ties by ascending
+//                  gid; the wire index of a slot is slot - cbase(c, m).
 //   memory proxies [0, P): memory partition m, then master c, then the own
-//                  region (ascending gid) and the cross-mirror region
+//                  region (same order as the compute group, so bitvector
+//                  words map one to one) and the cross-mirror region
 //                  (ascending gid), each padded to 64.  The (m, c) own+cross
-//                  range is the reference's memory-side translation table for
-//                  master c in the same order (DistributedGraph.cpp:644-660).
+//                  range holds the reference's memory-side translation table
+//                  for master c (DistributedGraph.cpp:644-660).
+//
+// The degree order is a permutation of the reference's wire order inside each
+// bitvector (same length, same popcount, same bytes moved).  The pull gathers
+// a source's value once per out-arc, so it packs the most gathered values of a
+// memory partition into the fewest cache lines.
 //
 // Because every pair region starts on a 64-bit word, a pair's bitvector is a
 // word-aligned slice of the frontier / dirty bitmaps; bitvector sync copies

@@ -8,13 +8,14 @@
 //                                            Partitioner.hpp:33, :73-105, :177
 //   cross mirrors of m = {v : u -> v, mirror(u) = m, mirror(v) != m}
 //                                            DistributedGraph.cpp:345-349, :367-371
-//   memory lids: own (ascending gid) then cross (ascending gid)
+//   memory lids: own then cross (ascending gid)
 //                                            DistributedGraph.cpp:517-525, :579-588
 //   translation tables: compute side own-on-m then cross-on-m, memory side
 //   own then cross per master, both ascending  DistributedGraph.cpp:644-660, :701-734
 // The resulting slots/proxies are stored in the padded, pair-grouped order
-// described in grd_internal.hpp.  Local CSR rows keep the global row order
-// (DistributedGraph.cpp:590-611); the pull CSC rows are sorted by source.
+// described in grd_internal.hpp: own vertices of a (master, mirror) pair by
+// out-degree rank, cross mirrors by gid.  grd_graph_table still returns the
+// reference's ascending tables.  The pull CSC rows are sorted by source.
 #include <algorithm>
 #include <atomic>
 #include <cstring>
@@ -225,15 +226,30 @@ static grd_status build_impl(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const
   g->X = X;
 
   // own slot / own proxy per gid
+  // (position in a group: out-degree rank, highest first, ties by gid)
   std::vector<uint32_t> own_proxy(n);
   g->h_gid_slot.assign(n, kNone);
   {
-    std::vector<uint32_t> ctr((size_t)C * M, 0);
-    for (uint32_t v = 0; v < n; v++) {
-      uint32_t c = v % C, mm = mir[v];
-      uint32_t r = ctr[(size_t)c * M + mm]++;
-      g->h_gid_slot[v] = g->cbase[(size_t)c * M + mm] + r;
-      own_proxy[v] = g->obase[(size_t)mm * C + c] + r;
+    const size_t NG = (size_t)C * M;
+    std::vector<uint64_t> gstart(NG + 1, 0);
+    for (size_t k = 0; k < NG; k++) gstart[k + 1] = gstart[k] + g->ccnt[k];
+    // key = (~deg << 32) | gid sorts by degree descending, then gid ascending
+    std::vector<uint64_t> key(n);
+    {
+      std::vector<uint64_t> ctr(gstart.begin(), gstart.end() - 1);
+      for (uint32_t v = 0; v < n; v++)
+        key[ctr[(size_t)(v % C) * M + mir[v]]++] = ((uint64_t)(~g->h_deg[v]) << 32) | v;
+    }
+    parallel_for(T, NG, 1, [&](uint64_t b, uint64_t e) {
+      for (uint64_t k = b; k < e; k++) std::sort(key.begin() + gstart[k], key.begin() + gstart[k + 1]);
+    });
+    for (size_t k = 0; k < NG; k++) {
+      const size_t c = k / M, mm = k % M;
+      for (uint64_t i = gstart[k]; i < gstart[k + 1]; i++) {
+        const uint32_t v = (uint32_t)key[i], r = (uint32_t)(i - gstart[k]);
+        g->h_gid_slot[v] = g->cbase[k] + r;
+        own_proxy[v] = g->obase[mm * C + c] + r;
+      }
     }
   }
   // cross proxy per (m, gid)
