@@ -4,7 +4,9 @@
 //
 // Every vertex is active every iteration, so both exchange directions use the
 // dense form: the compute partitions put r/deg of their vertices into the
-// memory partitions' own proxies, each memory partition pulls over its local
+// memory partitions' own proxies (the update kernel stores them there
+// directly: the own-proxy words of a (master, mirror) pair map one to one onto
+// the pair's slot words), each memory partition pulls over its local
 // CSC (FP32, Kahan-compensated per row) and writes one partial sum per proxy
 // (own and cross mirrors) into its reduce buffer, and the compute partitions
 // combine the partial sums of each vertex's proxies (own first, then cross
@@ -20,26 +22,15 @@ namespace grd {
 grd_status prp_pull_launch(grd_state *s);
 grd_status prp_pull_fixup(grd_state *s);
 
-// dense receive: memory own proxies <- compute contributions
-__global__ void __launch_bounds__(kBlock) k_prp_recv(uint32_t nw, const uint32_t *__restrict__ word_dst,
-                                                     const float *__restrict__ valB, float *__restrict__ mprop)
-{
-  for (uint32_t w = blockIdx.x * kWavesPerBlock + wave_in_block(); w < nw; w += gridDim.x * kWavesPerBlock) {
-    uint32_t pw = word_dst[w];
-    if (pw == 0xFFFFFFFFu) continue;
-    mprop[pw * 64 + lane_id()] = valB[w * 64 + lane_id()];
-  }
-}
-
 // compute update: combine proxy partials, new rank, FP64 residual and
-// dangling mass, and the next contributions (dense put source)
+// dangling mass, and the next contributions put into the own proxies
 __global__ void __launch_bounds__(kBlock) k_prp_update(uint32_t nw, const uint32_t *__restrict__ slot_gid,
                                                        const uint32_t *__restrict__ deg,
                                                        const uint32_t *__restrict__ word_dst,
                                                        const uint32_t *__restrict__ xptr,
                                                        const uint32_t *__restrict__ xlist,
                                                        const float *__restrict__ valR, float *__restrict__ r,
-                                                       float *__restrict__ valB, float damping, double inv_n,
+                                                       float *__restrict__ mprop, float damping, double inv_n,
                                                        int parity, Counters *cnt)
 {
   const double Dcur = cnt->dangling[parity];
@@ -58,7 +49,7 @@ __global__ void __launch_bounds__(kBlock) k_prp_update(uint32_t nw, const uint32
     uint32_t dg = deg[s];
     if (dg == 0) dang += (double)rn;
     r[s] = rn;
-    valB[s] = dg ? rn / (float)dg : 0.0f;
+    mprop[pw * 64 + lane_id()] = dg ? rn / (float)dg : 0.0f;
   }
   res = block_sum(res);
   dang = block_sum(dang);
@@ -69,14 +60,16 @@ __global__ void __launch_bounds__(kBlock) k_prp_update(uint32_t nw, const uint32
 }
 
 __global__ void __launch_bounds__(kBlock) k_prp_init(uint32_t S, const uint32_t *__restrict__ slot_gid,
-                                                     const uint32_t *__restrict__ deg, float *__restrict__ r,
-                                                     float *__restrict__ valB, float r0)
+                                                     const uint32_t *__restrict__ deg,
+                                                     const uint32_t *__restrict__ word_dst, float *__restrict__ r,
+                                                     float *__restrict__ mprop, float r0)
 {
   for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < S; s += gridDim.x * kBlock) {
     bool valid = slot_gid[s] != 0xFFFFFFFFu;
     uint32_t dg = deg[s];
     r[s] = valid ? r0 : 0.0f;
-    valB[s] = (valid && dg) ? r0 / (float)dg : 0.0f;
+    uint32_t pw = word_dst[s >> 6];
+    if (pw != 0xFFFFFFFFu) mprop[pw * 64 + (s & 63)] = (valid && dg) ? r0 / (float)dg : 0.0f;
   }
 }
 
@@ -95,7 +88,7 @@ grd_status prp_init(grd_state *s)
   hipStream_t st = g->ctx->stream;
   const float r0 = (float)(1.0 / (double)g->n);
   hipLaunchKernelGGL(k_prp_init, dim3(std::min<uint32_t>(4096, g->S / kBlock + 1)), dim3(kBlock), 0, st, g->S,
-                     g->slot_gid, g->slot_deg, s->pr, (float *)s->valB, r0);
+                     g->slot_gid, g->slot_deg, g->word_dst, s->pr, (float *)s->mprop, r0);
   uint64_t sinks = 0;
   for (uint32_t v = 0; v < g->n; v++) sinks += (g->h_deg[v] == 0);
   // D0 = sum over sinks of the FP32 initial rank, accumulated in FP64
@@ -116,15 +109,15 @@ static uint32_t prp_grid(const grd_graph *g)
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((nwS + 3) / 4, (uint64_t)g->ctx->num_cus * 16));
 }
 
-// broadcast step: dense put of r/deg into the memory partitions' own proxies
+// broadcast step: the dense put of r/deg into the memory partitions' own
+// proxies was stored by the previous update (or init); what is left is the
+// iteration's counter reset
 grd_status prp_broadcast(grd_state *s)
 {
   grd_graph *g = s->g;
   hipStream_t st = g->ctx->stream;
   const int parity = (int)(s->iteration & 1);
   hipLaunchKernelGGL(k_prp_counters, dim3(1), dim3(64), 0, st, s->cnt, parity, -1.0);
-  hipLaunchKernelGGL(k_prp_recv, dim3(prp_grid(g)), dim3(kBlock), 0, st, g->S / 64, g->word_dst,
-                     (const float *)s->valB, (float *)s->mprop);
   GRD_HIP(hipGetLastError());
   s->bytes += (uint64_t)g->n * 4;
   return GRD_OK;
@@ -156,7 +149,7 @@ grd_status prp_update(grd_state *s)
   grd_graph *g = s->g;
   const int parity = (int)(s->iteration & 1);
   hipLaunchKernelGGL(k_prp_update, dim3(prp_grid(g)), dim3(kBlock), 0, g->ctx->stream, g->S / 64, g->slot_gid,
-                     g->slot_deg, g->word_dst, g->xptr, g->xlist, (const float *)s->valR, s->pr, (float *)s->valB,
+                     g->slot_deg, g->word_dst, g->xptr, g->xlist, (const float *)s->valR, s->pr, (float *)s->mprop,
                      (float)s->prm.damping, 1.0 / (double)g->n, parity, s->cnt);
   GRD_HIP(hipGetLastError());
   return GRD_OK;
