@@ -82,7 +82,7 @@ static grd_status finish_kernel_timing(grd_state *s, uint64_t frontier_k)
   GRD_HIP(hipEventElapsedTime(&ms, s->kev[0], s->kev[1]));
   s->kern_ms += ms;
   s->kern_launches++;
-  if (s->kern_pending == 2) s->kern_bytes += 12 * s->hcnt->edges + 16 * frontier_k;
+  if (s->kern_pending == 2) s->kern_bytes += 12 * (s->hcnt->edges + s->hcnt->heavy_edges) + 16 * frontier_k;
   else s->kern_bytes += s->kern_pending_bytes;
   s->kern_pending = 0;
   return GRD_OK;

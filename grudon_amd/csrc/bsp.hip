@@ -96,10 +96,14 @@ static grd_status word_ranks(hipStream_t st, const uint64_t *w, uint32_t nw, uin
   return GRD_OK;
 }
 
+// Grid of the word-parallel passes (exchange, update): up to 64 workgroups
+// per CU, so each wave walks only a few 64-bit bitmap words.  A word costs a
+// short chain of dependent loads; the latency is hidden by the waves in
+// flight, while the per-workgroup counter atomics stay few.
 static inline uint32_t wave_grid(const grd_ctx *ctx, uint64_t nwords)
 {
   uint64_t want = (nwords + kWavesPerBlock - 1) / kWavesPerBlock;
-  uint64_t cap = (uint64_t)ctx->num_cus * 16;
+  uint64_t cap = (uint64_t)ctx->num_cus * 64;
   return (uint32_t)std::max<uint64_t>(1, std::min(want, cap));
 }
 
@@ -210,14 +214,13 @@ __global__ void __launch_bounds__(kBlock) k_bcast_recv(uint32_t nw, const uint64
 // H are processed together: the block scans their degrees and every thread
 // walks the concatenated edge range (coalesced col reads), locating its
 // source by binary search in the LDS prefix array.  Vertices with degree
-// >= H are appended to the heavy list and handled by k_push_heavy, where the
-// whole grid shares each heavy vertex.
+// >= kHeavyDeg are appended to the heavy list as chunks of kHeavyDeg arcs and
+// handled by k_push_heavy, one workgroup per chunk.
 template <class Op>
 __global__ void __launch_bounds__(kBlock) k_push(const uint32_t *__restrict__ AL, const uint32_t *__restrict__ nact,
                                                  const uint32_t *__restrict__ rp, const uint32_t *__restrict__ col,
                                                  const uint32_t *__restrict__ wt, const typename Op::V *__restrict__ mprop,
-                                                 typename Op::V *upd, uint64_t *dirty, uint32_t *heavy, Counters *cnt,
-                                                 uint32_t H)
+                                                 typename Op::V *upd, uint64_t *dirty, uint2 *heavy, Counters *cnt)
 {
   using V = typename Op::V;
   __shared__ uint32_t s_off[kBlock + 1];
@@ -234,9 +237,10 @@ __global__ void __launch_bounds__(kBlock) k_push(const uint32_t *__restrict__ AL
       beg = rp[p];
       deg = rp[p + 1] - beg;
       v = mprop[p];
-      if (deg >= H) {
-        unsigned long long slot = atomicAdd(&cnt->heavy, 1ull);
-        heavy[slot] = p;
+      if (deg >= kHeavyDeg) {
+        const uint32_t nch = (deg + kHeavyDeg - 1) / kHeavyDeg;
+        unsigned long long slot = atomicAdd(&cnt->heavy, (unsigned long long)nch);
+        for (uint32_t c = 0; c < nch; c++) heavy[slot + c] = make_uint2(p, c * kHeavyDeg);
         atomicAdd(&cnt->heavy_edges, (unsigned long long)deg);
         deg = 0;
       }
@@ -267,19 +271,19 @@ __global__ void __launch_bounds__(kBlock) k_push(const uint32_t *__restrict__ AL
 }
 
 template <class Op>
-__global__ void __launch_bounds__(kBlock) k_push_heavy(const uint32_t *__restrict__ heavy, const Counters *cnt,
+__global__ void __launch_bounds__(kBlock) k_push_heavy(const uint2 *__restrict__ heavy, const Counters *cnt,
                                                        const uint32_t *__restrict__ rp, const uint32_t *__restrict__ col,
                                                        const uint32_t *__restrict__ wt,
                                                        const typename Op::V *__restrict__ mprop, typename Op::V *upd,
                                                        uint64_t *dirty)
 {
-  const uint32_t nh = (uint32_t)cnt->heavy;
-  for (uint32_t h = 0; h < nh; h++) {
-    uint32_t p = heavy[h];
-    uint32_t b = rp[p], e = rp[p + 1];
-    typename Op::V v = mprop[p];
-    for (uint32_t x = b + blockIdx.x * kBlock + threadIdx.x; x < e; x += gridDim.x * kBlock)
-      relax<Op>(upd, dirty, col[x], Op::edge(v, wt ? wt[x] : 1u));
+  const uint32_t nch = (uint32_t)cnt->heavy;
+  for (uint32_t h = blockIdx.x; h < nch; h += gridDim.x) {
+    const uint2 c = heavy[h];
+    const uint32_t end = rp[c.x + 1];
+    const uint32_t b = rp[c.x] + c.y, e = min(end, b + kHeavyDeg);
+    const typename Op::V v = mprop[c.x];
+    for (uint32_t x = b + threadIdx.x; x < e; x += kBlock) relax<Op>(upd, dirty, col[x], Op::edge(v, wt ? wt[x] : 1u));
   }
 }
 
@@ -325,13 +329,17 @@ __global__ void __launch_bounds__(kBlock) k_reduce_recv(uint32_t nw, const uint6
   const int wpb = kWavesPerBlock;
   for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
     uint64_t word = bv[w];
-    if (!word) continue;
+    if (!word) continue;  // wave-uniform
+    uint32_t s = 0xFFFFFFFFu;
+    typename Op::V x{};
     if ((word >> lane_id()) & 1ull) {
       uint32_t p = w * 64 + lane_id();
-      typename Op::V x = dense ? val[p] : val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())];
-      uint32_t s = proxy_slot[p];
-      if (s != 0xFFFFFFFFu) relax<Op, true>(upd, Dc, s, x);  // aggregate(): minUpdate / addUpdate
+      x = dense ? val[p] : val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())];
+      s = proxy_slot[p];
     }
+    // aggregate(): minUpdate / addUpdate, then the dirty marks of the wave
+    const bool mark = s != 0xFFFFFFFFu && relax_nomark<Op, true>(upd, s, x);
+    mark_wave(Dc, s, mark);
   }
 }
 
@@ -538,8 +546,10 @@ grd_status state_alloc(grd_state *s)
   GRD_TRY(m.alloc_n(&s->wrankR, nwP));
   GRD_TRY(m.alloc_n(&s->totals, 64));
   GRD_TRY(m.alloc_n(&s->AL, S));
-  GRD_TRY(m.alloc_n(&s->heavy, S));
+  // heavy chunks: sum over heavy vertices of ceil(deg / kHeavyDeg) <= 2 m / kHeavyDeg
+  GRD_TRY(m.alloc_n(&s->heavy, 2 * (g->m / kHeavyDeg) + 64));
   GRD_TRY(m.alloc(&s->partial, (uint64_t)std::max<uint32_t>(1, g->n_pblk) * 8));
+  if (s->algo == GRD_PR_PULL) GRD_TRY(m.alloc_n(&s->red, 2ull * prp_grid(g)));
   GRD_TRY(m.alloc_n(&s->cnt, 1));
   GRD_HIP(hipHostMalloc((void **)&s->hcnt, sizeof(Counters), hipHostMallocDefault));
   memset(s->hcnt, 0, sizeof(Counters));
@@ -593,19 +603,21 @@ grd_status state_init(grd_state *s)
   GRD_HIP(hipGetLastError());
   GRD_TRY(op_zero_counters(s));
   s->hi = (s->algo == GRD_SSSP && s->prm.delta) ? (uint64_t)s->prm.delta : ~0ull;
-  // initial frontier statistics for the first schedule decision
+  // initial frontier statistics for the first schedule decision: PR starts
+  // from every vertex with out-edges, CC from every vertex, SSSP from the source
   uint64_t fr = 0, fe = 0;
   uint32_t mx = 0;
-  for (uint32_t v = 0; v < g->n; v++) {
-    bool in = false;
-    if (s->algo == GRD_PR) in = g->h_deg[v] != 0;
-    else if (s->algo == GRD_CC) in = true;
-    else in = (v == s->prm.source);
-    if (in) {
-      fr++;
-      fe += g->h_deg[v];
-      mx = std::max(mx, g->h_deg[v]);
-    }
+  if (s->algo == GRD_PR) {
+    fr = g->n - g->n_sinks;
+    fe = g->m;
+    mx = g->max_deg;
+  } else if (s->algo == GRD_CC) {
+    fr = g->n;
+    fe = g->m;
+    mx = g->max_deg;
+  } else {
+    fr = 1;
+    fe = mx = g->h_deg[s->prm.source];
   }
   s->last_frontier = fr;
   s->prev_frontier = fr;
@@ -672,18 +684,17 @@ static grd_status traverse_t(grd_state *s, int option)
     GRD_TRY(bsp_pull_fixup<Op>(s));
     s->edges += g->m;  // a pull examines every arc of the memory partitions
   } else {
-    const uint32_t H = 4096;
     uint32_t want = (uint32_t)std::min<uint64_t>((s->last_frontier + kBlock - 1) / kBlock, (uint64_t)ctx->num_cus * 8);
     uint32_t grid = std::max<uint32_t>(1, want);
     if (timed) GRD_HIP(hipEventRecord(s->kev[0], st));
     hipLaunchKernelGGL((k_push<Op>), dim3(grid), dim3(kBlock), 0, st, s->AL, s->totals + 0, g->rp, g->col, wt,
-                       (const V *)s->mprop, (V *)s->mupd, s->Du, s->heavy, s->cnt, H);
-    if (timed) {
-      GRD_HIP(hipEventRecord(s->kev[1], st));
-      s->kern_pending = 2;  // bytes from the edge counter after the iteration
-    }
+                       (const V *)s->mprop, (V *)s->mupd, s->Du, s->heavy, s->cnt);
     hipLaunchKernelGGL((k_push_heavy<Op>), dim3(ctx->num_cus * 4), dim3(kBlock), 0, st, s->heavy, s->cnt, g->rp, g->col,
                        wt, (const V *)s->mprop, (V *)s->mupd, s->Du);
+    if (timed) {
+      GRD_HIP(hipEventRecord(s->kev[1], st));
+      s->kern_pending = 2;  // bytes from the edge counters after the iteration
+    }
   }
   GRD_HIP(hipGetLastError());
   return GRD_OK;

@@ -74,7 +74,7 @@ struct Counters {
   unsigned long long dirty;           // dirty memory proxies at reduce
   unsigned long long pending;         // SSSP delta-stepping: pending vertices
   unsigned long long improved;        // compute vertices improved by update
-  unsigned long long heavy;           // push: heavy vertices appended this iteration
+  unsigned long long heavy;           // push: heavy-vertex chunks appended this iteration
   unsigned long long edges;           // edges examined by the traversal
   unsigned int frontier_maxdeg;
   unsigned int pend_min;
@@ -94,6 +94,7 @@ struct PullLong {
 };
 constexpr uint32_t kPullNnz = 2048;     // arcs per block (8 per thread)
 constexpr uint32_t kPullMaxRows = 1024;  // rows per block
+constexpr uint32_t kHeavyDeg = 4096;     // push: out-degree handled by k_push_heavy, in chunks of this many arcs
 
 }  // namespace grd
 
@@ -127,6 +128,8 @@ struct grd_graph {
   std::vector<uint32_t> h_deg;                  // gid -> out-degree
   std::vector<uint32_t> h_proxy_gid;            // proxy -> gid (kNone for padding)
   double avg_degree = 0.0;
+  uint64_t n_sinks = 0;   // vertices with out-degree 0
+  uint32_t max_deg = 0;   // largest out-degree
   // device arrays
   grd::DevMem mem;
   uint32_t *slot_gid = nullptr;    // [S]
@@ -175,8 +178,10 @@ struct grd_state {
   void *valB = nullptr, *valR = nullptr;
   // scan scratch
   uint32_t *blk = nullptr, *wrankB = nullptr, *wrankR = nullptr, *totals = nullptr;
-  uint32_t *AL = nullptr, *heavy = nullptr;
+  uint32_t *AL = nullptr;
+  uint2 *heavy = nullptr;  // push: (proxy, first arc offset) of each heavy-vertex chunk
   void *partial = nullptr;  // pull: one 8-byte partial per row block (long-row segments)
+  double *red = nullptr;    // PR pull update: (residual, dangling) per workgroup
   grd::Counters *cnt = nullptr;   // device
   grd::Counters *hcnt = nullptr;  // pinned host
   // snapshot copies
@@ -215,6 +220,7 @@ grd_status op_zero_counters(grd_state *s);
 grd_status op_read_counters(grd_state *s);
 // prpull.hip
 grd_status prp_init(grd_state *s);
+uint32_t prp_grid(const grd_graph *g);  // workgroups of the PR pull update
 grd_status prp_iteration(grd_state *s);
 grd_status prp_broadcast(grd_state *s);
 grd_status prp_traverse(grd_state *s);

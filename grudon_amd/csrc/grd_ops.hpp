@@ -76,6 +76,49 @@ __device__ __forceinline__ void relax(typename Op::V *upd, uint64_t *dirty, uint
   }
 }
 
+// relax without the dirty mark; returns whether the destination must be
+// marked (same rules as relax)
+template <class Op, bool kAlwaysMark = false>
+__device__ __forceinline__ bool relax_nomark(typename Op::V *upd, uint32_t dst, typename Op::V cand)
+{
+  if constexpr (Op::kMin) {
+    uint32_t cur = __atomic_load_n(upd + dst, __ATOMIC_RELAXED);
+    if (cand < cur) return cand < atomicMin(upd + dst, cand);
+    return false;
+  } else {
+    float old = atomicAdd(upd + dst, cand);
+    return kAlwaysMark || old == 0.0f;
+  }
+}
+
+// Sets bit i for every lane with pred.  Called by the whole wave: when all
+// marked bits fall into one 64-bit word (a wave over a word of own proxies
+// maps onto one word of slots) they are OR-ed across the wave and written by
+// one atomic instead of up to 64 atomics on the same word.
+__device__ __forceinline__ void mark_wave(uint64_t *bits, uint32_t i, bool pred)
+{
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  const uint32_t wl = __shfl(i >> 6, leader, kWave);
+  const bool same = !pred || (i >> 6) == wl;
+  if (__ballot(!same) == 0) {
+    uint32_t lo = 0, hi = 0;
+    if (pred) {
+      if ((i & 63) < 32) lo = 1u << (i & 31);
+      else hi = 1u << (i & 31);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lo |= __shfl_xor(lo, o, kWave);
+      hi |= __shfl_xor(hi, o, kWave);
+    }
+    if (lane_id() == leader) atomicOr((unsigned long long *)(bits + wl), ((unsigned long long)hi << 32) | lo);
+  } else if (pred) {
+    set_bit(bits, i);
+  }
+}
+
 // pull launchers (pull.hip)
 template <class Op>
 grd_status bsp_pull_launch(grd_state *s, const uint32_t *cw);

@@ -159,7 +159,12 @@ static grd_status build_impl(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const
       return fail(GRD_ERR_ARG, "graph: column index out of range");
     }
   g->h_deg.resize(n);
-  for (uint32_t v = 0; v < n; v++) g->h_deg[v] = rp[v + 1] - rp[v];
+  for (uint32_t v = 0; v < n; v++) {
+    const uint32_t d = rp[v + 1] - rp[v];
+    g->h_deg[v] = d;
+    g->n_sinks += (d == 0);
+    g->max_deg = std::max(g->max_deg, d);
+  }
 
   // cross-node mirrors: one bitmap per memory partition
   std::vector<uint64_t> crossw((uint64_t)M * NW, 0);

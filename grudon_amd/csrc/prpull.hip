@@ -11,7 +11,8 @@
 // (own and cross mirrors) into its reduce buffer, and the compute partitions
 // combine the partial sums of each vertex's proxies (own first, then cross
 // mirrors in ascending partition order, again compensated).  The dangling
-// mass D and the L1 residual are accumulated in FP64.
+// mass D and the L1 residual are accumulated in FP64, per workgroup and then
+// in a fixed order by k_prp_finish.
 #include <algorithm>
 
 #include "grd_device.hpp"
@@ -31,7 +32,7 @@ __global__ void __launch_bounds__(kBlock) k_prp_update(uint32_t nw, const uint32
                                                        const uint32_t *__restrict__ xlist,
                                                        const float *__restrict__ valR, float *__restrict__ r,
                                                        float *__restrict__ mprop, float damping, double inv_n,
-                                                       int parity, Counters *cnt)
+                                                       int parity, const Counters *cnt, double *__restrict__ red)
 {
   const double Dcur = cnt->dangling[parity];
   const float base = (float)((1.0 - (double)damping) * inv_n + (double)damping * Dcur * inv_n);
@@ -54,8 +55,26 @@ __global__ void __launch_bounds__(kBlock) k_prp_update(uint32_t nw, const uint32
   res = block_sum(res);
   dang = block_sum(dang);
   if (threadIdx.x == 0) {
-    if (res != 0.0) atomicAdd(&cnt->residual, res);
-    if (dang != 0.0) atomicAdd(&cnt->dangling[parity ^ 1], dang);
+    red[2 * blockIdx.x] = res;
+    red[2 * blockIdx.x + 1] = dang;
+  }
+}
+
+// Sums the update's per-workgroup (residual, dangling) pairs in a fixed order,
+// so the convergence test does not depend on scheduling.
+__global__ void __launch_bounds__(kBlock) k_prp_finish(const double *__restrict__ red, uint32_t nblk, int parity,
+                                                       Counters *cnt)
+{
+  double res = 0.0, dang = 0.0;
+  for (uint32_t b = threadIdx.x; b < nblk; b += kBlock) {
+    res += red[2 * b];
+    dang += red[2 * b + 1];
+  }
+  res = block_sum(res);
+  dang = block_sum(dang);
+  if (threadIdx.x == 0) {
+    cnt->residual = res;
+    cnt->dangling[parity ^ 1] = dang;
   }
 }
 
@@ -89,10 +108,8 @@ grd_status prp_init(grd_state *s)
   const float r0 = (float)(1.0 / (double)g->n);
   hipLaunchKernelGGL(k_prp_init, dim3(std::min<uint32_t>(4096, g->S / kBlock + 1)), dim3(kBlock), 0, st, g->S,
                      g->slot_gid, g->slot_deg, g->word_dst, s->pr, (float *)s->mprop, r0);
-  uint64_t sinks = 0;
-  for (uint32_t v = 0; v < g->n; v++) sinks += (g->h_deg[v] == 0);
   // D0 = sum over sinks of the FP32 initial rank, accumulated in FP64
-  double d0 = (double)sinks * (double)r0;
+  double d0 = (double)g->n_sinks * (double)r0;
   hipLaunchKernelGGL(k_prp_counters, dim3(1), dim3(64), 0, st, s->cnt, 0, d0);
   GRD_HIP(hipGetLastError());
   s->iteration = 0;
@@ -103,23 +120,19 @@ grd_status prp_init(grd_state *s)
   return GRD_OK;
 }
 
-static uint32_t prp_grid(const grd_graph *g)
+// workgroups of the update: up to 64 per CU, a few slot words per wave
+uint32_t prp_grid(const grd_graph *g)
 {
   const uint32_t nwS = g->S / 64;
-  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((nwS + 3) / 4, (uint64_t)g->ctx->num_cus * 16));
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((nwS + 3) / 4, (uint64_t)g->ctx->num_cus * 64));
 }
 
 // broadcast step: the dense put of r/deg into the memory partitions' own
-// proxies was stored by the previous update (or init); what is left is the
-// iteration's counter reset
+// proxies was stored by the previous update (or init); only the byte count
+// of the exchange is left
 grd_status prp_broadcast(grd_state *s)
 {
-  grd_graph *g = s->g;
-  hipStream_t st = g->ctx->stream;
-  const int parity = (int)(s->iteration & 1);
-  hipLaunchKernelGGL(k_prp_counters, dim3(1), dim3(64), 0, st, s->cnt, parity, -1.0);
-  GRD_HIP(hipGetLastError());
-  s->bytes += (uint64_t)g->n * 4;
+  s->bytes += (uint64_t)s->g->n * 4;
   return GRD_OK;
 }
 
@@ -148,9 +161,12 @@ grd_status prp_update(grd_state *s)
 {
   grd_graph *g = s->g;
   const int parity = (int)(s->iteration & 1);
-  hipLaunchKernelGGL(k_prp_update, dim3(prp_grid(g)), dim3(kBlock), 0, g->ctx->stream, g->S / 64, g->slot_gid,
-                     g->slot_deg, g->word_dst, g->xptr, g->xlist, (const float *)s->valR, s->pr, (float *)s->mprop,
-                     (float)s->prm.damping, 1.0 / (double)g->n, parity, s->cnt);
+  const uint32_t grid = prp_grid(g);
+  hipLaunchKernelGGL(k_prp_update, dim3(grid), dim3(kBlock), 0, g->ctx->stream, g->S / 64, g->slot_gid, g->slot_deg,
+                     g->word_dst, g->xptr, g->xlist, (const float *)s->valR, s->pr, (float *)s->mprop,
+                     (float)s->prm.damping, 1.0 / (double)g->n, parity, s->cnt, s->red);
+  hipLaunchKernelGGL(k_prp_finish, dim3(1), dim3(kBlock), 0, g->ctx->stream, (const double *)s->red, grid, parity,
+                     s->cnt);
   GRD_HIP(hipGetLastError());
   return GRD_OK;
 }
