@@ -542,7 +542,7 @@ def main():
     ap.add_argument("--ef", type=int, default=16)
     ap.add_argument("--parts", type=int, default=4)
     ap.add_argument("--compute", type=int, default=1)
-    ap.add_argument("--delta", type=int, default=64, help="SSSP delta-stepping bucket width")
+    ap.add_argument("--delta", type=int, default=256, help="SSSP delta-stepping bucket width")
     ap.add_argument("--workload", default="pr_pull", choices=["pr_pull", "cc", "sssp", "pr", "batch", "learned"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 / exactness check")

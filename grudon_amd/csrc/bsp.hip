@@ -96,15 +96,37 @@ static grd_status word_ranks(hipStream_t st, const uint64_t *w, uint32_t nw, uin
   return GRD_OK;
 }
 
-// Grid of the word-parallel passes (exchange, update): up to 64 workgroups
-// per CU, so each wave walks only a few 64-bit bitmap words.  A word costs a
-// short chain of dependent loads; the latency is hidden by the waves in
-// flight, while the per-workgroup counter atomics stay few.
+// Grid of the grid-stride word loops (init): up to 64 workgroups per CU.
 static inline uint32_t wave_grid(const grd_ctx *ctx, uint64_t nwords)
 {
   uint64_t want = (nwords + kWavesPerBlock - 1) / kWavesPerBlock;
   uint64_t cap = (uint64_t)ctx->num_cus * 64;
   return (uint32_t)std::max<uint64_t>(1, std::min(want, cap));
+}
+
+// Word-group passes (exchange, update, bucket selection): each wave owns
+// kGroupWords consecutive bitmap words.  Lane j < kGroupWords loads word j of
+// the group (one coalesced access), the lanes finish their zero words
+// themselves, and the wave then walks only the nonzero words with one lane per
+// bit.  A sparse superstep therefore costs one pass over the bitmap, and a
+// dense one a few dependent loads per word.
+constexpr uint32_t kGroupWords = 8;
+
+__device__ __forceinline__ uint32_t group_base()
+{
+  return (blockIdx.x * kWavesPerBlock + wave_in_block()) * kGroupWords;
+}
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t x, int src)
+{
+  const uint32_t lo = __shfl((uint32_t)x, src, kWave), hi = __shfl((uint32_t)(x >> 32), src, kWave);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+static inline uint32_t group_grid(uint64_t nwords)
+{
+  const uint64_t per = (uint64_t)kGroupWords * kWavesPerBlock;
+  return (uint32_t)std::max<uint64_t>(1, (nwords + per - 1) / per);
 }
 
 // ------------------------------------------------------------------------
@@ -170,16 +192,19 @@ __global__ void __launch_bounds__(kBlock) k_bcast_send(uint32_t nw, const uint64
                                                        const V *__restrict__ prop, const uint32_t *__restrict__ wrank,
                                                        int dense, uint64_t *__restrict__ bv, V *__restrict__ val)
 {
-  const int wpb = kWavesPerBlock;
-  for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
-    uint64_t word = F[w];
-    uint32_t s = w * 64 + lane_id();
-    if (lane_id() == 0) bv[w] = word;
-    if (dense) {
-      val[s] = prop[s];
-    } else if ((word >> lane_id()) & 1ull) {
-      val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())] = prop[s];
-    }
+  const uint32_t base = group_base();
+  const int l = lane_id();
+  const bool ok = l < (int)kGroupWords && base + l < nw;
+  const uint64_t mine = ok ? F[base + l] : 0ull;
+  if (ok) bv[base + l] = mine;
+  uint64_t todo = dense ? __ballot(ok) : __ballot(mine != 0);
+  while (todo) {
+    const int j = __ffsll((unsigned long long)todo) - 1;
+    todo &= todo - 1;
+    const uint64_t word = shfl_u64(mine, j);
+    const uint32_t w = base + j, s = w * 64 + l;
+    if (dense) val[s] = prop[s];
+    else if ((word >> l) & 1ull) val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())] = prop[s];
   }
 }
 
@@ -190,16 +215,23 @@ __global__ void __launch_bounds__(kBlock) k_bcast_recv(uint32_t nw, const uint64
                                                        V *__restrict__ mprop, uint64_t *__restrict__ Dp,
                                                        uint32_t *__restrict__ AL)
 {
-  const int wpb = kWavesPerBlock;
-  for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
-    uint32_t pw = word_dst[w];
-    if (pw == 0xFFFFFFFFu) continue;
-    uint64_t word = bv[w];
-    if (lane_id() == 0) Dp[pw] = word;
-    if ((word >> lane_id()) & 1ull) {
+  const uint32_t base = group_base();
+  const int l = lane_id();
+  const bool ok = l < (int)kGroupWords && base + l < nw;
+  const uint32_t pwl = ok ? word_dst[base + l] : 0xFFFFFFFFu;
+  const uint64_t mine = (pwl != 0xFFFFFFFFu) ? bv[base + l] : 0ull;
+  if (pwl != 0xFFFFFFFFu) Dp[pwl] = mine;
+  uint64_t todo = __ballot(mine != 0);
+  while (todo) {
+    const int j = __ffsll((unsigned long long)todo) - 1;
+    todo &= todo - 1;
+    const uint64_t word = shfl_u64(mine, j);
+    const uint32_t pw = __shfl(pwl, j, kWave);
+    const uint32_t w = base + j;
+    if ((word >> l) & 1ull) {
       uint32_t r = (dense && !want_list) ? 0u : wrank[w] + (uint32_t)__popcll(word & lanemask_lt());
-      V x = dense ? val[w * 64 + lane_id()] : val[r];
-      uint32_t p = pw * 64 + lane_id();
+      V x = dense ? val[w * 64 + l] : val[r];
+      uint32_t p = pw * 64 + l;
       mprop[p] = x;
       if (want_list) AL[r] = p;
     }
@@ -211,7 +243,7 @@ __global__ void __launch_bounds__(kBlock) k_bcast_recv(uint32_t nw, const uint64
 // ------------------------------------------------------------------------
 
 // A block stages 256 active vertices in LDS.  Vertices with out-degree below
-// H are processed together: the block scans their degrees and every thread
+// kHeavyDeg are processed together: the block scans their degrees and every thread
 // walks the concatenated edge range (coalesced col reads), locating its
 // source by binary search in the LDS prefix array.  Vertices with degree
 // >= kHeavyDeg are appended to the heavy list as chunks of kHeavyDeg arcs and
@@ -296,26 +328,27 @@ __global__ void __launch_bounds__(kBlock) k_reduce_send(uint32_t nw, uint64_t *_
                                                         const uint32_t *__restrict__ wrank, int dense, int clear_vals,
                                                         uint64_t *__restrict__ bv, V *__restrict__ val, Counters *cnt)
 {
-  const int wpb = kWavesPerBlock;
-  uint32_t mine = 0;
-  for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
-    uint64_t word = Du[w];
-    if (lane_id() == 0) {
-      bv[w] = word;
-      if (word) Du[w] = 0;
-      mine += (uint32_t)__popcll(word);
-    }
-    uint32_t p = w * 64 + lane_id();
-    bool bit = (word >> lane_id()) & 1ull;
-    if (dense) {
-      V x = mupd[p];
-      val[p] = x;
-    } else if (bit) {
-      val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())] = mupd[p];
-    }
+  const uint32_t base = group_base();
+  const int l = lane_id();
+  const bool ok = l < (int)kGroupWords && base + l < nw;
+  const uint64_t mine = ok ? Du[base + l] : 0ull;
+  if (ok) {
+    bv[base + l] = mine;
+    if (mine) Du[base + l] = 0;
+  }
+  uint32_t ndirty = (uint32_t)__popcll(mine);
+  uint64_t todo = dense ? __ballot(ok) : __ballot(mine != 0);
+  while (todo) {
+    const int j = __ffsll((unsigned long long)todo) - 1;
+    todo &= todo - 1;
+    const uint64_t word = shfl_u64(mine, j);
+    const uint32_t w = base + j, p = w * 64 + l;
+    const bool bit = (word >> l) & 1ull;
+    if (dense) val[p] = mupd[p];
+    else if (bit) val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())] = mupd[p];
     if (clear_vals && bit) mupd[p] = V(0);  // PropertyList::clear (DistributedGraph.hpp:140-166)
   }
-  uint32_t t = block_sum(mine);
+  uint32_t t = block_sum(ndirty);
   if (threadIdx.x == 0 && t) atomicAdd(&cnt->dirty, (unsigned long long)t);
 }
 
@@ -326,14 +359,20 @@ __global__ void __launch_bounds__(kBlock) k_reduce_recv(uint32_t nw, const uint6
                                                         const uint32_t *__restrict__ proxy_slot, int dense,
                                                         typename Op::V *upd, uint64_t *Dc)
 {
-  const int wpb = kWavesPerBlock;
-  for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
-    uint64_t word = bv[w];
-    if (!word) continue;  // wave-uniform
+  const uint32_t base = group_base();
+  const int l = lane_id();
+  const bool ok = l < (int)kGroupWords && base + l < nw;
+  const uint64_t mine = ok ? bv[base + l] : 0ull;
+  uint64_t todo = __ballot(mine != 0);
+  while (todo) {
+    const int j = __ffsll((unsigned long long)todo) - 1;
+    todo &= todo - 1;
+    const uint64_t word = shfl_u64(mine, j);
+    const uint32_t w = base + j;
     uint32_t s = 0xFFFFFFFFu;
     typename Op::V x{};
-    if ((word >> lane_id()) & 1ull) {
-      uint32_t p = w * 64 + lane_id();
+    if ((word >> l) & 1ull) {
+      uint32_t p = w * 64 + l;
       x = dense ? val[p] : val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())];
       s = proxy_slot[p];
     }
@@ -353,14 +392,21 @@ __global__ void __launch_bounds__(kBlock) k_update(uint32_t nw, uint64_t *__rest
                                                    uint64_t *__restrict__ F, uint64_t *__restrict__ Pend,
                                                    unsigned long long hi, int use_delta, Counters *cnt)
 {
-  const int wpb = kWavesPerBlock;
+  const uint32_t base = group_base();
+  const int l = lane_id();
+  const bool ok = l < (int)kGroupWords && base + l < nw;
+  const uint64_t mine = ok ? Dc[base + l] : 0ull;
+  if (ok && !mine) F[base + l] = 0;  // nothing changed: empty next-frontier word
   uint64_t f_cnt = 0, f_edges = 0, improved = 0;
   uint32_t f_max = 0;
-  for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
-    uint64_t d = Dc[w];
+  uint64_t todo = __ballot(mine != 0);
+  while (todo) {
+    const int j = __ffsll((unsigned long long)todo) - 1;
+    todo &= todo - 1;
+    const uint64_t d = shfl_u64(mine, j);
+    const uint32_t w = base + j, s = w * 64 + l;
     bool f = false, pend = false, imp = false;
-    uint32_t s = w * 64 + lane_id();
-    if ((d >> lane_id()) & 1ull) {
+    if ((d >> l) & 1ull) {
       if constexpr (ALGO == GRD_PR) {
         float *prop = (float *)prop_, *upd = (float *)upd_;
         float u = upd[s];
@@ -381,14 +427,15 @@ __global__ void __launch_bounds__(kBlock) k_update(uint32_t nw, uint64_t *__rest
         }
       }
     }
-    uint64_t fm = __ballot(f);
-    if (lane_id() == 0) {
+    const uint64_t fm = __ballot(f);
+    if (l == 0) {
       F[w] = fm;
-      if (d) Dc[w] = 0;
+      Dc[w] = 0;
+      f_cnt += (uint64_t)__popcll(fm);
     }
     if (use_delta) {
-      uint64_t pm = __ballot(pend);
-      if (lane_id() == 0 && (pm | fm)) Pend[w] = (Pend[w] | pm) & ~fm;
+      const uint64_t pm = __ballot(pend);
+      if (l == 0 && (pm | fm)) Pend[w] = (Pend[w] | pm) & ~fm;
     }
     if (f) {
       uint32_t dg = deg[s];
@@ -396,7 +443,6 @@ __global__ void __launch_bounds__(kBlock) k_update(uint32_t nw, uint64_t *__rest
       f_max = dg > f_max ? dg : f_max;
     }
     improved += imp;
-    if (lane_id() == 0) f_cnt += (uint64_t)__popcll(fm);
   }
   uint64_t a = block_sum(f_cnt);
   uint64_t b = block_sum(f_edges);
@@ -414,13 +460,19 @@ __global__ void __launch_bounds__(kBlock) k_update(uint32_t nw, uint64_t *__rest
 __global__ void __launch_bounds__(kBlock) k_pend_min(uint32_t nw, const uint64_t *__restrict__ Pend,
                                                      const uint32_t *__restrict__ prop, Counters *cnt)
 {
-  const int wpb = kWavesPerBlock;
+  const uint32_t base = group_base();
+  const int l = lane_id();
+  const bool ok = l < (int)kGroupWords && base + l < nw;
+  const uint64_t mine = ok ? Pend[base + l] : 0ull;
   uint32_t mn = 0xFFFFFFFFu;
   uint64_t c = 0;
-  for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
-    uint64_t pm = Pend[w];
-    if ((pm >> lane_id()) & 1ull) {
-      uint32_t v = prop[w * 64 + lane_id()];
+  uint64_t todo = __ballot(mine != 0);
+  while (todo) {
+    const int j = __ffsll((unsigned long long)todo) - 1;
+    todo &= todo - 1;
+    const uint64_t pm = shfl_u64(mine, j);
+    if ((pm >> l) & 1ull) {
+      uint32_t v = prop[(base + j) * 64 + l];
       mn = v < mn ? v : mn;
       c++;
     }
@@ -438,16 +490,22 @@ __global__ void __launch_bounds__(kBlock) k_select(uint32_t nw, uint64_t *__rest
                                                    const uint32_t *__restrict__ prop, const uint32_t *__restrict__ deg,
                                                    unsigned long long hi, uint64_t *__restrict__ F, Counters *cnt)
 {
-  const int wpb = kWavesPerBlock;
+  const uint32_t base = group_base();
+  const int l = lane_id();
+  const bool ok = l < (int)kGroupWords && base + l < nw;
+  const uint64_t mine = ok ? Pend[base + l] : 0ull;
+  if (ok && !mine) F[base + l] = 0;
   uint64_t f_cnt = 0, f_edges = 0;
   uint32_t f_max = 0;
-  for (uint32_t w = blockIdx.x * wpb + wave_in_block(); w < nw; w += gridDim.x * wpb) {
-    uint64_t pm = Pend[w];
-    bool f = false;
-    uint32_t s = w * 64 + lane_id();
-    if ((pm >> lane_id()) & 1ull) f = (unsigned long long)prop[s] < hi;
-    uint64_t fm = __ballot(f);
-    if (lane_id() == 0) {
+  uint64_t todo = __ballot(mine != 0);
+  while (todo) {
+    const int j = __ffsll((unsigned long long)todo) - 1;
+    todo &= todo - 1;
+    const uint64_t pm = shfl_u64(mine, j);
+    const uint32_t w = base + j, s = w * 64 + l;
+    const bool f = ((pm >> l) & 1ull) && (unsigned long long)prop[s] < hi;
+    const uint64_t fm = __ballot(f);
+    if (l == 0) {
       F[w] = fm;
       if (fm) Pend[w] = pm & ~fm;
       f_cnt += (uint64_t)__popcll(fm);
@@ -650,7 +708,7 @@ grd_status op_broadcast(grd_state *s, int option)
   const bool pull = option & 1, dense = option & 2;
   const bool need_rank = !dense || !pull;
   if (need_rank) GRD_TRY(word_ranks(st, s->F, nwS, s->blk, s->wrankB, s->totals + 0));
-  const uint32_t grid = wave_grid(g->ctx, nwS);
+  const uint32_t grid = group_grid(nwS);
   hipLaunchKernelGGL(k_bcast_send<uint32_t>, dim3(grid), dim3(kBlock), 0, st, nwS, s->F, (const uint32_t *)s->prop,
                      s->wrankB, (int)dense, s->bvB, (uint32_t *)s->valB);
   hipLaunchKernelGGL(k_bcast_recv<uint32_t>, dim3(grid), dim3(kBlock), 0, st, nwS, s->bvB, (const uint32_t *)s->valB,
@@ -719,7 +777,7 @@ static grd_status reduce_t(grd_state *s, int option)
   const uint32_t nwP = g->P / 64;
   const bool dense = option & 2;
   if (!dense) GRD_TRY(word_ranks(st, s->Du, nwP, s->blk, s->wrankR, s->totals + 1));
-  const uint32_t grid = wave_grid(g->ctx, nwP);
+  const uint32_t grid = group_grid(nwP);
   hipLaunchKernelGGL(k_reduce_send<V>, dim3(grid), dim3(kBlock), 0, st, nwP, s->Du, (V *)s->mupd, s->wrankR, (int)dense,
                      (int)(s->algo == GRD_PR), s->bvR, (V *)s->valR, s->cnt);
   hipLaunchKernelGGL((k_reduce_recv<Op>), dim3(grid), dim3(kBlock), 0, st, nwP, s->bvR, (const V *)s->valR, s->wrankR,
@@ -743,7 +801,7 @@ grd_status op_update(grd_state *s)
   grd_graph *g = s->g;
   hipStream_t st = g->ctx->stream;
   const uint32_t nwS = g->S / 64;
-  const uint32_t grid = wave_grid(g->ctx, nwS);
+  const uint32_t grid = group_grid(nwS);
   const int use_delta = (s->algo == GRD_SSSP && s->prm.delta) ? 1 : 0;
   switch (s->algo) {
   case GRD_PR:
@@ -772,7 +830,7 @@ grd_status op_select_bucket(grd_state *s, bool *any_pending)
   grd_graph *g = s->g;
   hipStream_t st = g->ctx->stream;
   const uint32_t nwS = g->S / 64;
-  const uint32_t grid = wave_grid(g->ctx, nwS);
+  const uint32_t grid = group_grid(nwS);
   *any_pending = false;
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, s->cnt);
   hipLaunchKernelGGL(k_pend_min, dim3(grid), dim3(kBlock), 0, st, nwS, s->Pend, (const uint32_t *)s->prop, s->cnt);

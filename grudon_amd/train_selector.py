@@ -1,12 +1,15 @@
 """Build the schedule selector: label iterations by timing, train on the GPU.
 
     python -m grudon_amd.train_selector --budget 240 --out grudon_amd/selector_bf16.bin
+    python -m grudon_amd.train_selector --budget 900 --seed 2 --records r2.npz --no-train
+    python -m grudon_amd.train_selector --load-records r1.npz,r2.npz --out grudon_amd/selector_bf16.bin
 
 1. Records.  Seeded RMAT (scale 16-22, edge factor 8-32) and Erdos-Renyi
    graphs (n 2^16-2^21, mean degree 4-64), partitioned with M in {1, 2, 4},
-   run CC (symmetric), SSSP hop count (several sources) and the reference
-   delta PageRank once per schedule option (push/pull x sparse/dense) with
-   per-iteration device timing.  CC and SSSP supersteps are deterministic, so
+   run CC (symmetric), SSSP hop count and weighted delta-stepping SSSP
+   (weights [1,255], several sources) and the reference delta PageRank once
+   per schedule option (push/pull x sparse/dense) with per-iteration device
+   timing.  CC and SSSP supersteps are deterministic, so
    the four runs visit identical states and iteration i of each run is the
    same decision; PageRank iterations are kept only where the four runs
    still agree on the frontier size.  A record is (the 16 features before
@@ -65,27 +68,35 @@ def collect(ctx, budget_s, seed=1, max_scale=22):
     while time.time() - t0 < budget_s:
         kind = rng.choice(["rmat", "er"], p=[0.7, 0.3])
         M = int(rng.choice([1, 2, 4]))
+        wseed = int(rng.integers(1, 1 << 30))
         if kind == "rmat":
             scale = int(rng.integers(16, max_scale + 1))
             ef = int(rng.choice([8, 16, 32]))
             gseed = int(rng.integers(1, 1 << 30))
-            Gd = api.Graph.rmat(ctx, scale, ef, seed=gseed, C=1, M=M)
-            Gs = api.Graph.rmat(ctx, scale, ef, seed=gseed, symmetrize=True, C=1, M=M)
+            Gd = api.Graph.rmat(ctx, scale, ef, seed=gseed, wseed=wseed, C=1, M=M)
+            Gs = api.Graph.rmat(ctx, scale, ef, seed=gseed, symmetrize=True, wseed=wseed, C=1, M=M)
         else:
             n = int(2 ** rng.uniform(16, min(21, max_scale)))
             deg = float(rng.uniform(4, 64))
             gseed = int(rng.integers(1, 1 << 30))
-            Gs = api.Graph.er(ctx, n, deg, seed=gseed, C=1, M=M)
+            Gs = api.Graph.er(ctx, n, deg, seed=gseed, wseed=wseed, C=1, M=M)
             Gd = Gs
         deg_d = Gd.out_degree()
         sources = np.flatnonzero(deg_d > 0)
         jobs = [(Gs, api.CC, {}, 200, True)]
-        for s in rng.choice(sources, size=min(4, len(sources)), replace=False):
+        for s in rng.choice(sources, size=min(3, len(sources)), replace=False):
             jobs.append((Gd, api.SSSP, {"source": int(s)}, 400, True))
+        for s in rng.choice(sources, size=min(2, len(sources)), replace=False):
+            delta = int(rng.choice([0, 32, 64, 128, 256]))
+            jobs.append((Gd, api.SSSP, {"source": int(s), "weighted": True, "delta": delta}, 2000, True))
         if Gs is not Gd:
             deg_s = Gs.out_degree()
-            for s in rng.choice(np.flatnonzero(deg_s > 0), size=2, replace=False):
+            srcs = np.flatnonzero(deg_s > 0)
+            for s in rng.choice(srcs, size=2, replace=False):
                 jobs.append((Gs, api.SSSP, {"source": int(s)}, 400, True))
+            s = int(rng.choice(srcs))
+            jobs.append((Gs, api.SSSP, {"source": s, "weighted": True, "delta": int(rng.choice([32, 64, 128]))},
+                         2000, True))
         if kind == "rmat" and Gd.n <= (1 << 20):
             jobs.append((Gd, api.PR, {}, 60, False))
         for G, algo, kw, mi, exact in jobs:
@@ -111,15 +122,21 @@ def main():
     ap.add_argument("--records", default="", help="also save the records (.npz)")
     ap.add_argument("--load-records", default="", help="train from saved records instead of collecting")
     ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--seed", type=int, default=1, help="seed of the graph / source draw")
+    ap.add_argument("--no-train", action="store_true", help="only collect (and save) records")
     args = ap.parse_args()
     ctx = api.Context(0)
     if args.load_records:
-        d = np.load(args.load_records)
-        feats, costs = d["feat"], d["cost"]
+        parts = [np.load(f) for f in args.load_records.split(",")]
+        feats = np.concatenate([d["feat"] for d in parts])
+        costs = np.concatenate([d["cost"] for d in parts])
     else:
-        feats, costs = collect(ctx, args.budget, max_scale=args.max_scale)
+        feats, costs = collect(ctx, args.budget, seed=args.seed, max_scale=args.max_scale)
         if args.records:
             np.savez_compressed(args.records, feat=feats, cost=costs)
+        if args.no_train:
+            log("records %d saved to %s" % (len(feats), args.records))
+            return
     labels = costs.argmin(axis=1).astype(np.uint32)
     log("records %d, label histogram %s" % (len(labels), np.bincount(labels, minlength=4).tolist()))
     # hold out 10% to report the selector's regret
