@@ -94,6 +94,7 @@ struct PullLong {
 };
 constexpr uint32_t kPullNnz = 2048;     // arcs per block (8 per thread)
 constexpr uint32_t kPullMaxRows = 1024;  // rows per block
+constexpr uint32_t kPullHot = 2048;      // PR pull: hottest sources of a partition kept in LDS
 constexpr uint32_t kHeavyDeg = 4096;     // push: out-degree handled by k_push_heavy, in chunks of this many arcs
 
 }  // namespace grd
@@ -142,6 +143,7 @@ struct grd_graph {
   // pull row blocks (pull.hip): at most kPullNnz arcs per block, longer rows
   // split into segments merged by a fix-up pass
   grd::PullBlock *pblk = nullptr;
+  uint32_t *pblk_hot = nullptr;  // per row block: first proxy of its partition's own region
   uint32_t n_pblk = 0;
   grd::PullLong *plong = nullptr;
   uint32_t n_plong = 0;

@@ -115,6 +115,18 @@ static grd_status build_row_blocks(grd_graph *g, const std::vector<uint32_t> &cr
     blks.push_back(PullBlock{r0, r - r0, e0, crp[r]});
   }
   if (blks.size() >= 0x7FFFFFFFull) return fail(GRD_ERR_ARG, "graph: too many pull row blocks");
+  // hot window of each block: the first own proxies (highest out-degree) of
+  // the memory partition its first row belongs to
+  std::vector<uint32_t> hot(blks.size());
+  {
+    const int C = g->C, M = g->M;
+    int m = 0;
+    for (size_t i = 0; i < blks.size(); i++) {
+      while (m + 1 < M && g->obase[(size_t)(m + 1) * C] <= blks[i].r0) m++;
+      hot[i] = g->obase[(size_t)m * C];
+    }
+  }
+  GRD_TRY(upload(g, &g->pblk_hot, hot));
   g->n_pblk = (uint32_t)blks.size();
   g->n_plong = (uint32_t)longs.size();
   if (longs.empty()) longs.push_back(PullLong{0, 0, 0, 0});

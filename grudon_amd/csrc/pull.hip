@@ -37,12 +37,14 @@ __device__ __forceinline__ int group_size(uint32_t nrows)
 // ---- PageRank pull: Kahan FP32 ----
 
 __global__ void __launch_bounds__(kBlock) k_prp_pull_rb(const PullBlock *__restrict__ blocks,
+                                                        const uint32_t *__restrict__ hot_base,
                                                         const uint32_t *__restrict__ crp,
                                                         const uint32_t *__restrict__ ccol,
-                                                        const float *__restrict__ mprop, float *__restrict__ out,
-                                                        float2 *__restrict__ partial)
+                                                        const float *__restrict__ mprop, uint32_t P,
+                                                        float *__restrict__ out, float2 *__restrict__ partial)
 {
   __shared__ float s_v[kPullNnz];
+  __shared__ float s_hot[kPullHot];
   __shared__ float s_ws[kWavesPerBlock], s_wc[kWavesPerBlock];
   const uint32_t bid = blockIdx.x;
   const PullBlock d = blocks[bid];
@@ -67,7 +69,9 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_rb(const PullBlock *__restr
   const uint32_t ne = d.e1 - d.e0;
   {
     // all column loads first, then all gathers: 8 independent requests in
-    // flight per thread for each stage
+    // flight per thread for each stage.  The values of the partition's
+    // kPullHot highest-degree sources (the first own proxies) are staged in
+    // LDS and gathered from there.
     constexpr int kPer = kPullNnz / kBlock;
     uint32_t src[kPer];
     float v[kPer];
@@ -76,10 +80,15 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_rb(const PullBlock *__restr
       uint32_t i = threadIdx.x + k * kBlock;
       src[k] = i < ne ? ccol[d.e0 + i] : 0u;
     }
+    const uint32_t hb = hot_base[bid];
+    const uint32_t hn = min(kPullHot, P - hb);
+    for (uint32_t i = threadIdx.x; i < hn; i += kBlock) s_hot[i] = mprop[hb + i];
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
       uint32_t i = threadIdx.x + k * kBlock;
-      v[k] = i < ne ? mprop[src[k]] : 0.0f;
+      const uint32_t h = src[k] - hb;
+      v[k] = i < ne ? (h < hn ? s_hot[h] : mprop[src[k]]) : 0.0f;
     }
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
@@ -250,8 +259,8 @@ grd_status prp_pull_launch(grd_state *s)
 {
   grd_graph *g = s->g;
   hipStream_t st = g->ctx->stream;
-  hipLaunchKernelGGL(k_prp_pull_rb, dim3(g->n_pblk), dim3(kBlock), 0, st, g->pblk, g->crp, g->ccol,
-                     (const float *)s->mprop, (float *)s->valR, (float2 *)s->partial);
+  hipLaunchKernelGGL(k_prp_pull_rb, dim3(g->n_pblk), dim3(kBlock), 0, st, g->pblk, g->pblk_hot, g->crp, g->ccol,
+                     (const float *)s->mprop, g->P, (float *)s->valR, (float2 *)s->partial);
   GRD_HIP(hipGetLastError());
   return GRD_OK;
 }
