@@ -338,6 +338,16 @@ def run_pagerank(api, ctx, args):
     n, m, bytes_x = G.n, G.m, last["bytes_exchanged"]
     S.free()
     G.free()
+    if pull and kl:
+        # the pull's gather rate beside the device's random-gather ceiling for
+        # a table of one memory partition's own vertices and for an
+        # L2-resident table (4 MB)
+        ceil = {"pull_gathers_per_s": round(m / (kms / kl * 1e-3), 1)}
+        for name, entries in (("partition_table", -(-n // args.parts)), ("l2_table", 1 << 20)):
+            _, rate = ctx.probe_gather(entries)
+            ceil[name] = {"entries": entries, "uniform_random_gathers_per_s": round(rate, 1)}
+        extra["gather_ceiling"] = ceil
+        log("gather ceiling: %s" % json.dumps(ceil))
     if pull and args.speedup_scale:
         extra["learned_vs_plain"] = schedule_speedup(api, ctx, args.speedup_scale, args.parts)
     cpu = None
@@ -482,9 +492,12 @@ def run_batch(api, ctx, args):
         # RMAT half against the partitioned BSP path of this library
         worst = 0.0
         cc_ok = True
+        it_pr, _ = B.pagerank(tolerance=1e-6, max_iter=1000)
         for i, (rp, col) in enumerate(graphs):
             ref = fp64_pagerank(rp, col, int(it_pr[i]))
             worst = max(worst, float(np.abs(B.result(i, api.PR_PULL).astype(np.float64) - ref).sum()))
+        B.cc(max_iter=1000)
+        for i, (rp, col) in enumerate(graphs):
             lab = B.result(i, api.CC)
             if desc[i].startswith("er"):
                 cc_ok &= bool(np.array_equal(lab, cc_labels_scipy(rp, col)))

@@ -28,7 +28,7 @@ EXPORTS = [
     "grd_features", "grd_snapshot",
     "grd_restore", "grd_selector_load", "grd_selector_train", "grd_selector_infer", "grd_selector_grad",
     "grd_batch_create", "grd_batch_free", "grd_batch_tune", "grd_batch_autotune", "grd_batch_pagerank",
-    "grd_batch_cc", "grd_batch_download",
+    "grd_batch_cc", "grd_batch_download", "grd_probe_gather",
 ]
 
 
@@ -110,6 +110,7 @@ def lib():
             "grd_batch_pagerank": [vp, ct.c_double, ct.c_double, ct.c_uint32, vp, ct.POINTER(ct.c_double)],
             "grd_batch_cc": [vp, ct.c_uint32, vp, ct.POINTER(ct.c_double)],
             "grd_batch_download": [vp, ct.c_uint32, ct.c_int, vp],
+            "grd_probe_gather": [vp, ct.c_uint64, ct.c_uint64, vp, vp],
         }
         for name, args in sigs.items():
             fn = getattr(L, name, None)
@@ -176,6 +177,13 @@ class Context:
         w = np.zeros(max(len(col), 1), dtype=np.uint32)
         _check(lib().grd_edge_weights(self.h, len(rp) - 1, _ptr(rp), _ptr(col), wseed, _ptr(w)), "grd_edge_weights")
         return w[: len(col)]
+
+    def probe_gather(self, table_entries, n_gathers=1 << 28):
+        """Random-gather ceiling: (ms per pass, gathers per second)."""
+        ms, rate = ct.c_double(), ct.c_double()
+        _check(lib().grd_probe_gather(self.h, table_entries, n_gathers, ct.byref(ms), ct.byref(rate)),
+               "grd_probe_gather")
+        return ms.value, rate.value
 
     # selector
     def selector_load(self, w_bf16, min_margin=0.0):

@@ -389,6 +389,7 @@ grd_status grd_broadcast(grd_state *s, int option)
   if (option < 0 || option > 3) return fail(GRD_ERR_ARG, "grd_broadcast: bad option");
   if (s->algo == GRD_PR_PULL) return prp_broadcast(s);
   GRD_TRY(op_zero_counters(s));
+  s->bcast_option = option;
   return op_broadcast(s, option);
 }
 
@@ -397,6 +398,10 @@ grd_status grd_traverse(grd_state *s, int option)
   if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_traverse: state not initialized");
   if (option < 0 || option > 3) return fail(GRD_ERR_ARG, "grd_traverse: bad option");
   if (s->algo == GRD_PR_PULL) return prp_traverse(s);
+  // the broadcast prepared the memory side for one direction: the active
+  // list for a push, every own proxy (identity when inactive) for a pull
+  if (s->bcast_option < 0 || ((s->bcast_option ^ option) & 1))
+    return fail(GRD_ERR_STATE, "grd_traverse: direction differs from the preceding grd_broadcast");
   return op_traverse(s, option);
 }
 

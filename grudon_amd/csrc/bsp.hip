@@ -208,11 +208,15 @@ __global__ void __launch_bounds__(kBlock) k_bcast_send(uint32_t nw, const uint64
   }
 }
 
+// Receive of the broadcast.  For a push the active proxies get their value
+// and the active list; for a pull (want_list == 0) every own proxy is written,
+// inactive ones with the combine identity, so that the pull gathers one value
+// per arc and needs no activity test.
 template <typename V>
 __global__ void __launch_bounds__(kBlock) k_bcast_recv(uint32_t nw, const uint64_t *__restrict__ bv,
                                                        const V *__restrict__ val, const uint32_t *__restrict__ wrank,
                                                        const uint32_t *__restrict__ word_dst, int dense, int want_list,
-                                                       V *__restrict__ mprop, uint64_t *__restrict__ Dp,
+                                                       V ident, V *__restrict__ mprop, uint64_t *__restrict__ Dp,
                                                        uint32_t *__restrict__ AL)
 {
   const uint32_t base = group_base();
@@ -221,19 +225,21 @@ __global__ void __launch_bounds__(kBlock) k_bcast_recv(uint32_t nw, const uint64
   const uint32_t pwl = ok ? word_dst[base + l] : 0xFFFFFFFFu;
   const uint64_t mine = (pwl != 0xFFFFFFFFu) ? bv[base + l] : 0ull;
   if (pwl != 0xFFFFFFFFu) Dp[pwl] = mine;
-  uint64_t todo = __ballot(mine != 0);
+  uint64_t todo = want_list ? __ballot(mine != 0) : __ballot(pwl != 0xFFFFFFFFu);
   while (todo) {
     const int j = __ffsll((unsigned long long)todo) - 1;
     todo &= todo - 1;
     const uint64_t word = shfl_u64(mine, j);
     const uint32_t pw = __shfl(pwl, j, kWave);
     const uint32_t w = base + j;
+    const uint32_t p = pw * 64 + l;
     if ((word >> l) & 1ull) {
       uint32_t r = (dense && !want_list) ? 0u : wrank[w] + (uint32_t)__popcll(word & lanemask_lt());
       V x = dense ? val[w * 64 + l] : val[r];
-      uint32_t p = pw * 64 + l;
       mprop[p] = x;
       if (want_list) AL[r] = p;
+    } else if (!want_list) {
+      mprop[p] = ident;
     }
   }
 }
@@ -685,6 +691,7 @@ grd_status state_init(grd_state *s)
   s->iteration = 0;
   s->bytes = 0;
   s->edges = 0;
+  s->bcast_option = -1;
   return GRD_OK;
 }
 
@@ -711,8 +718,9 @@ grd_status op_broadcast(grd_state *s, int option)
   const uint32_t grid = group_grid(nwS);
   hipLaunchKernelGGL(k_bcast_send<uint32_t>, dim3(grid), dim3(kBlock), 0, st, nwS, s->F, (const uint32_t *)s->prop,
                      s->wrankB, (int)dense, s->bvB, (uint32_t *)s->valB);
+  const uint32_t ident = (s->algo == GRD_PR) ? 0u : 0xFFFFFFFFu;  // 0.0f or the min identity
   hipLaunchKernelGGL(k_bcast_recv<uint32_t>, dim3(grid), dim3(kBlock), 0, st, nwS, s->bvB, (const uint32_t *)s->valB,
-                     s->wrankB, g->word_dst, (int)dense, (int)!pull, (uint32_t *)s->mprop, s->Dp, s->AL);
+                     s->wrankB, g->word_dst, (int)dense, (int)!pull, ident, (uint32_t *)s->mprop, s->Dp, s->AL);
   GRD_HIP(hipGetLastError());
   // exchange bytes (GraphAlgorithm.cpp:251-265): bitvector of every pair +
   // packed values (sparse) or the pair's full own slice (dense)

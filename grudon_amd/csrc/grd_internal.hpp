@@ -157,6 +157,7 @@ struct grd_state {
   uint32_t iteration = 0;
   uint64_t hi = ~0ull;  // SSSP delta-stepping bucket bound
   int last_option = 0;
+  int bcast_option = -1;  // option of the last broadcast (the traversal must use the same direction)
   uint64_t last_frontier = 0, prev_frontier = 0, last_frontier_edges = 0, last_dirty = 0;
   uint32_t last_maxdeg = 0;
   uint64_t bytes = 0, edges = 0;

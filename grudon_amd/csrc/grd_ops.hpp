@@ -12,13 +12,14 @@ struct OpCC {
   static constexpr bool kMin = true;
   __device__ static V edge(V v, uint32_t) { return v; }
 };
-// SSSP hop count: dist + 1 (sssp.cpp:69)
+// SSSP hop count: dist + 1 (sssp.cpp:69); the unreached value stays unreached
 struct OpHop {
   using V = uint32_t;
   static constexpr bool kMin = true;
-  __device__ static V edge(V v, uint32_t) { return v + 1u; }
+  __device__ static V edge(V v, uint32_t) { return v == 0xFFFFFFFFu ? v : v + 1u; }
 };
-// SSSP weighted (extension): saturating dist + w
+// SSSP weighted (extension): saturating dist + w (the unreached value stays
+// unreached)
 struct OpWeighted {
   using V = uint32_t;
   static constexpr bool kMin = true;

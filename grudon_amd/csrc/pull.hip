@@ -131,15 +131,17 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_fixup(const PullLong *__res
 }
 
 // ---- BSP pull: min / + over the active sources ----
+//
+// The broadcast receive of a pull superstep writes every own proxy: active
+// ones with their value, inactive ones with the combine identity (Op::edge
+// keeps the identity), so one gather per arc gives the arc's contribution.
 
 template <class Op>
 __device__ __forceinline__ typename Op::V pull_val(uint32_t e, const uint32_t *__restrict__ ccol,
-                                                   const uint32_t *__restrict__ cw, const uint64_t *__restrict__ Dp,
+                                                   const uint32_t *__restrict__ cw,
                                                    const typename Op::V *__restrict__ mprop)
 {
-  uint32_t src = ccol[e];
-  if ((Dp[src >> 6] >> (src & 63)) & 1ull) return Op::edge(mprop[src], cw ? cw[e] : 1u);
-  return op_identity<Op>();
+  return Op::edge(mprop[ccol[e]], cw ? cw[e] : 1u);
 }
 
 // apply the pulled value to the proxy's running update; true = dirty
@@ -166,7 +168,7 @@ template <class Op>
 __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restrict__ blocks,
                                                         const uint32_t *__restrict__ crp,
                                                         const uint32_t *__restrict__ ccol,
-                                                        const uint32_t *__restrict__ cw, const uint64_t *__restrict__ Dp,
+                                                        const uint32_t *__restrict__ cw,
                                                         const typename Op::V *__restrict__ mprop,
                                                         typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty,
                                                         typename Op::V *__restrict__ partial)
@@ -179,7 +181,7 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
   if (d.nrows == 0) {
     V acc = op_identity<Op>();
     for (uint32_t e = d.e0 + threadIdx.x; e < d.e1; e += kBlock)
-      acc = op_combine<Op>(acc, pull_val<Op>(e, ccol, cw, Dp, mprop));
+      acc = op_combine<Op>(acc, pull_val<Op>(e, ccol, cw, mprop));
     if constexpr (Op::kMin) acc = block_min(acc);
     else acc = block_sum(acc);
     if (threadIdx.x == 0) partial[bid] = acc;
@@ -188,26 +190,20 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
   for (int i = threadIdx.x; i < kDirtyWords; i += kBlock) s_dirty[i] = 0ull;
   const uint32_t ne = d.e1 - d.e0;
   {
-    // batched stages: column ids, active bits, then the gathers of the
-    // active sources
+    // batched stages: column ids (and weights), then the gathers
     constexpr int kPer = kPullNnz / kBlock;
-    uint32_t src[kPer];
-    uint64_t act[kPer];
+    uint32_t src[kPer], wt[kPer];
     V v[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
       uint32_t i = threadIdx.x + k * kBlock;
       src[k] = i < ne ? ccol[d.e0 + i] : 0u;
+      wt[k] = (cw && i < ne) ? cw[d.e0 + i] : 1u;
     }
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
       uint32_t i = threadIdx.x + k * kBlock;
-      act[k] = i < ne ? (Dp[src[k] >> 6] >> (src[k] & 63)) & 1ull : 0ull;
-    }
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-      uint32_t i = threadIdx.x + k * kBlock;
-      v[k] = act[k] ? Op::edge(mprop[src[k]], cw ? cw[d.e0 + i] : 1u) : op_identity<Op>();
+      v[k] = i < ne ? Op::edge(mprop[src[k]], wt[k]) : op_identity<Op>();
     }
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
@@ -281,7 +277,7 @@ grd_status bsp_pull_launch(grd_state *s, const uint32_t *cw)
   grd_graph *g = s->g;
   using V = typename Op::V;
   hipLaunchKernelGGL(k_bsp_pull_rb<Op>, dim3(g->n_pblk), dim3(kBlock), 0, g->ctx->stream, g->pblk, g->crp, g->ccol, cw,
-                     s->Dp, (const V *)s->mprop, (V *)s->mupd, s->Du, (V *)s->partial);
+                     (const V *)s->mprop, (V *)s->mupd, s->Du, (V *)s->partial);
   GRD_HIP(hipGetLastError());
   return GRD_OK;
 }

@@ -233,6 +233,15 @@ grd_status grd_batch_cc(grd_batch *b, uint32_t max_iter, uint32_t *iters_out, do
 /* Results of graph i: PR (float[n]) or CC labels (uint32[n]). */
 grd_status grd_batch_download(grd_batch *b, uint32_t i, int algo, void *out);
 
+/* ---- measurement ---- */
+/* No counterpart in the reference: the random-gather ceiling of the device,
+ * against which the pull kernels' gather rate is read.  Times n_gathers 4-byte
+ * gathers at uniform random positions of a table of table_entries floats,
+ * with the pull's access structure (8 coalesced index loads, then 8
+ * independent gathers per thread); allocates and frees its own buffers. */
+grd_status grd_probe_gather(grd_ctx *ctx, uint64_t table_entries, uint64_t n_gathers, double *ms_out,
+                            double *gathers_per_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,6 +111,22 @@ def test_stepwise_api_matches_run(ctx):
     assert np.array_equal(S.result(), ref) and n_iter == int(st[0])
 
 
+def test_stepwise_direction_must_match_broadcast(ctx):
+    _say("stepwise direction")
+    g = O.rmat(9, 8, seed=4, symmetrize=True)
+    G = api.Graph.from_csr(ctx, g.rp, g.col, C=1, M=2)
+    S = api.State(G, api.CC)
+    S.init()
+    L = api.lib()
+    assert L.grd_traverse(S.h, api.OPT_PUSH_SPARSE) == 4  # no broadcast yet: GRD_ERR_STATE
+    assert L.grd_broadcast(S.h, api.OPT_PUSH_SPARSE) == 0
+    assert L.grd_traverse(S.h, api.OPT_PULL_SPARSE) == 4
+    assert b"direction" in L.grd_last_error()
+    # the matching direction still completes the superstep correctly
+    assert L.grd_traverse(S.h, api.OPT_PUSH_DENSE) == 0
+    S.free()
+
+
 def test_cli_runs_all_algorithms(ctx, tmp_path):
     _say("cli")
     cli = os.path.join(ROOT, "grudon_amd", "bin", "grudon")
@@ -140,3 +156,11 @@ def test_cli_runs_all_algorithms(ctx, tmp_path):
     assert p.returncode == 0, p.stdout
     gw = O.rmat(10, 8, seed=3, symmetrize=True).with_weights(5)
     assert np.array_equal(np.fromfile(res2, dtype=np.uint32), O.dijkstra(gw, 0))
+
+
+def test_gather_probe_runs(ctx):
+    _say("gather probe")
+    ms, rate = ctx.probe_gather(1 << 16, 1 << 20)
+    assert ms > 0 and rate > 1e8
+    with pytest.raises(api.GrdError):
+        ctx.probe_gather(0, 1 << 20)
