@@ -313,10 +313,11 @@ grd_status grd_graph_table(const grd_graph *g, int c, int m, uint32_t *out, uint
   const uint32_t own = g->ocnt[k], cross = g->xcnt[k];
   *count = (uint64_t)own + cross;
   if (out) {
-    // own proxies sit in out-degree order on the device; the table is by gid
+    // proxies sit in out-degree / slot order on the device; the table is by gid
     for (uint32_t i = 0; i < own; i++) out[i] = g->h_proxy_gid[g->obase[k] + i];
     std::sort(out, out + own);
     for (uint32_t i = 0; i < cross; i++) out[own + i] = g->h_proxy_gid[g->xbase[k] + i];
+    std::sort(out + own, out + own + cross);
   }
   return GRD_OK;
 }

@@ -11,11 +11,11 @@
 //   memory proxies [0, P): memory partition m, then master c, then the own
 //                  region (same order as the compute group, so bitvector
 //                  words map one to one) and the cross-mirror region
-//                  (ascending gid), each padded to 64.  The (m, c) own+cross
+//                  (ordered by the vertices' compute slots), each padded to 64.  The (m, c) own+cross
 //                  range holds the reference's memory-side translation table
 //                  for master c (DistributedGraph.cpp:644-660).
 //
-// The degree order is a permutation of the reference's wire order inside each
+// Both orders are permutations of the reference's wire order inside each
 // bitvector (same length, same popcount, same bytes moved).  The pull gathers
 // a source's value once per out-arc, so it packs the most gathered values of a
 // memory partition into the fewest cache lines.

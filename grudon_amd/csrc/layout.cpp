@@ -14,8 +14,9 @@
 //   own then cross per master, both ascending  DistributedGraph.cpp:644-660, :701-734
 // The resulting slots/proxies are stored in the padded, pair-grouped order
 // described in grd_internal.hpp: own vertices of a (master, mirror) pair by
-// out-degree rank, cross mirrors by gid.  grd_graph_table still returns the
-// reference's ascending tables.  The pull CSC rows are sorted by source.
+// out-degree rank, cross mirrors by compute slot.  grd_graph_table still
+// returns the reference's ascending tables.  The pull CSC rows are sorted by
+// source.
 #include <algorithm>
 #include <atomic>
 #include <cstring>
@@ -269,24 +270,35 @@ static grd_status build_impl(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const
       }
     }
   }
-  // cross proxy per (m, gid)
+  // cross proxy per (m, gid).  Within a (m, c) cross region the proxies are
+  // ordered by their vertex's compute slot, so that the compute side, which
+  // walks its slots in order, reads each partition's cross partials in
+  // ascending order.
   std::vector<uint32_t> xp((uint64_t)M * n, kNone);
   g->h_proxy_gid.assign(g->P, kNone);
   for (uint32_t v = 0; v < n; v++) g->h_proxy_gid[own_proxy[v]] = v;
-  for (int mm = 0; mm < M; mm++) {
-    std::vector<uint32_t> ctr(C, 0);
-    for (uint32_t wd = 0; wd < NW; wd++) {
-      uint64_t x = crossw[(uint64_t)mm * NW + wd];
-      while (x) {
-        uint32_t v = wd * 64 + (uint32_t)__builtin_ctzll(x);
-        x &= x - 1;
-        uint32_t c = v % C;
-        uint32_t p = g->xbase[(size_t)mm * C + c] + ctr[c]++;
+  parallel_for(T, (uint64_t)M * C, 1, [&](uint64_t b, uint64_t e) {
+    for (uint64_t k = b; k < e; k++) {
+      const int mm = (int)(k / C);
+      const uint32_t c = (uint32_t)(k % C);
+      std::vector<uint64_t> key;  // slot << 32 | gid
+      key.reserve(g->xcnt[k]);
+      for (uint32_t wd = 0; wd < NW; wd++) {
+        uint64_t x = crossw[(uint64_t)mm * NW + wd];
+        while (x) {
+          const uint32_t v = wd * 64 + (uint32_t)__builtin_ctzll(x);
+          x &= x - 1;
+          if (v % C == c) key.push_back(((uint64_t)g->h_gid_slot[v] << 32) | v);
+        }
+      }
+      std::sort(key.begin(), key.end());
+      for (size_t i = 0; i < key.size(); i++) {
+        const uint32_t v = (uint32_t)key[i], p = g->xbase[k] + (uint32_t)i;
         xp[(uint64_t)mm * n + v] = p;
         g->h_proxy_gid[p] = v;
       }
     }
-  }
+  });
   std::vector<uint64_t>().swap(crossw);
 
   // per-slot and per-proxy arrays

@@ -166,15 +166,17 @@ __device__ __forceinline__ bool pull_apply(typename Op::V *upd, uint32_t r, type
 
 template <class Op>
 __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restrict__ blocks,
+                                                        const uint32_t *__restrict__ hot_base,
                                                         const uint32_t *__restrict__ crp,
                                                         const uint32_t *__restrict__ ccol,
                                                         const uint32_t *__restrict__ cw,
-                                                        const typename Op::V *__restrict__ mprop,
+                                                        const typename Op::V *__restrict__ mprop, uint32_t P,
                                                         typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty,
                                                         typename Op::V *__restrict__ partial)
 {
   using V = typename Op::V;
   __shared__ V s_v[kPullNnz];
+  __shared__ V s_hot[kPullHot];
   __shared__ unsigned long long s_dirty[kDirtyWords];
   const uint32_t bid = blockIdx.x;
   const PullBlock d = blocks[bid];
@@ -190,7 +192,8 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
   for (int i = threadIdx.x; i < kDirtyWords; i += kBlock) s_dirty[i] = 0ull;
   const uint32_t ne = d.e1 - d.e0;
   {
-    // batched stages: column ids (and weights), then the gathers
+    // batched stages: column ids (and weights), then the gathers; the
+    // partition's kPullHot highest-degree sources are read from LDS
     constexpr int kPer = kPullNnz / kBlock;
     uint32_t src[kPer], wt[kPer];
     V v[kPer];
@@ -200,10 +203,15 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
       src[k] = i < ne ? ccol[d.e0 + i] : 0u;
       wt[k] = (cw && i < ne) ? cw[d.e0 + i] : 1u;
     }
+    const uint32_t hb = hot_base[bid];
+    const uint32_t hn = min(kPullHot, P - hb);
+    for (uint32_t i = threadIdx.x; i < hn; i += kBlock) s_hot[i] = mprop[hb + i];
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
       uint32_t i = threadIdx.x + k * kBlock;
-      v[k] = i < ne ? Op::edge(mprop[src[k]], wt[k]) : op_identity<Op>();
+      const uint32_t h = src[k] - hb;
+      v[k] = i < ne ? Op::edge(h < hn ? s_hot[h] : mprop[src[k]], wt[k]) : op_identity<Op>();
     }
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
@@ -276,8 +284,8 @@ grd_status bsp_pull_launch(grd_state *s, const uint32_t *cw)
 {
   grd_graph *g = s->g;
   using V = typename Op::V;
-  hipLaunchKernelGGL(k_bsp_pull_rb<Op>, dim3(g->n_pblk), dim3(kBlock), 0, g->ctx->stream, g->pblk, g->crp, g->ccol, cw,
-                     (const V *)s->mprop, (V *)s->mupd, s->Du, (V *)s->partial);
+  hipLaunchKernelGGL(k_bsp_pull_rb<Op>, dim3(g->n_pblk), dim3(kBlock), 0, g->ctx->stream, g->pblk, g->pblk_hot, g->crp,
+                     g->ccol, cw, (const V *)s->mprop, g->P, (V *)s->mupd, s->Du, (V *)s->partial);
   GRD_HIP(hipGetLastError());
   return GRD_OK;
 }
