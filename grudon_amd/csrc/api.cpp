@@ -88,14 +88,18 @@ static grd_status finish_kernel_timing(grd_state *s, uint64_t frontier_k)
   return GRD_OK;
 }
 
-// one BSP superstep with the given option; reads the counters back
-static grd_status superstep(grd_state *s, int option)
+// one BSP superstep with the given option; reads the counters back.  With
+// select_next, the learned selector's decision for the next superstep is
+// computed on the device from the new counters and arrives with them.
+static grd_status superstep(grd_state *s, int option, bool select_next = false)
 {
   GRD_TRY(op_zero_counters(s));
   GRD_TRY(op_broadcast(s, option));
   GRD_TRY(op_traverse(s, option));
   GRD_TRY(op_reduce(s, option));
   GRD_TRY(op_update(s));
+  if (select_next) GRD_TRY(mlp_select_queue(s, s->last_frontier));
+  s->sel_queued = select_next;
   GRD_TRY(op_read_counters(s));
   GRD_TRY(finish_kernel_timing(s, s->last_frontier));
   const Counters &c = *s->hcnt;
@@ -120,6 +124,7 @@ static grd_status superstep(grd_state *s, int option)
       s->last_frontier_edges = s->hcnt->frontier_edges;
       s->last_maxdeg = s->hcnt->frontier_maxdeg;
     }
+    s->sel_queued = false;  // the next frontier is the new bucket's
   }
   return GRD_OK;
 }
@@ -419,6 +424,7 @@ grd_status grd_reduce(grd_state *s, int option)
 
 grd_status grd_update(grd_state *s, uint64_t *frontier_out)
 {
+  if (s) s->sel_queued = false;
   if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_update: state not initialized");
   if (s->algo == GRD_PR_PULL) {
     GRD_TRY(prp_update(s));
@@ -510,7 +516,8 @@ grd_status grd_run(grd_state *s, grd_run_stats *stats, grd_iter_rec *trace, uint
         } else {
           float margin = 0.0f;
           int o = h;
-          GRD_TRY(mlp_select(ctx, feat, &o, &margin));
+          if (s->sel_queued) mlp_select_read(ctx, &o, &margin);
+          else GRD_TRY(mlp_select(ctx, feat, &o, &margin));
           if (o != h && margin < ctx->sel_margin) {
             option = h;  // not confident enough to overrule the heuristic
             st.fallbacks++;
@@ -526,7 +533,7 @@ grd_status grd_run(grd_state *s, grd_run_stats *stats, grd_iter_rec *trace, uint
       rec.frontier_edges = s->last_frontier_edges;
       memcpy(rec.feat, feat, sizeof(feat));
       if (timing) GRD_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-      GRD_TRY(superstep(s, option));
+      GRD_TRY(superstep(s, option, s->prm.sched == GRD_SCHED_LEARNED && ctx->sel_loaded));
       if (timing) GRD_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
       rec.dirty = s->last_dirty;
       rec.next_frontier = s->last_frontier;

@@ -692,6 +692,7 @@ grd_status state_init(grd_state *s)
   s->bytes = 0;
   s->edges = 0;
   s->bcast_option = -1;
+  s->sel_queued = false;
   return GRD_OK;
 }
 

@@ -158,6 +158,7 @@ struct grd_state {
   uint64_t hi = ~0ull;  // SSSP delta-stepping bucket bound
   int last_option = 0;
   int bcast_option = -1;  // option of the last broadcast (the traversal must use the same direction)
+  bool sel_queued = false;  // learned: the next decision was computed on the device with the counters
   uint64_t last_frontier = 0, prev_frontier = 0, last_frontier_edges = 0, last_dirty = 0;
   uint32_t last_maxdeg = 0;
   uint64_t bytes = 0, edges = 0;
@@ -238,4 +239,12 @@ int heuristic_option(const grd_state *s);
 void make_features(const grd_state *s, float *f);
 // mlp.hip
 grd_status mlp_select(grd_ctx *ctx, const float *feat16_host, int *option, float *margin);
+// Device-side selection for the next superstep from the counters of the one
+// just run: computes the 16 features from the counters on the device, runs
+// the MLP and queues the logits' copy to the host, so that the counters'
+// readback of the superstep also brings the next decision.  prev_frontier is
+// the frontier size before the superstep.
+grd_status mlp_select_queue(grd_state *s, uint64_t prev_frontier);
+// Reads the decision queued by mlp_select_queue (after the stream sync).
+void mlp_select_read(const grd_ctx *ctx, int *option, float *margin);
 }  // namespace grd

@@ -237,14 +237,93 @@ __global__ void k_mlp_select(const uint16_t *__restrict__ wb, const float *__res
   }
 }
 
+// Graph-level inputs of the features (the per-superstep ones come from the
+// counters); the same formulas as make_features on the host.
+struct SelInputs {
+  double n, m, P, X, avg_deg, prev_frontier;
+  uint32_t M, C, algo, pad;
+};
+
+__global__ void k_mlp_select_state(const uint16_t *__restrict__ wb, const Counters *__restrict__ cnt, SelInputs in,
+                                   float *__restrict__ out)
+{
+  __shared__ float s_x[kIn];
+  __shared__ float s_h1[kWavesPerBlock][kH];
+  if (threadIdx.x == 0) {
+    const double fr = (double)cnt->frontier, fe = (double)cnt->frontier_edges;
+    const double m = in.m > 1.0 ? in.m : 1.0;
+    s_x[0] = (float)(log2(1.0 + fr) / 32.0);
+    s_x[1] = (float)(fr / in.n);
+    s_x[2] = (float)(fe / m);
+    s_x[3] = (float)(log2(1.0 + fe) / 40.0);
+    s_x[4] = (float)((double)cnt->dirty / in.P);
+    s_x[5] = (float)((double)cnt->frontier_maxdeg / (fe + 1.0));
+    s_x[6] = (float)fmin(4.0, (fr > 0 ? fe / fr : 0.0) / fmax(1e-9, in.avg_deg * 4.0));
+    s_x[7] = (float)(log2(in.n) / 32.0);
+    s_x[8] = (float)fmin(4.0, in.avg_deg / 64.0);
+    s_x[9] = (float)(in.M / 16.0);
+    s_x[10] = (float)(in.C / 16.0);
+    s_x[11] = (float)fmin(4.0, in.X / in.n / 4.0);
+    const double ratio = in.prev_frontier > 0 ? fr / in.prev_frontier : 1.0;
+    s_x[12] = (float)(fmax(-1.0, fmin(1.0, log2(fmax(ratio, 1e-9)) / 16.0)) * 0.5 + 0.5);
+    s_x[13] = in.algo == GRD_PR ? 1.0f : 0.0f;
+    s_x[14] = in.algo == GRD_CC ? 1.0f : 0.0f;
+    s_x[15] = in.algo == GRD_SSSP ? 1.0f : 0.0f;
+  }
+  __syncthreads();
+  float x[kIn];
+#pragma unroll
+  for (int i = 0; i < kIn; i++) x[i] = s_x[i];
+  Fwd r = forward_wave<true>(nullptr, wb, x, s_h1[wave_in_block()]);
+  if (threadIdx.x == 0) {
+    int best = 0;
+    for (int o = 0; o < kOut; o++) {
+      out[o] = r.logit[o];
+      if (r.logit[o] > r.logit[best]) best = o;
+    }
+    out[4] = (float)best;
+  }
+}
+
 // ------------------------------------------------------------------------
 
-grd_status mlp_select(grd_ctx *ctx, const float *feat16, int *option, float *margin)
+static grd_status sel_buffers(grd_ctx *ctx)
 {
   if (!ctx->sel_out) {
     GRD_HIP(hipMalloc((void **)&ctx->sel_out, 32 * sizeof(float)));
     GRD_HIP(hipHostMalloc((void **)&ctx->sel_out_host, 32 * sizeof(float), hipHostMallocDefault));
   }
+  return GRD_OK;
+}
+
+static void decode_logits(const float *o, int *option, float *margin)
+{
+  int best = (int)o[4];
+  float second = -1e30f;
+  for (int k = 0; k < kOut; k++)
+    if (k != best) second = std::max(second, o[k]);
+  *option = best;
+  *margin = o[best] - second;
+}
+
+grd_status mlp_select_queue(grd_state *s, uint64_t prev_frontier)
+{
+  grd_graph *g = s->g;
+  grd_ctx *ctx = g->ctx;
+  GRD_TRY(sel_buffers(ctx));
+  SelInputs in{(double)g->n, (double)g->m, (double)g->P, (double)g->X, g->avg_degree, (double)prev_frontier,
+               (uint32_t)g->M, (uint32_t)g->C, (uint32_t)s->algo, 0u};
+  hipLaunchKernelGGL(k_mlp_select_state, dim3(1), dim3(kWave), 0, ctx->stream, ctx->sel_w, s->cnt, in, ctx->sel_out);
+  GRD_HIP(hipGetLastError());
+  GRD_HIP(hipMemcpyAsync(ctx->sel_out_host, ctx->sel_out, 8 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  return GRD_OK;
+}
+
+void mlp_select_read(const grd_ctx *ctx, int *option, float *margin) { decode_logits(ctx->sel_out_host, option, margin); }
+
+grd_status mlp_select(grd_ctx *ctx, const float *feat16, int *option, float *margin)
+{
+  GRD_TRY(sel_buffers(ctx));
   memcpy(ctx->sel_out_host + 8, feat16, 16 * sizeof(float));
   GRD_HIP(hipMemcpyAsync(ctx->sel_out + 8, ctx->sel_out_host + 8, 16 * sizeof(float), hipMemcpyHostToDevice,
                          ctx->stream));
@@ -252,13 +331,7 @@ grd_status mlp_select(grd_ctx *ctx, const float *feat16, int *option, float *mar
   GRD_HIP(hipGetLastError());
   GRD_HIP(hipMemcpyAsync(ctx->sel_out_host, ctx->sel_out, 8 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   GRD_HIP(hipStreamSynchronize(ctx->stream));
-  const float *o = ctx->sel_out_host;
-  int best = (int)o[4];
-  float second = -1e30f;
-  for (int k = 0; k < kOut; k++)
-    if (k != best) second = std::max(second, o[k]);
-  *option = best;
-  *margin = o[best] - second;
+  decode_logits(ctx->sel_out_host, option, margin);
   return GRD_OK;
 }
 

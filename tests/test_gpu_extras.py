@@ -164,3 +164,28 @@ def test_gather_probe_runs(ctx):
     assert ms > 0 and rate > 1e8
     with pytest.raises(api.GrdError):
         ctx.probe_gather(0, 1 << 20)
+
+
+def test_learned_run_decisions_match_host_features(ctx):
+    """The learned run computes each next decision on the device from the
+    counters; it must equal the MLP applied to the host-side features that
+    the trace records (ties within 1e-3 of the top logit excepted)."""
+    _say("learned decisions")
+    w = np.fromfile(os.path.join(ROOT, "grudon_amd", "selector_bf16.bin"), dtype="<u2")
+    ctx.selector_load(w, min_margin=0.0)
+    for sym, algo, kw in [(True, api.CC, {}), (False, api.SSSP, {"source": 1, "weighted": True, "delta": 64}),
+                          (False, api.PR, {"max_iter": 40})]:
+        g = O.rmat(13, 16, seed=21, symmetrize=sym)
+        if algo == api.SSSP:
+            g = g.with_weights(3)
+        G = api.Graph.from_csr(ctx, g.rp, g.col, g.w, C=1, M=2)
+        S = api.State(G, algo)
+        S.init(sched=api.SCHED_LEARNED, **kw)
+        st, tr = S.run(trace_cap=2000)
+        feats = np.array([np.frombuffer(bytes(r.feat), dtype=np.float32) for r in tr])
+        logits = ctx.selector_infer(feats)
+        for r, lg in zip(tr, logits):
+            top = np.sort(lg)[-2:]
+            if top[1] - top[0] > 1e-3:
+                assert r.option == int(np.argmax(lg)), (algo, r.iteration)
+        S.free()
