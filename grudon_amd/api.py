@@ -19,7 +19,7 @@ MLP_PARAMS = 5508
 
 # every extern "C" symbol declared in include/grudon_hip.h
 EXPORTS = [
-    "grd_last_error", "grd_create", "grd_destroy", "grd_set_host_threads",
+    "grd_last_error", "grd_create", "grd_destroy", "grd_set_host_threads", "grd_set_layout", "grd_graph_digest",
     "grd_generate_rmat", "grd_generate_er", "grd_edge_weights", "grd_graph_from_csr", "grd_graph_rmat",
     "grd_graph_er", "grd_graph_load", "grd_graph_read", "grd_graph_write_gr", "grd_graph_free", "grd_graph_info",
     "grd_graph_table", "grd_graph_out_degree",
@@ -78,6 +78,7 @@ def lib():
         pp = ct.POINTER(ct.c_void_p)
         sigs = {
             "grd_create": [ct.c_int, pp], "grd_destroy": [vp], "grd_set_host_threads": [vp, ct.c_int],
+            "grd_set_layout": [vp, ct.c_int], "grd_graph_digest": [vp, vp],
             "grd_generate_rmat": [vp, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_double, ct.c_uint64,
                                   ct.c_int, vp, vp, ct.c_uint64, ct.POINTER(ct.c_uint64)],
             "grd_generate_er": [vp, ct.c_uint32, ct.c_double, ct.c_uint64, vp, vp, ct.c_uint64,
@@ -150,6 +151,10 @@ class Context:
 
     def set_host_threads(self, n):
         _check(lib().grd_set_host_threads(self.h, n), "grd_set_host_threads")
+
+    def set_layout(self, on_device):
+        """Build partitioned layouts with device kernels (True) or host threads."""
+        _check(lib().grd_set_layout(self.h, int(bool(on_device))), "grd_set_layout")
 
     def generate_rmat(self, scale, ef=16, seed=1, symmetrize=False, abc=RMAT_ABC):
         n = 1 << scale
@@ -267,6 +272,12 @@ class Graph:
         out = np.zeros(max(cnt.value, 1), dtype=np.uint32)
         _check(lib().grd_graph_table(self.h, c, m, _ptr(out), ct.byref(cnt)), "grd_graph_table")
         return out[: cnt.value]
+
+    def digest(self):
+        """16 digests of the partitioned layout (for comparing two builds)."""
+        out = np.zeros(16, dtype=np.uint64)
+        _check(lib().grd_graph_digest(self.h, _ptr(out)), "grd_graph_digest")
+        return out
 
     def out_degree(self):
         out = np.zeros(self.n, dtype=np.uint32)

@@ -129,6 +129,31 @@ static grd_status superstep(grd_state *s, int option, bool select_next = false)
   return GRD_OK;
 }
 
+// FNV-1a over the bytes of a device array (downloaded)
+template <typename T>
+static grd_status digest_dev(hipStream_t st, const T *p, uint64_t n, uint64_t *h)
+{
+  std::vector<T> v(n);
+  if (n && p) {
+    GRD_HIP(hipMemcpyAsync(v.data(), p, n * sizeof(T), hipMemcpyDeviceToHost, st));
+    GRD_HIP(hipStreamSynchronize(st));
+  }
+  const unsigned char *b = reinterpret_cast<const unsigned char *>(v.data());
+  uint64_t x = 1469598103934665603ull;
+  for (uint64_t i = 0; i < n * sizeof(T); i++) x = (x ^ b[i]) * 1099511628211ull;
+  *h = x;
+  return GRD_OK;
+}
+
+template <typename T>
+static uint64_t digest_host(const std::vector<T> &v)
+{
+  const unsigned char *b = reinterpret_cast<const unsigned char *>(v.data());
+  uint64_t x = 1469598103934665603ull;
+  for (uint64_t i = 0; i < v.size() * sizeof(T); i++) x = (x ^ b[i]) * 1099511628211ull;
+  return x;
+}
+
 }  // namespace grd
 
 using namespace grd;
@@ -173,6 +198,45 @@ grd_status grd_destroy(grd_ctx *c)
   if (c->sel_out_host) (void)hipHostFree(c->sel_out_host);
   (void)hipStreamDestroy(c->stream);
   delete c;
+  return GRD_OK;
+}
+
+grd_status grd_set_layout(grd_ctx *c, int on_device)
+{
+  if (!c) return fail(GRD_ERR_ARG, "grd_set_layout: NULL context");
+  c->layout_on_device = on_device != 0;
+  return GRD_OK;
+}
+
+grd_status grd_graph_digest(const grd_graph *g, uint64_t *d)
+{
+  if (!g || !d) return fail(GRD_ERR_ARG, "grd_graph_digest: NULL argument");
+  GRD_HIP(hipSetDevice(g->ctx->device));
+  hipStream_t st = g->ctx->stream;
+  const uint64_t S = g->S, P = g->P, m = g->m;
+  GRD_TRY(digest_dev(st, g->slot_gid, S, d + 0));
+  GRD_TRY(digest_dev(st, g->slot_deg, S, d + 1));
+  GRD_TRY(digest_dev(st, g->word_dst, S / 64, d + 2));
+  GRD_TRY(digest_dev(st, g->proxy_slot, P, d + 3));
+  GRD_TRY(digest_dev(st, g->rp, P + 1, d + 4));
+  GRD_TRY(digest_dev(st, g->col, m, d + 5));
+  GRD_TRY(digest_dev(st, g->w, g->weighted ? m : 0, d + 6));
+  GRD_TRY(digest_dev(st, g->crp, P + 1, d + 7));
+  GRD_TRY(digest_dev(st, g->ccol, m, d + 8));
+  GRD_TRY(digest_dev(st, g->cw, g->weighted ? m : 0, d + 9));
+  GRD_TRY(digest_dev(st, g->xptr, S + 1, d + 10));
+  GRD_TRY(digest_dev(st, g->xlist, g->X, d + 11));
+  uint64_t a = 0, b = 0, c = 0;
+  GRD_TRY(digest_dev(st, g->pblk, g->n_pblk, &a));
+  GRD_TRY(digest_dev(st, g->pblk_hot, g->n_pblk, &b));
+  GRD_TRY(digest_dev(st, g->plong, g->n_plong, &c));
+  d[12] = a ^ (b * 31) ^ (c * 131);
+  d[13] = digest_host(g->h_gid_slot);
+  d[14] = digest_host(g->h_proxy_gid) ^ (digest_host(g->h_deg) * 31);
+  std::vector<uint64_t> sc = {g->n, g->m, g->S, g->P, g->X, g->n_pblk, g->n_plong, g->n_sinks, g->max_deg};
+  for (auto *vec : {&g->cbase, &g->ccnt, &g->obase, &g->ocnt, &g->xbase, &g->xcnt})
+    for (uint32_t x : *vec) sc.push_back(x);
+  d[15] = digest_host(sc);
   return GRD_OK;
 }
 

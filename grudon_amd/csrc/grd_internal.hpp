@@ -105,6 +105,7 @@ struct grd_ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int host_threads = 0;
   int num_cus = 256;
+  bool layout_on_device = true;  // build the partitioned layout with device kernels (else on the host)
   // learned selector weights (BF16, device) and threshold
   uint16_t *sel_w = nullptr;
   bool sel_loaded = false;
@@ -198,6 +199,11 @@ struct grd_state {
 
 namespace grd {
 // layout.cpp
+grd_status check_row_ptr(uint32_t n, const uint32_t *rp);
+grd_status build_row_blocks(grd_graph *g, const std::vector<uint32_t> &crp);
+// layout_dev.hip: the same layout built by device kernels
+grd_status build_graph_device(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const uint32_t *col, const uint32_t *w,
+                              int C, int M, const uint32_t *mirror, grd_graph **out);
 grd_status build_graph(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const uint32_t *col, const uint32_t *w,
                        int C, int M, const uint32_t *mirror, grd_graph **out);
 

@@ -93,7 +93,7 @@ static grd_status upload(grd_graph *g, T **dst, const std::vector<T> &src)
 // holds at most kPullNnz arcs and kPullMaxRows rows; a row with more than
 // kPullNnz arcs becomes segments of kPullNnz arcs (one block each) plus a
 // PullLong entry that merges them in order.
-static grd_status build_row_blocks(grd_graph *g, const std::vector<uint32_t> &crp)
+grd_status build_row_blocks(grd_graph *g, const std::vector<uint32_t> &crp)
 {
   std::vector<PullBlock> blks;
   std::vector<PullLong> longs;
@@ -142,8 +142,7 @@ static grd_status build_impl(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const
     return fail(GRD_ERR_ARG, "graph: need n >= 1 and 1 <= C, M <= 64");
   const uint64_t m = rp[n];
   if (m >= 0xFFFFFFF0ull) return fail(GRD_ERR_ARG, "graph: more than 2^32-16 arcs is beyond the u32 offset limit");
-  for (uint32_t v = 0; v < n; v++)
-    if (rp[v + 1] < rp[v]) return fail(GRD_ERR_ARG, "graph: row_ptr is not monotone");
+  GRD_TRY(check_row_ptr(n, rp));
   const int T = host_threads(ctx);
   const uint32_t NW = (n + 63) / 64;
 
@@ -401,9 +400,18 @@ static grd_status build_impl(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const
   return GRD_OK;
 }
 
+grd_status check_row_ptr(uint32_t n, const uint32_t *rp)
+{
+  if (rp[0] != 0) return fail(GRD_ERR_ARG, "graph: row_ptr[0] must be 0");
+  for (uint32_t v = 0; v < n; v++)
+    if (rp[v + 1] < rp[v]) return fail(GRD_ERR_ARG, "graph: row_ptr is not monotone");
+  return GRD_OK;
+}
+
 grd_status build_graph(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const uint32_t *col, const uint32_t *w, int C,
                        int M, const uint32_t *mirror, grd_graph **out)
 {
+  if (ctx->layout_on_device) return build_graph_device(ctx, n, rp, col, w, C, M, mirror, out);
   try {
     return build_impl(ctx, n, rp, col, w, C, M, mirror, out);
   } catch (const std::bad_alloc &) {

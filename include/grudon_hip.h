@@ -104,6 +104,10 @@ grd_status grd_create(int device, grd_ctx **out);
 grd_status grd_destroy(grd_ctx *ctx);
 /* Host threads used for graph construction (reference -t, src/main.cpp:25). */
 grd_status grd_set_host_threads(grd_ctx *ctx, int nthreads);
+/* Where the partitioned layout (DistributedGraph ctor,
+ * src/DistributedGraph.cpp:13-785) is built: on_device != 0 (the default)
+ * with device kernels, 0 with host threads.  Both give the same layout. */
+grd_status grd_set_layout(grd_ctx *ctx, int on_device);
 
 /* ---- graphs ---- */
 
@@ -157,6 +161,10 @@ grd_status grd_graph_info(const grd_graph *g, uint64_t *info16);
 grd_status grd_graph_table(const grd_graph *g, int c, int m, uint32_t *out_gids, uint64_t *count);
 /* Out-degree per gid (host copy). */
 grd_status grd_graph_out_degree(const grd_graph *g, uint32_t *out);
+/* No counterpart in the reference: 16 FNV-1a digests of the partitioned
+ * layout (device arrays, host tables, sizes), so that two builds of the same
+ * graph can be compared (e.g. the device and the host construction). */
+grd_status grd_graph_digest(const grd_graph *g, uint64_t *digest16);
 
 /* ---- algorithm state: GraphAlgorithm<VP> (GraphAlgorithm.hpp:24-86) ---- */
 grd_status grd_state_create(grd_graph *g, int algo, grd_state **out);
