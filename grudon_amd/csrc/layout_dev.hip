@@ -94,16 +94,31 @@ __global__ void k_cross(uint64_t m, uint32_t n, const uint32_t *__restrict__ src
   }
 }
 
+// The per-pair counts below are few (C * M <= 4096): every workgroup counts
+// into LDS and adds its nonzero counts to the global array once.
+constexpr uint32_t kMaxPairs = 64 * 64;
+
 // own count per pair (c, m) -> ccnt[c*M + m]
 __global__ void k_own_count(uint32_t n, const uint8_t *__restrict__ mir, uint32_t C, uint32_t M, uint32_t *ccnt)
 {
-  GRID_LOOP(v, n) atomicAdd(&ccnt[(uint32_t)(v % C) * M + mir[v]], 1u);
+  __shared__ uint32_t s_c[kMaxPairs];
+  const uint32_t np = C * M;
+  for (uint32_t i = threadIdx.x; i < np; i += kBlock) s_c[i] = 0;
+  __syncthreads();
+  GRID_LOOP(v, n) atomicAdd(&s_c[(uint32_t)(v % C) * M + mir[v]], 1u);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < np; i += kBlock)
+    if (s_c[i]) atomicAdd(&ccnt[i], s_c[i]);
 }
 
 // cross count per region (m, c) -> xcnt[m*C + c] and set bits per word
 __global__ void k_cross_count(uint64_t nwords, uint32_t NW, const uint64_t *__restrict__ crossw, uint32_t C,
-                              uint32_t *xcnt, uint32_t *wcount)
+                              uint32_t M, uint32_t *xcnt, uint32_t *wcount)
 {
+  __shared__ uint32_t s_c[kMaxPairs];
+  const uint32_t np = C * M;
+  for (uint32_t i = threadIdx.x; i < np; i += kBlock) s_c[i] = 0;
+  __syncthreads();
   GRID_LOOP(i, nwords)
   {
     uint64_t x = crossw[i];
@@ -111,15 +126,18 @@ __global__ void k_cross_count(uint64_t nwords, uint32_t NW, const uint64_t *__re
     if (!x) continue;
     const uint32_t mm = (uint32_t)(i / NW);
     if (C == 1) {
-      atomicAdd(&xcnt[mm], (uint32_t)__popcll(x));
+      atomicAdd(&s_c[mm], (uint32_t)__popcll(x));
     } else {
       while (x) {
         const uint32_t v = (uint32_t)(i % NW) * 64 + (uint32_t)__ffsll((unsigned long long)x) - 1;
         x &= x - 1;
-        atomicAdd(&xcnt[mm * C + v % C], 1u);
+        atomicAdd(&s_c[mm * C + v % C], 1u);
       }
     }
   }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < np; i += kBlock)
+    if (s_c[i]) atomicAdd(&xcnt[i], s_c[i]);
 }
 
 // sort keys of the own slots: (pair << 32) | ~deg, value gid
@@ -228,9 +246,19 @@ __global__ void k_push_csr(uint64_t m, uint32_t n, const uint32_t *__restrict__ 
   }
 }
 
-__global__ void k_count32(uint64_t m, const uint32_t *__restrict__ key, uint32_t *cnt)
+// Row pointers of rows [0, R) from sorted row keys: ptr[r] = first position
+// with key >= r (ptr[R] = count).  Each row is written by exactly one thread:
+// the one at the first key of the gap that ends at it.
+__global__ void k_rowptr_sorted(uint64_t count, const uint32_t *__restrict__ key, uint32_t R, uint32_t *__restrict__ ptr)
 {
-  GRID_LOOP(e, m) atomicAdd(&cnt[key[e]], 1u);
+  GRID_LOOP(e, count)
+  {
+    const uint32_t k = key[e];
+    const uint64_t lo = e ? (uint64_t)key[e - 1] + 1 : 0;
+    for (uint64_t r = lo; r <= k; r++) ptr[r] = (uint32_t)e;
+    if (e + 1 == count)
+      for (uint64_t r = (uint64_t)k + 1; r <= R; r++) ptr[r] = (uint32_t)count;
+  }
 }
 
 __global__ void k_csc_unpack(uint64_t m, const uint64_t *__restrict__ val, uint32_t *__restrict__ ccol,
@@ -387,8 +415,8 @@ grd_status build_dev_impl(grd_ctx *ctx, uint32_t n, const uint32_t *h_rp, const 
   hipLaunchKernelGGL(k_fill32, dim3(1), dim3(kBlock), 0, st, xcnt, NP, 0u);
   hipLaunchKernelGGL(k_fill32, dim3(1), dim3(kBlock), 0, st, wcount + nwords, 1, 0u);
   hipLaunchKernelGGL(k_own_count, dim3(grid_for(n)), dim3(kBlock), 0, st, n, mir, (uint32_t)C, (uint32_t)M, ccnt);
-  hipLaunchKernelGGL(k_cross_count, dim3(grid_for(nwords)), dim3(kBlock), 0, st, nwords, NW, crossw, (uint32_t)C, xcnt,
-                     wcount);
+  hipLaunchKernelGGL(k_cross_count, dim3(grid_for(nwords)), dim3(kBlock), 0, st, nwords, NW, crossw, (uint32_t)C,
+                     (uint32_t)M, xcnt, wcount);
   GRD_TRY(exclusive_scan_u32(tmp, st, wcount, wpos, nwords + 1));
   GRD_TRY(download(st, g->ccnt, ccnt, NP));
   GRD_TRY(download(st, g->xcnt, xcnt, NP));
@@ -495,35 +523,31 @@ grd_status build_dev_impl(grd_ctx *ctx, uint32_t n, const uint32_t *h_rp, const 
   hipLaunchKernelGGL(k_push_csr, dim3(grid_for(m)), dim3(kBlock), 0, st, m, n, src, rp, col, w, mir, own_proxy, xp,
                      g->rp, g->col, g->w, pval);
   // pull CSC: stable sort of the push arcs (ascending source proxy) by target
-  uint32_t *ckey = nullptr, *ccnt_p = nullptr;
+  uint32_t *ckey = nullptr;
   GRD_TRY(tmp.alloc_n(&ckey, std::max<uint64_t>(m, 1)));
   if (m) GRD_HIP(hipMemcpyAsync(ckey, g->col, m * 4, hipMemcpyDeviceToDevice, st));
-  GRD_TRY(tmp.alloc_n(&ccnt_p, (uint64_t)P + 1));
-  hipLaunchKernelGGL(k_fill32, dim3(grid_for(P + 1)), dim3(kBlock), 0, st, ccnt_p, (uint64_t)P + 1, 0u);
-  hipLaunchKernelGGL(k_count32, dim3(grid_for(m)), dim3(kBlock), 0, st, m, ckey, ccnt_p);
-  GRD_TRY(gm.alloc_n(&g->crp, (uint64_t)P + 1));
-  GRD_TRY(exclusive_scan_u32(tmp, st, ccnt_p, g->crp, (uint64_t)P + 1));
   if (m) GRD_TRY(sort_pairs(tmp, st, ckey, pval, m, bits_of(P)));
+  GRD_TRY(gm.alloc_n(&g->crp, (uint64_t)P + 1));
+  if (m) hipLaunchKernelGGL(k_rowptr_sorted, dim3(grid_for(m)), dim3(kBlock), 0, st, m, ckey, P, g->crp);
+  else hipLaunchKernelGGL(k_fill32, dim3(grid_for(P + 1)), dim3(kBlock), 0, st, g->crp, (uint64_t)P + 1, 0u);
   GRD_TRY(gm.alloc_n(&g->ccol, std::max<uint64_t>(m, 1)));
   if (h_w) GRD_TRY(gm.alloc_n(&g->cw, std::max<uint64_t>(m, 1)));
   hipLaunchKernelGGL(k_csc_unpack, dim3(grid_for(m)), dim3(kBlock), 0, st, m, pval, g->ccol, g->cw);
   // cross proxies of every slot, in proxy order
   GRD_TRY(gm.alloc_n(&g->xptr, (uint64_t)S + 1));
   GRD_TRY(gm.alloc_n(&g->xlist, std::max<uint64_t>(X, 1)));
-  uint32_t *xcount = nullptr;
-  GRD_TRY(tmp.alloc_n(&xcount, (uint64_t)S + 1));
-  hipLaunchKernelGGL(k_fill32, dim3(grid_for(S + 1)), dim3(kBlock), 0, st, xcount, (uint64_t)S + 1, 0u);
   if (X) {
     uint32_t *lkey = nullptr, *lval = nullptr;
     GRD_TRY(tmp.alloc_n(&lkey, X));
     GRD_TRY(tmp.alloc_n(&lval, X));
     hipLaunchKernelGGL(k_cross_list, dim3(64, std::min<uint32_t>(NP, 65535)), dim3(kBlock), 0, st, NP, d_xbase, xcnt,
                        d_xstart, g->proxy_slot, lkey, lval);
-    hipLaunchKernelGGL(k_count32, dim3(grid_for(X)), dim3(kBlock), 0, st, X, lkey, xcount);
     GRD_TRY(sort_pairs(tmp, st, lkey, lval, X, bits_of(S)));
     GRD_HIP(hipMemcpyAsync(g->xlist, lval, X * 4, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_rowptr_sorted, dim3(grid_for(X)), dim3(kBlock), 0, st, X, lkey, S, g->xptr);
+  } else {
+    hipLaunchKernelGGL(k_fill32, dim3(grid_for(S + 1)), dim3(kBlock), 0, st, g->xptr, (uint64_t)S + 1, 0u);
   }
-  GRD_TRY(exclusive_scan_u32(tmp, st, xcount, g->xptr, (uint64_t)S + 1));
   GRD_HIP(hipGetLastError());
   // host copies
   std::vector<uint32_t> crp_h;

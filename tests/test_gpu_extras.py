@@ -156,6 +156,19 @@ def test_cli_runs_all_algorithms(ctx, tmp_path):
     assert p.returncode == 0, p.stdout
     gw = O.rmat(10, 8, seed=3, symmetrize=True).with_weights(5)
     assert np.array_equal(np.fromfile(res2, dtype=np.uint32), O.dijkstra(gw, 0))
+    # learned schedule with the packaged weights and a partition file
+    pf = tmp_path / "parts.txt"
+    mirror = (np.arange(g.n) * 5 + 1) % 3
+    pf.write_text("".join("%d\n" % x for x in mirror))
+    res3 = tmp_path / "cc3.bin"
+    p = subprocess.run([cli, "-g", "rmat:11:8:3:sym", "-c", "2", "-m", "3", "-p", str(pf), "-a", "cc", "--sched",
+                        "learned", "--selector", os.path.join(ROOT, "grudon_amd", "selector_bf16.bin"), "--verify",
+                        "--result", str(res3)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert p.returncode == 0, p.stdout
+    ref3, st3 = O.Partition(g, 2, 3, mirror=mirror.astype(np.uint32)).cc()
+    assert np.array_equal(np.fromfile(res3, dtype=np.uint32), ref3)
+    assert "STAT, CC, Iterations, %d" % int(st3[0]) in p.stdout
 
 
 def test_gather_probe_runs(ctx):
