@@ -216,15 +216,13 @@ template <typename V>
 __global__ void __launch_bounds__(kBlock) k_bcast_recv(uint32_t nw, const uint64_t *__restrict__ bv,
                                                        const V *__restrict__ val, const uint32_t *__restrict__ wrank,
                                                        const uint32_t *__restrict__ word_dst, int dense, int want_list,
-                                                       V ident, V *__restrict__ mprop, uint64_t *__restrict__ Dp,
-                                                       uint32_t *__restrict__ AL)
+                                                       V ident, V *__restrict__ mprop, uint32_t *__restrict__ AL)
 {
   const uint32_t base = group_base();
   const int l = lane_id();
   const bool ok = l < (int)kGroupWords && base + l < nw;
   const uint32_t pwl = ok ? word_dst[base + l] : 0xFFFFFFFFu;
   const uint64_t mine = (pwl != 0xFFFFFFFFu) ? bv[base + l] : 0ull;
-  if (pwl != 0xFFFFFFFFu) Dp[pwl] = mine;
   uint64_t todo = want_list ? __ballot(mine != 0) : __ballot(pwl != 0xFFFFFFFFu);
   while (todo) {
     const int j = __ffsll((unsigned long long)todo) - 1;
@@ -598,7 +596,6 @@ grd_status state_alloc(grd_state *s)
   GRD_TRY(m.alloc_n(&s->Pend, nwS));
   GRD_TRY(m.alloc(&s->mprop, P * V));
   GRD_TRY(m.alloc(&s->mupd, P * V));
-  GRD_TRY(m.alloc_n(&s->Dp, nwP));
   GRD_TRY(m.alloc_n(&s->Du, nwP));
   GRD_TRY(m.alloc_n(&s->bvB, nwS));
   GRD_TRY(m.alloc_n(&s->bvR, nwP));
@@ -619,9 +616,9 @@ grd_status state_alloc(grd_state *s)
   memset(s->hcnt, 0, sizeof(Counters));
   // snapshot list: every array whose content carries state between iterations
   s->snap = {{s->prop, nullptr}, {s->upd, nullptr}, {s->F, nullptr},     {s->Dc, nullptr},
-             {s->Pend, nullptr}, {s->mprop, nullptr}, {s->mupd, nullptr}, {s->Dp, nullptr},
+             {s->Pend, nullptr}, {s->mprop, nullptr}, {s->mupd, nullptr},
              {s->Du, nullptr}};
-  s->snap_bytes = {S * V, S * V, nwS * 8ull, nwS * 8ull, nwS * 8ull, P * V, P * V, nwP * 8ull, nwP * 8ull};
+  s->snap_bytes = {S * V, S * V, nwS * 8ull, nwS * 8ull, nwS * 8ull, P * V, P * V, nwP * 8ull};
   if (s->pr) {
     s->snap.push_back({s->pr, nullptr});
     s->snap_bytes.push_back(S * 4);
@@ -651,7 +648,6 @@ grd_status state_init(grd_state *s)
   const uint32_t nwS = g->S / 64, nwP = g->P / 64;
   GRD_TRY(fill_u64(st, s->Dc, nwS, 0));
   GRD_TRY(fill_u64(st, s->Pend, nwS, 0));
-  GRD_TRY(fill_u64(st, s->Dp, nwP, 0));
   GRD_TRY(fill_u64(st, s->Du, nwP, 0));
   GRD_TRY(fill_u64(st, s->bvB, nwS, 0));
   GRD_TRY(fill_u64(st, s->bvR, nwP, 0));
@@ -721,7 +717,7 @@ grd_status op_broadcast(grd_state *s, int option)
                      s->wrankB, (int)dense, s->bvB, (uint32_t *)s->valB);
   const uint32_t ident = (s->algo == GRD_PR) ? 0u : 0xFFFFFFFFu;  // 0.0f or the min identity
   hipLaunchKernelGGL(k_bcast_recv<uint32_t>, dim3(grid), dim3(kBlock), 0, st, nwS, s->bvB, (const uint32_t *)s->valB,
-                     s->wrankB, g->word_dst, (int)dense, (int)!pull, ident, (uint32_t *)s->mprop, s->Dp, s->AL);
+                     s->wrankB, g->word_dst, (int)dense, (int)!pull, ident, (uint32_t *)s->mprop, s->AL);
   GRD_HIP(hipGetLastError());
   // exchange bytes (GraphAlgorithm.cpp:251-265): bitvector of every pair +
   // packed values (sparse) or the pair's full own slice (dense)

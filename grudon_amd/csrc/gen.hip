@@ -72,13 +72,28 @@ __global__ void __launch_bounds__(kBlock) k_keep(const uint64_t *__restrict__ ke
 
 __global__ void __launch_bounds__(kBlock) k_compact(const uint64_t *__restrict__ keys, uint64_t cnt, uint64_t n,
                                                     const uint32_t *__restrict__ keep, const uint32_t *__restrict__ pos,
-                                                    uint32_t *__restrict__ col, uint32_t *__restrict__ deg)
+                                                    uint32_t *__restrict__ col, uint32_t *__restrict__ src)
 {
   for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < cnt; i += (uint64_t)gridDim.x * kBlock) {
     if (!keep[i]) continue;
     uint64_t k = keys[i];
     col[pos[i]] = (uint32_t)(k % n);
-    atomicAdd(deg + (k / n), 1u);
+    src[pos[i]] = (uint32_t)(k / n);
+  }
+}
+
+// row pointers of the kept arcs, whose sources ascend: the first arc of each
+// source writes its index into the row heads of that source and of the empty
+// rows before it, so every entry of rp[0..n] is written exactly once
+__global__ void __launch_bounds__(kBlock) k_gen_rowptr(const uint32_t *__restrict__ src, uint64_t m, uint64_t n,
+                                                       uint32_t *__restrict__ rp)
+{
+  for (uint64_t j = blockIdx.x * (uint64_t)kBlock + threadIdx.x; j < m; j += (uint64_t)gridDim.x * kBlock) {
+    const uint32_t s = src[j];
+    const uint64_t lo = j ? (uint64_t)src[j - 1] + 1 : 0;
+    for (uint64_t v = lo; v <= s; v++) rp[v] = (uint32_t)j;
+    if (j + 1 == m)
+      for (uint64_t v = (uint64_t)s + 1; v <= n; v++) rp[v] = (uint32_t)m;
   }
 }
 
@@ -105,7 +120,7 @@ static grd_status keys_to_csr(grd_ctx *ctx, uint64_t *d_keys, uint64_t cnt, uint
   GRD_TRY(mem.alloc(&tmp, tmp_bytes));
   GRD_HIP(rocprim::radix_sort_keys(tmp, tmp_bytes, db, (size_t)cnt, 0, end_bit, st));
   uint64_t *keys = db.current();
-  uint32_t *keep = nullptr, *pos = nullptr, *d_col = nullptr, *d_deg = nullptr, *d_rp = nullptr;
+  uint32_t *keep = nullptr, *pos = nullptr, *d_col = nullptr, *d_src = nullptr, *d_rp = nullptr;
   GRD_TRY(mem.alloc_n(&keep, cnt + 1));
   GRD_TRY(mem.alloc_n(&pos, cnt + 1));
   uint32_t grid = (uint32_t)std::min<uint64_t>(65535, (cnt + kBlock - 1) / kBlock + 1);
@@ -121,16 +136,17 @@ static grd_status keys_to_csr(grd_ctx *ctx, uint64_t *d_keys, uint64_t cnt, uint
   GRD_HIP(hipMemcpyAsync(&m, pos + cnt, 4, hipMemcpyDeviceToHost, st));
   GRD_HIP(hipStreamSynchronize(st));
   GRD_TRY(mem.alloc_n(&d_col, std::max<uint32_t>(m, 1)));
-  GRD_TRY(mem.alloc_n(&d_deg, n + 1));
+  GRD_TRY(mem.alloc_n(&d_src, std::max<uint32_t>(m, 1)));
   GRD_TRY(mem.alloc_n(&d_rp, n + 1));
-  GRD_HIP(hipMemsetAsync(d_deg, 0, (n + 1) * 4, st));
-  hipLaunchKernelGGL(k_compact, dim3(grid), dim3(kBlock), 0, st, keys, cnt, n, keep, pos, d_col, d_deg);
+  hipLaunchKernelGGL(k_compact, dim3(grid), dim3(kBlock), 0, st, keys, cnt, n, keep, pos, d_col, d_src);
   GRD_HIP(hipGetLastError());
-  size_t sb2 = 0;
-  GRD_HIP(rocprim::exclusive_scan(nullptr, sb2, d_deg, d_rp, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(), st));
-  void *stmp2 = nullptr;
-  GRD_TRY(mem.alloc(&stmp2, sb2));
-  GRD_HIP(rocprim::exclusive_scan(stmp2, sb2, d_deg, d_rp, 0u, (size_t)n + 1, rocprim::plus<uint32_t>(), st));
+  if (m) {
+    uint32_t g2 = (uint32_t)std::min<uint64_t>(65535, ((uint64_t)m + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_gen_rowptr, dim3(g2), dim3(kBlock), 0, st, d_src, (uint64_t)m, n, d_rp);
+    GRD_HIP(hipGetLastError());
+  } else {
+    GRD_HIP(hipMemsetAsync(d_rp, 0, (n + 1) * 4, st));
+  }
   rp.resize(n + 1);
   col.resize(m);
   GRD_HIP(hipMemcpyAsync(rp.data(), d_rp, (n + 1) * 4, hipMemcpyDeviceToHost, st));

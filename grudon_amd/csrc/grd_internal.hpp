@@ -177,7 +177,7 @@ struct grd_state {
   uint64_t *F = nullptr, *Dc = nullptr, *Pend = nullptr;  // [S/64]
   // memory side [P]
   void *mprop = nullptr, *mupd = nullptr;
-  uint64_t *Dp = nullptr, *Du = nullptr;  // [P/64]
+  uint64_t *Du = nullptr;  // [P/64] dirty proxies
   // wire
   uint64_t *bvB = nullptr, *bvR = nullptr;
   void *valB = nullptr, *valR = nullptr;
