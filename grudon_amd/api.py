@@ -12,8 +12,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libgrudon_hip.so")
 
 PR, CC, SSSP, PR_PULL = 0, 1, 2, 3
-OPT_PUSH_SPARSE, OPT_PULL_SPARSE, OPT_PUSH_DENSE, OPT_PULL_DENSE = 0, 1, 2, 3
-SCHED_PLAIN, SCHED_HEURISTIC, SCHED_LEARNED, SCHED_FIXED = 0, 1, 2, 3
+OPT_PUSH_SPARSE, OPT_PULL_SPARSE, OPT_PUSH_DENSE, OPT_PULL_DENSE, OPT_SHIP_EDGES = 0, 1, 2, 3, 4
+SCHED_PLAIN, SCHED_HEURISTIC, SCHED_LEARNED, SCHED_FIXED, SCHED_NDP_ENGINE = 0, 1, 2, 3, 4
 RMAT_ABC = (0.57, 0.19, 0.19)
 MLP_PARAMS = 5508
 
@@ -22,7 +22,7 @@ EXPORTS = [
     "grd_last_error", "grd_create", "grd_destroy", "grd_set_host_threads", "grd_set_layout", "grd_graph_digest",
     "grd_generate_rmat", "grd_generate_er", "grd_edge_weights", "grd_graph_from_csr", "grd_graph_rmat",
     "grd_graph_er", "grd_graph_load", "grd_graph_read", "grd_graph_write_gr", "grd_graph_free", "grd_graph_info",
-    "grd_graph_table", "grd_graph_out_degree",
+    "grd_graph_table", "grd_graph_out_degree", "grd_graph_coverage",
     "grd_state_create", "grd_state_free", "grd_init", "grd_broadcast", "grd_traverse", "grd_reduce",
     "grd_update", "grd_aggregate_one", "grd_run", "grd_select_option", "grd_state_stats", "grd_download",
     "grd_features", "grd_snapshot",
@@ -48,7 +48,7 @@ class IterRec(ct.Structure):
 class RunStats(ct.Structure):
     _fields_ = [("iterations", ct.c_uint64), ("bytes_exchanged", ct.c_uint64), ("edges_examined", ct.c_uint64),
                 ("hit_max_iter", ct.c_uint64), ("ms", ct.c_double), ("residual", ct.c_double),
-                ("option_counts", ct.c_uint64 * 4), ("fallbacks", ct.c_uint64), ("kernel_ms", ct.c_double),
+                ("option_counts", ct.c_uint64 * 5), ("fallbacks", ct.c_uint64), ("kernel_ms", ct.c_double),
                 ("kernel_launches", ct.c_uint64), ("kernel_bytes", ct.c_uint64)]
 
     def as_dict(self):
@@ -90,7 +90,7 @@ def lib():
             "grd_graph_load": [vp, ct.c_char_p, ct.c_char_p, ct.c_int, ct.c_int, pp],
             "grd_graph_free": [vp], "grd_graph_info": [vp, vp], "grd_graph_table": [vp, ct.c_int, ct.c_int, vp,
                                                                                    ct.POINTER(ct.c_uint64)],
-            "grd_graph_out_degree": [vp, vp],
+            "grd_graph_out_degree": [vp, vp], "grd_graph_coverage": [vp, vp, ct.POINTER(ct.c_uint64)],
             "grd_edge_weights": [vp, ct.c_uint32, vp, vp, ct.c_uint64, vp],
             "grd_graph_read": [ct.c_char_p, ct.POINTER(ct.c_uint32), ct.POINTER(ct.c_uint64), vp, vp, vp,
                                ct.POINTER(ct.c_int)],
@@ -283,6 +283,13 @@ class Graph:
         out = np.zeros(self.n, dtype=np.uint32)
         _check(lib().grd_graph_out_degree(self.h, _ptr(out)), "grd_graph_out_degree")
         return out
+
+    def coverage(self):
+        """(covered[gid] as uint8, number of vertices not covered)."""
+        out = np.zeros(self.n, dtype=np.uint8)
+        cnt = ct.c_uint64()
+        _check(lib().grd_graph_coverage(self.h, _ptr(out), ct.byref(cnt)), "grd_graph_coverage")
+        return out, int(cnt.value)
 
     def free(self):
         if self.h:

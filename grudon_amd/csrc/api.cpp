@@ -36,6 +36,44 @@ int heuristic_option(const grd_state *s)
   return (pull ? 1 : 0) | (dense ? 2 : 0);
 }
 
+// NDPEngine (include/offload_engine/NDPEngine.hpp:10-51) on every compute
+// partition, combined by the MIN allReduce of run() (NDP_OFFLOAD = 0 wins,
+// src/GraphAlgorithm.cpp:186-187): ship the edges (NO_OFFLOAD) only when every
+// partition's frontier is non-empty, at most n/20 (GraphAlgorithm.cpp:166,
+// NDPEngine.hpp:17-21) and covered (:23-26).  The mirror-balance test of
+// :28-46 cannot vote NDP: the largest per-mirror count is never below the
+// mean.  Coverage is the evident intent of DistributedGraph.cpp:338-343 (all
+// out-neighbours share the master); the reference's guard there only clears
+// a flag that is already clear, so its vector stays all true.
+static bool ndp_engine_ships(const grd_state *s)
+{
+  if (s->last_uncovered) return false;
+  const uint64_t thr = s->g->n / 20;
+  for (int c = 0; c < s->g->C; c++) {
+    const uint64_t f = s->last_fpart[c];
+    if (f == 0 || f > thr) return false;
+  }
+  return true;
+}
+
+// the option the non-learned schedules run next; *fallback = a fixed edge
+// shipping that an uncovered frontier turned into the offloaded push
+static int fixed_schedule_option(const grd_state *s, bool *fallback)
+{
+  *fallback = false;
+  switch (s->prm.sched) {
+  case GRD_SCHED_FIXED:
+    if (s->prm.fixed_option == GRD_OPT_SHIP_EDGES && s->last_uncovered) {
+      *fallback = true;
+      return GRD_OPT_PUSH_SPARSE;
+    }
+    return (int)s->prm.fixed_option;
+  case GRD_SCHED_HEURISTIC: return heuristic_option(s);
+  case GRD_SCHED_NDP_ENGINE: return ndp_engine_ships(s) ? GRD_OPT_SHIP_EDGES : GRD_OPT_PUSH_SPARSE;
+  default: return GRD_OPT_PUSH_SPARSE;
+  }
+}
+
 // 16 selector features of the state before an iteration
 void make_features(const grd_state *s, float *f)
 {
@@ -93,10 +131,18 @@ static grd_status finish_kernel_timing(grd_state *s, uint64_t frontier_k)
 // computed on the device from the new counters and arrives with them.
 static grd_status superstep(grd_state *s, int option, bool select_next = false)
 {
+  const bool ship = option == GRD_OPT_SHIP_EDGES;
   GRD_TRY(op_zero_counters(s));
-  GRD_TRY(op_broadcast(s, option));
-  GRD_TRY(op_traverse(s, option));
-  GRD_TRY(op_reduce(s, option));
+  // the edge-shipping schedule broadcasts like the offloaded push
+  // (GraphAlgorithm.cpp:199-343 runs for both decisions), then ships the
+  // frontier's edge lists instead of generating and reducing updates
+  GRD_TRY(op_broadcast(s, ship ? GRD_OPT_PUSH_SPARSE : option));
+  if (ship) {
+    GRD_TRY(op_ship(s));
+  } else {
+    GRD_TRY(op_traverse(s, option));
+    GRD_TRY(op_reduce(s, option));
+  }
   GRD_TRY(op_update(s));
   if (select_next) GRD_TRY(mlp_select_queue(s, s->last_frontier));
   s->sel_queued = select_next;
@@ -104,10 +150,14 @@ static grd_status superstep(grd_state *s, int option, bool select_next = false)
   GRD_TRY(finish_kernel_timing(s, s->last_frontier));
   const Counters &c = *s->hcnt;
   const grd_graph *g = s->g;
+  if (c.ship_error) return fail(GRD_ERR_STATE, "edge shipping: the active list differs from the frontier");
   // reduce bytes (GraphAlgorithm.cpp:409-423): bitvector + dirty values
-  // (sparse) or every translation-table entry (dense)
-  uint64_t vals = (option & 2) ? (uint64_t)g->n + g->X : c.dirty;
-  s->bytes += bitvector_bytes(g) + vals * s->vbytes;
+  // (sparse) or every translation-table entry (dense); a shipping superstep
+  // has no reduce (its wire bytes were counted by op_ship)
+  if (!ship) {
+    uint64_t vals = (option & 2) ? (uint64_t)g->n + g->X : c.dirty;
+    s->bytes += bitvector_bytes(g) + vals * s->vbytes;
+  }
   s->edges += c.edges + c.heavy_edges;
   s->prev_frontier = s->last_frontier;
   s->last_dirty = c.dirty;
@@ -116,6 +166,7 @@ static grd_status superstep(grd_state *s, int option, bool select_next = false)
   s->last_maxdeg = c.frontier_maxdeg;
   s->last_option = option;
   s->iteration++;
+  take_frontier_parts(s);
   if (c.frontier == 0 && s->algo == GRD_SSSP && s->prm.delta) {
     bool any = false;
     GRD_TRY(op_select_bucket(s, &any));
@@ -123,6 +174,7 @@ static grd_status superstep(grd_state *s, int option, bool select_next = false)
       s->last_frontier = s->hcnt->frontier;
       s->last_frontier_edges = s->hcnt->frontier_edges;
       s->last_maxdeg = s->hcnt->frontier_maxdeg;
+      take_frontier_parts(s);
     }
     s->sel_queued = false;  // the next frontier is the new bucket's
   }
@@ -398,6 +450,22 @@ grd_status grd_graph_out_degree(const grd_graph *g, uint32_t *out)
   return GRD_OK;
 }
 
+grd_status grd_graph_coverage(const grd_graph *g, uint8_t *covered, uint64_t *n_uncovered)
+{
+  if (!g) return fail(GRD_ERR_ARG, "grd_graph_coverage: NULL graph");
+  if (n_uncovered) *n_uncovered = g->n_uncovered;
+  if (!covered) return GRD_OK;
+  GRD_HIP(hipSetDevice(g->ctx->device));
+  const uint32_t nwS = g->S / 64;
+  std::vector<uint64_t> u(std::max<uint32_t>(nwS, 1));
+  GRD_HIP(hipMemcpy(u.data(), g->uncov, (uint64_t)nwS * 8, hipMemcpyDeviceToHost));
+  for (uint32_t v = 0; v < g->n; v++) {
+    const uint32_t sl = g->h_gid_slot[v];
+    covered[v] = ((u[sl >> 6] >> (sl & 63)) & 1ull) ? 0 : 1;
+  }
+  return GRD_OK;
+}
+
 grd_status grd_state_create(grd_graph *g, int algo, grd_state **out)
 {
   if (!g || !out) return fail(GRD_ERR_ARG, "grd_state_create: NULL argument");
@@ -438,7 +506,8 @@ grd_status grd_init(grd_state *s, const grd_params *p)
   if (s->algo == GRD_SSSP && p->source >= s->g->n) return fail(GRD_ERR_ARG, "grd_init: SSSP source out of range");
   if (s->algo == GRD_SSSP && p->weighted && !s->g->weighted)
     return fail(GRD_ERR_ARG, "grd_init: weighted SSSP on a graph without weights");
-  if (p->sched > GRD_SCHED_FIXED || p->fixed_option > 3) return fail(GRD_ERR_ARG, "grd_init: bad schedule");
+  if (p->sched > GRD_SCHED_NDP_ENGINE || p->fixed_option > GRD_OPT_SHIP_EDGES)
+    return fail(GRD_ERR_ARG, "grd_init: bad schedule");
   s->prm = *p;
   s->kern_ms = 0.0;
   s->kern_launches = s->kern_bytes = 0;
@@ -456,33 +525,43 @@ grd_status grd_init(grd_state *s, const grd_params *p)
 grd_status grd_broadcast(grd_state *s, int option)
 {
   if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_broadcast: state not initialized");
-  if (option < 0 || option > 3) return fail(GRD_ERR_ARG, "grd_broadcast: bad option");
+  if (option < 0 || option > GRD_OPT_SHIP_EDGES) return fail(GRD_ERR_ARG, "grd_broadcast: bad option");
   if (s->algo == GRD_PR_PULL) return prp_broadcast(s);
   GRD_TRY(op_zero_counters(s));
   s->bcast_option = option;
-  return op_broadcast(s, option);
+  return op_broadcast(s, option == GRD_OPT_SHIP_EDGES ? GRD_OPT_PUSH_SPARSE : option);
 }
 
 grd_status grd_traverse(grd_state *s, int option)
 {
   if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_traverse: state not initialized");
-  if (option < 0 || option > 3) return fail(GRD_ERR_ARG, "grd_traverse: bad option");
+  if (option < 0 || option > GRD_OPT_SHIP_EDGES) return fail(GRD_ERR_ARG, "grd_traverse: bad option");
   if (s->algo == GRD_PR_PULL) return prp_traverse(s);
   // the broadcast prepared the memory side for one direction: the active
-  // list for a push, every own proxy (identity when inactive) for a pull
+  // list for a push (or edge shipping), every own proxy (identity when
+  // inactive) for a pull
   if (s->bcast_option < 0 || ((s->bcast_option ^ option) & 1))
     return fail(GRD_ERR_STATE, "grd_traverse: direction differs from the preceding grd_broadcast");
+  if (option == GRD_OPT_SHIP_EDGES) {
+    // a compute partition can only apply updates to its own vertices
+    if (s->last_uncovered)
+      return fail(GRD_ERR_STATE, "grd_traverse: edge shipping needs every frontier vertex covered (" +
+                                     std::to_string(s->last_uncovered) + " are not)");
+    return op_ship(s);
+  }
   return op_traverse(s, option);
 }
 
 grd_status grd_reduce(grd_state *s, int option)
 {
   if (!s || !s->initialized) return fail(GRD_ERR_STATE, "grd_reduce: state not initialized");
-  if (option < 0 || option > 3) return fail(GRD_ERR_ARG, "grd_reduce: bad option");
+  if (option < 0 || option > GRD_OPT_SHIP_EDGES) return fail(GRD_ERR_ARG, "grd_reduce: bad option");
   // PR_PULL: the pull already wrote every proxy's partial sum into the
   // dense reduce buffer; the compute side combines them in grd_update
   if (s->algo == GRD_PR_PULL) return GRD_OK;
   s->last_option = option;
+  // edge shipping: the compute side applied the updates during the traversal
+  if (option == GRD_OPT_SHIP_EDGES) return GRD_OK;
   return op_reduce(s, option);
 }
 
@@ -503,8 +582,11 @@ grd_status grd_update(grd_state *s, uint64_t *frontier_out)
   GRD_TRY(op_read_counters(s));
   GRD_TRY(finish_kernel_timing(s, s->last_frontier));
   const Counters &c = *s->hcnt;
-  uint64_t vals = (s->last_option & 2) ? (uint64_t)s->g->n + s->g->X : c.dirty;
-  s->bytes += bitvector_bytes(s->g) + vals * s->vbytes;
+  if (c.ship_error) return fail(GRD_ERR_STATE, "edge shipping: the active list differs from the frontier");
+  if (s->last_option != GRD_OPT_SHIP_EDGES) {
+    uint64_t vals = (s->last_option & 2) ? (uint64_t)s->g->n + s->g->X : c.dirty;
+    s->bytes += bitvector_bytes(s->g) + vals * s->vbytes;
+  }
   s->edges += c.edges + c.heavy_edges;
   s->prev_frontier = s->last_frontier;
   s->last_dirty = c.dirty;
@@ -512,12 +594,15 @@ grd_status grd_update(grd_state *s, uint64_t *frontier_out)
   s->last_frontier_edges = c.frontier_edges;
   s->last_maxdeg = c.frontier_maxdeg;
   s->iteration++;
+  take_frontier_parts(s);
   if (c.frontier == 0 && s->algo == GRD_SSSP && s->prm.delta) {
     bool any = false;
     GRD_TRY(op_select_bucket(s, &any));
     if (any) {
       s->last_frontier = s->hcnt->frontier;
       s->last_frontier_edges = s->hcnt->frontier_edges;
+      s->last_maxdeg = s->hcnt->frontier_maxdeg;
+      take_frontier_parts(s);
     }
   }
   if (frontier_out) *frontier_out = s->last_frontier;
@@ -570,8 +655,14 @@ grd_status grd_run(grd_state *s, grd_run_stats *stats, grd_iter_rec *trace, uint
       int option = GRD_OPT_PUSH_SPARSE;
       switch (s->prm.sched) {
       case GRD_SCHED_PLAIN: option = GRD_OPT_PUSH_SPARSE; break;
-      case GRD_SCHED_FIXED: option = (int)s->prm.fixed_option; break;
-      case GRD_SCHED_HEURISTIC: option = heuristic_option(s); break;
+      case GRD_SCHED_FIXED:
+      case GRD_SCHED_HEURISTIC:
+      case GRD_SCHED_NDP_ENGINE: {
+        bool fb = false;
+        option = fixed_schedule_option(s, &fb);
+        if (fb) st.fallbacks++;
+        break;
+      }
       case GRD_SCHED_LEARNED: {
         int h = heuristic_option(s);
         if (!ctx->sel_loaded) {
@@ -637,8 +728,13 @@ grd_status grd_select_option(grd_state *s, int *option)
   grd_ctx *ctx = s->g->ctx;
   switch (s->prm.sched) {
   case GRD_SCHED_PLAIN: *option = GRD_OPT_PUSH_SPARSE; return GRD_OK;
-  case GRD_SCHED_FIXED: *option = (int)s->prm.fixed_option; return GRD_OK;
-  case GRD_SCHED_HEURISTIC: *option = heuristic_option(s); return GRD_OK;
+  case GRD_SCHED_FIXED:
+  case GRD_SCHED_HEURISTIC:
+  case GRD_SCHED_NDP_ENGINE: {
+    bool fb = false;
+    *option = fixed_schedule_option(s, &fb);
+    return GRD_OK;
+  }
   default: {
     int h = heuristic_option(s), o = h;
     if (ctx->sel_loaded) {
@@ -724,6 +820,9 @@ grd_status grd_snapshot(grd_state *s)
   GRD_HIP(hipStreamSynchronize(st));
   s->snap_iteration = s->iteration;
   s->snap_hi = s->hi;
+  s->snap_host = {s->last_frontier, s->prev_frontier, s->last_frontier_edges, s->last_dirty, s->last_maxdeg,
+                  s->last_uncovered};
+  memcpy(s->snap_fpart, s->last_fpart, sizeof(s->snap_fpart));
   return GRD_OK;
 }
 
@@ -738,6 +837,16 @@ grd_status grd_restore(grd_state *s)
   GRD_HIP(hipStreamSynchronize(st));
   s->iteration = s->snap_iteration;
   s->hi = s->snap_hi;
+  // the frontier statistics the next decision and the wire sizing use
+  s->last_frontier = s->snap_host[0];
+  s->prev_frontier = s->snap_host[1];
+  s->last_frontier_edges = s->snap_host[2];
+  s->last_dirty = s->snap_host[3];
+  s->last_maxdeg = (uint32_t)s->snap_host[4];
+  s->last_uncovered = s->snap_host[5];
+  memcpy(s->last_fpart, s->snap_fpart, sizeof(s->snap_fpart));
+  s->sel_queued = false;
+  s->bcast_option = -1;
   return GRD_OK;
 }
 

@@ -542,7 +542,10 @@ __global__ void k_zero_counters(Counters *cnt)
     cnt->edges = 0;
     cnt->frontier_maxdeg = 0;
     cnt->pend_min = 0xFFFFFFFFu;
+    cnt->frontier_uncovered = 0;
+    cnt->ship_error = 0;
   }
+  for (uint32_t c = threadIdx.x; c < (uint32_t)kMaxParts; c += blockDim.x) cnt->fpart[c] = 0;
 }
 
 // single-element aggregate() (API compatibility path)
@@ -689,6 +692,11 @@ grd_status state_init(grd_state *s)
   s->edges = 0;
   s->bcast_option = -1;
   s->sel_queued = false;
+  if (g->C > 1) {  // per-partition frontier sizes for the first NDPEngine vote
+    GRD_TRY(op_frontier_parts(s));
+    GRD_TRY(op_read_counters(s));
+  }
+  take_frontier_parts(s);
   return GRD_OK;
 }
 
@@ -824,7 +832,19 @@ grd_status op_update(grd_state *s)
   default: return fail(GRD_ERR_STATE, "update: algorithm has no BSP update");
   }
   GRD_HIP(hipGetLastError());
+  if (g->C > 1) GRD_TRY(op_frontier_parts(s));
   return GRD_OK;
+}
+
+void take_frontier_parts(grd_state *s)
+{
+  if (s->g->C == 1) {
+    s->last_fpart[0] = s->last_frontier;
+    s->last_uncovered = 0;
+    return;
+  }
+  for (int c = 0; c < s->g->C; c++) s->last_fpart[c] = s->hcnt->fpart[c];
+  s->last_uncovered = s->hcnt->frontier_uncovered;
 }
 
 // Delta-stepping bucket advance when the frontier came out empty.  Reads the
@@ -848,6 +868,7 @@ grd_status op_select_bucket(grd_state *s, bool *any_pending)
   hipLaunchKernelGGL(k_select, dim3(grid), dim3(kBlock), 0, st, nwS, s->Pend, (const uint32_t *)s->prop, g->slot_deg,
                      (unsigned long long)s->hi, s->F, s->cnt);
   GRD_HIP(hipGetLastError());
+  if (g->C > 1) GRD_TRY(op_frontier_parts(s));
   GRD_TRY(op_read_counters(s));
   *any_pending = s->hcnt->frontier != 0;
   return GRD_OK;

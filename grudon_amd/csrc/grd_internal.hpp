@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <array>
 #include <functional>
 #include <string>
 #include <vector>
@@ -81,7 +82,10 @@ struct Counters {
   double residual;                    // PR pull: L1 change
   double dangling[2];                 // PR pull: FP64 dangling mass (ping-pong)
   unsigned long long heavy_edges;     // push: edges of heavy vertices (k_push_heavy)
-  unsigned long long pad[3];
+  unsigned long long frontier_uncovered;  // frontier vertices with an out-neighbour of another master
+  unsigned long long ship_error;      // edge shipping: the active list disagreed with the sized wire
+  unsigned long long pad;
+  unsigned long long fpart[64];       // frontier size of each compute partition (C > 1)
 };
 
 // pull row block: rows [r0, r0 + nrows) with arcs [e0, e1); nrows == 0 marks
@@ -96,6 +100,8 @@ constexpr uint32_t kPullNnz = 2048;     // arcs per block (8 per thread)
 constexpr uint32_t kPullMaxRows = 1024;  // rows per block
 constexpr uint32_t kPullHot = 2048;      // PR pull: hottest sources of a partition kept in LDS
 constexpr uint32_t kHeavyDeg = 4096;     // push: out-degree handled by k_push_heavy, in chunks of this many arcs
+constexpr uint32_t kShipTile = 1024;     // edge shipping: wire positions per workgroup tile
+constexpr int kMaxParts = 64;            // largest C and M
 
 }  // namespace grd
 
@@ -141,6 +147,11 @@ struct grd_graph {
   uint32_t *rp = nullptr, *col = nullptr, *w = nullptr;     // push CSR over proxies
   uint32_t *crp = nullptr, *ccol = nullptr, *cw = nullptr;  // pull CSC over proxies
   uint32_t *xptr = nullptr, *xlist = nullptr;               // [S+1], [X] cross proxies per slot
+  // edge shipping (NO_OFFLOAD schedule)
+  uint32_t *gid_slot = nullptr;  // [n] gid -> compute slot (the compute side's getLocalNode)
+  uint8_t *word_c = nullptr;     // [S/64] compute partition of each slot word
+  uint64_t *uncov = nullptr;     // [S/64] slots with an out-neighbour of another master (not covered)
+  uint64_t n_uncovered = 0;      // vertices not covered
   // pull row blocks (pull.hip): at most kPullNnz arcs per block, longer rows
   // split into segments merged by a fix-up pass
   grd::PullBlock *pblk = nullptr;
@@ -187,6 +198,17 @@ struct grd_state {
   uint2 *heavy = nullptr;  // push: (proxy, first arc offset) of each heavy-vertex chunk
   void *partial = nullptr;  // pull: one 8-byte partial per row block (long-row segments)
   double *red = nullptr;    // PR pull update: (residual, dangling) per workgroup
+  // edge shipping wire (grown on demand): record lengths / offsets, the first
+  // record of every tile, the records (gids) and their edge weights
+  grd::DevMem ship_mem;
+  uint32_t *ship_off = nullptr, *ship_tile = nullptr, *ship_buf = nullptr, *ship_w = nullptr;
+  uint64_t ship_kcap = 0, ship_cap = 0;
+  void *ship_tmp = nullptr;  // rocPRIM scan storage (in ship_mem)
+  size_t ship_tmp_bytes = 0;
+  // NDPEngine inputs for the next decision: frontier size per compute
+  // partition and frontier vertices not covered
+  uint64_t last_fpart[grd::kMaxParts] = {};
+  uint64_t last_uncovered = 0;
   grd::Counters *cnt = nullptr;   // device
   grd::Counters *hcnt = nullptr;  // pinned host
   // snapshot copies
@@ -195,6 +217,8 @@ struct grd_state {
   std::vector<uint64_t> snap_bytes;
   uint32_t snap_iteration = 0;
   uint64_t snap_hi = 0;
+  std::array<uint64_t, 6> snap_host{};  // frontier statistics at the snapshot
+  uint64_t snap_fpart[grd::kMaxParts] = {};
 };
 
 namespace grd {
@@ -228,6 +252,13 @@ grd_status op_download(grd_state *s, void *out);
 grd_status op_features(grd_state *s, float *out16);
 grd_status op_zero_counters(grd_state *s);
 grd_status op_read_counters(grd_state *s);
+// copies the per-partition frontier sizes and the uncovered count of the
+// frontier just computed from the counters read back into the state
+void take_frontier_parts(grd_state *s);
+// ship.hip
+grd_status build_ship_tables(grd_graph *g);
+grd_status op_frontier_parts(grd_state *s);  // per-partition frontier counts into the counters (C > 1)
+grd_status op_ship(grd_state *s);            // edge shipping: memory send + compute apply
 // prpull.hip
 grd_status prp_init(grd_state *s);
 uint32_t prp_grid(const grd_graph *g);  // workgroups of the PR pull update

@@ -411,12 +411,29 @@ grd_status check_row_ptr(uint32_t n, const uint32_t *rp)
 grd_status build_graph(grd_ctx *ctx, uint32_t n, const uint32_t *rp, const uint32_t *col, const uint32_t *w, int C,
                        int M, const uint32_t *mirror, grd_graph **out)
 {
-  if (ctx->layout_on_device) return build_graph_device(ctx, n, rp, col, w, C, M, mirror, out);
-  try {
-    return build_impl(ctx, n, rp, col, w, C, M, mirror, out);
-  } catch (const std::bad_alloc &) {
-    return fail(GRD_ERR_OOM, "graph: host allocation failed");
+  grd_status st = GRD_OK;
+  if (ctx->layout_on_device) {
+    st = build_graph_device(ctx, n, rp, col, w, C, M, mirror, out);
+  } else {
+    try {
+      st = build_impl(ctx, n, rp, col, w, C, M, mirror, out);
+    } catch (const std::bad_alloc &) {
+      st = fail(GRD_ERR_OOM, "graph: host allocation failed");
+    }
   }
+  if (st != GRD_OK) return st;
+  // both constructions end with the tables of the edge-shipping schedule
+  try {
+    st = build_ship_tables(*out);
+  } catch (const std::bad_alloc &) {
+    st = fail(GRD_ERR_OOM, "graph: host allocation failed");
+  }
+  if (st != GRD_OK) {
+    (void)hipStreamSynchronize(ctx->stream);
+    delete *out;
+    *out = nullptr;
+  }
+  return st;
 }
 
 }  // namespace grd

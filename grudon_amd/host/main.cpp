@@ -34,7 +34,9 @@ const char *kHelp =
     "  -p, --partitioning-scheme arg  Partitioning scheme file (one memory partition per line)\n"
     "  -a, --algorithm arg            pr | cc | sssp | prpull (default: pr)\n"
     "  -h, --help                     Print usage\n"
-    "      --sched arg                plain | heuristic | learned | fixed:N (default: plain)\n"
+    "      --sched arg                plain | heuristic | learned | ndp | fixed:N (default: plain)\n"
+    "                                 ndp = the NDPEngine offload vote (edge shipping when\n"
+    "                                 covered and small); fixed:4 = edge shipping\n"
     "      --selector arg             BF16 selector weights file for --sched learned\n"
     "      --source arg               SSSP source gid (default: 0)\n"
     "      --weighted                 SSSP: use the edge weights\n"
@@ -145,6 +147,7 @@ int main(int argc, char **argv)
   int fixed = 0;
   if (o.sched == "heuristic") sched = GRD_SCHED_HEURISTIC;
   else if (o.sched == "learned") sched = GRD_SCHED_LEARNED;
+  else if (o.sched == "ndp") sched = GRD_SCHED_NDP_ENGINE;
   else if (o.sched.rfind("fixed:", 0) == 0) {
     sched = GRD_SCHED_FIXED;
     fixed = std::atoi(o.sched.c_str() + 6);

@@ -43,16 +43,36 @@ typedef struct grd_state grd_state;
  * (FP32 Kahan contributions, FP64 dangling mass and residual). */
 typedef enum { GRD_PR = 0, GRD_CC = 1, GRD_SSSP = 2, GRD_PR_PULL = 3 } grd_algo;
 
-/* Per-iteration schedule option: bit 0 = pull traversal (else push),
- * bit 1 = dense sync (else sparse bitvector + packed values). */
-enum { GRD_OPT_PUSH_SPARSE = 0, GRD_OPT_PULL_SPARSE = 1, GRD_OPT_PUSH_DENSE = 2, GRD_OPT_PULL_DENSE = 3 };
+/* Per-iteration schedule option.  0-3 are the offloaded superstep
+ * (NDP_OFFLOAD, src/GraphAlgorithm.cpp:199-581): bit 0 = pull traversal
+ * (else push), bit 1 = dense sync (else sparse bitvector + packed values).
+ * GRD_OPT_SHIP_EDGES is the NO_OFFLOAD superstep (GraphAlgorithm.cpp:426-474,
+ * 582-637): after the sparse broadcast, the memory partitions ship the
+ * frontier's edge lists (gids) to the masters, which apply the updates
+ * themselves; there is no reduce.  It needs every frontier vertex covered
+ * (all out-neighbours on the same master, see grd_graph_coverage). */
+enum {
+  GRD_OPT_PUSH_SPARSE = 0,
+  GRD_OPT_PULL_SPARSE = 1,
+  GRD_OPT_PUSH_DENSE = 2,
+  GRD_OPT_PULL_DENSE = 3,
+  GRD_OPT_SHIP_EDGES = 4
+};
 
 /* Schedule selection policy for grd_run. */
 typedef enum {
   GRD_SCHED_PLAIN = 0,     /* the reference schedule: push + sparse every iteration */
   GRD_SCHED_HEURISTIC = 1, /* direction-optimizing heuristic */
   GRD_SCHED_LEARNED = 2,   /* MLP selector with the heuristic as fallback */
-  GRD_SCHED_FIXED = 3      /* the option in grd_params.fixed_option every iteration */
+  GRD_SCHED_FIXED = 3,     /* the option in grd_params.fixed_option every iteration
+                              (GRD_OPT_SHIP_EDGES falls back to the offloaded
+                              push while the frontier is not covered) */
+  GRD_SCHED_NDP_ENGINE = 4 /* the reference's offload vote (NDPEngine,
+                              include/offload_engine/NDPEngine.hpp:10-51, which
+                              run() overrides at GraphAlgorithm.cpp:172-174):
+                              edge shipping when every compute partition's
+                              frontier is non-empty, at most n/20 and covered,
+                              else the offloaded push */
 } grd_sched;
 
 typedef struct {
@@ -88,8 +108,9 @@ typedef struct {
   uint64_t hit_max_iter;
   double ms;                 /* device time of the whole iteration loop */
   double residual;           /* PR_PULL: last L1 change */
-  uint64_t option_counts[4];
-  uint64_t fallbacks;        /* learned mode: iterations decided by the heuristic */
+  uint64_t option_counts[5]; /* supersteps run with each GRD_OPT_* */
+  uint64_t fallbacks;        /* learned: iterations decided by the heuristic;
+                                fixed edge shipping: iterations run offloaded */
   /* time_kernel: the dominant traversal kernel (pull: k_pull / k_prp_pull,
    * push: k_push), summed over launches, with its algorithmic HBM bytes:
    * pull 8 B per edge + 8 B per row, push 12 B per edge + 16 B per vertex */
@@ -161,6 +182,12 @@ grd_status grd_graph_info(const grd_graph *g, uint64_t *info16);
 grd_status grd_graph_table(const grd_graph *g, int c, int m, uint32_t *out_gids, uint64_t *count);
 /* Out-degree per gid (host copy). */
 grd_status grd_graph_out_degree(const grd_graph *g, uint32_t *out);
+/* Coverage vector (reference coverage_vector, src/DistributedGraph.cpp:286-370,
+ * read by NDPEngine.hpp:23-26): covered[gid] = 1 when every out-neighbour of
+ * gid has gid's master (gid % C), so that edge shipping can apply all of its
+ * updates on that compute partition.  *n_uncovered = vertices not covered.
+ * Either output may be NULL. */
+grd_status grd_graph_coverage(const grd_graph *g, uint8_t *covered, uint64_t *n_uncovered);
 /* No counterpart in the reference: 16 FNV-1a digests of the partitioned
  * layout (device arrays, host tables, sizes), so that two builds of the same
  * graph can be compared (e.g. the device and the host construction). */

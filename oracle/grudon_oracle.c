@@ -194,6 +194,7 @@ struct orc_part {
   uint64_t m;
   const uint32_t *grp, *gcol, *gw;
   uint32_t *mirror; /* per gid */
+  uint8_t *covered; /* per gid: every out-neighbour has the same master */
   /* compute partitions: lid = rank of gid among gid % C == c (= gid / C) */
   uint32_t *nc;       /* [C] */
   uint32_t **c_gid;   /* [C][nc] */
@@ -377,6 +378,17 @@ orc_part *orc_part_build(uint32_t n, const uint32_t *rp, const uint32_t *col, co
     free(idx);
   }
   free(cross);
+  /* coverage vector (DistributedGraph.cpp:286-370, read by NDPEngine.hpp:23-26):
+   * the evident intent, a vertex is covered when none of its out-arcs leaves
+   * its master; the reference's guard at :338 only clears flags that are
+   * already clear, so its own vector stays all true */
+  p->covered = (uint8_t *)xcalloc(n, 1);
+  for (uint32_t g = 0; g < n; g++) {
+    uint8_t cov = 1;
+    for (uint32_t e = rp[g]; e < rp[g + 1] && cov; e++)
+      if (col[e] % (uint32_t)C != g % (uint32_t)C) cov = 0;
+    p->covered[g] = cov;
+  }
   return p;
 }
 
@@ -407,6 +419,7 @@ void orc_part_free(orc_part *p)
   free(p->nm_own); free(p->nm); free(p->em);
   free(p->T); free(p->c_rT); free(p->c_sT); free(p->m_rT);
   free(p->mirror);
+  free(p->covered);
   free(p);
 }
 
@@ -440,6 +453,10 @@ int64_t orc_part_query(const orc_part *p, int what, int i, int j, uint32_t *out)
   case 7: return p->nc[i];
   case 8: return p->nm[i];
   case 9: return (int64_t)p->em[i];
+  case 10:
+    if (out)
+      for (uint32_t g = 0; g < p->n; g++) out[g] = p->covered[g];
+    return p->n;
   default: return -1;
   }
 }
@@ -544,13 +561,29 @@ int orc_bsp_u32(orc_part *p, int algo, uint32_t source, int weighted, uint32_t d
   if (algo != ORC_CC && algo != ORC_SSSP) return -1;
   if (algo == ORC_SSSP && source >= p->n) return -2;
   double *dummy = NULL;
-  return bsp_run_u32(p, algo, source, weighted, delta, max_iter, nthreads, out, dummy, stats, trace);
+  return bsp_run_u32(p, algo, source, weighted, delta, max_iter, nthreads, 0, out, dummy, stats, trace, NULL);
 }
 
 int orc_bsp_pr(orc_part *p, int fp64, int max_iter, int nthreads, double *out, uint64_t *stats, uint32_t *trace)
 {
-  if (fp64) return bsp_run_f64(p, ORC_PR, 0, 0, 0, max_iter, nthreads, NULL, out, stats, trace);
-  return bsp_run_f32(p, ORC_PR, 0, 0, 0, max_iter, nthreads, NULL, out, stats, trace);
+  return orc_bsp_pr_sched(p, fp64, max_iter, nthreads, 0, out, stats, trace, NULL);
+}
+
+int orc_bsp_u32_sched(orc_part *p, int algo, uint32_t source, int weighted, uint32_t delta, int max_iter, int nthreads,
+                      int sched, uint32_t *out, uint64_t *stats, uint32_t *trace, uint8_t *votes)
+{
+  if (algo != ORC_CC && algo != ORC_SSSP) return -1;
+  if (algo == ORC_SSSP && source >= p->n) return -2;
+  if (sched < 0 || sched > 2) return -3;
+  return bsp_run_u32(p, algo, source, weighted, delta, max_iter, nthreads, sched, out, NULL, stats, trace, votes);
+}
+
+int orc_bsp_pr_sched(orc_part *p, int fp64, int max_iter, int nthreads, int sched, double *out, uint64_t *stats,
+                     uint32_t *trace, uint8_t *votes)
+{
+  if (sched < 0 || sched > 2) return -3;
+  if (fp64) return bsp_run_f64(p, ORC_PR, 0, 0, 0, max_iter, nthreads, sched, NULL, out, stats, trace, votes);
+  return bsp_run_f32(p, ORC_PR, 0, 0, 0, max_iter, nthreads, sched, NULL, out, stats, trace, votes);
 }
 
 /* ------------------------------------------------------------------ */

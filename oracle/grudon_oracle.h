@@ -83,6 +83,7 @@ void orc_part_free(orc_part *p);
  *   what 7: compute partition c size               (i=c)
  *   what 8: memory partition m size (own + cross)  (i=m)
  *   what 9: memory partition m edge count          (i=m)
+ *   what 10: coverage by gid, 1 = covered           (out[n])
  * Returns the count (or the scalar) and fills out when non-NULL. */
 int64_t orc_part_query(const orc_part *p, int what, int i, int j, uint32_t *out);
 
@@ -107,6 +108,25 @@ int orc_bsp_u32(orc_part *p, int algo, uint32_t source, int weighted,
  * (fp64 == 0) or FP64.  out[gid] = pr value. */
 int orc_bsp_pr(orc_part *p, int fp64, int max_iter, int nthreads,
                double *out, uint64_t *stats, uint32_t *trace);
+
+/* The same runs with a superstep schedule (src/GraphAlgorithm.cpp:169-195):
+ *   sched 0: the offloaded superstep every iteration (NDP_OFFLOAD; what the
+ *            reference runs, its vote being forced at :172-174);
+ *   sched 1: the NDPEngine vote (include/offload_engine/NDPEngine.hpp:10-51)
+ *            on every compute partition, combined by MIN: NO_OFFLOAD (edge
+ *            shipping, :426-474 and :582-637) when every partition's frontier
+ *            is non-empty, at most n/20 and covered;
+ *   sched 2: edge shipping whenever the whole frontier is covered.
+ * A shipping iteration sends [gid, out-neighbour gids] (4 bytes each, plus
+ * 4 bytes of weight per arc for weighted SSSP) from the memory partitions to
+ * the masters and has no reduce; the master aggregates with the algorithm's
+ * own operator.  votes (may be NULL, max_iter entries) receives 1 for every
+ * iteration that shipped. */
+int orc_bsp_u32_sched(orc_part *p, int algo, uint32_t source, int weighted,
+                      uint32_t delta, int max_iter, int nthreads, int sched,
+                      uint32_t *out, uint64_t *stats, uint32_t *trace, uint8_t *votes);
+int orc_bsp_pr_sched(orc_part *p, int fp64, int max_iter, int nthreads, int sched,
+                     double *out, uint64_t *stats, uint32_t *trace, uint8_t *votes);
 
 /* ---- closed forms ---- */
 

@@ -50,6 +50,12 @@ def lib():
                                   _P32, _P64, ct.c_void_p]
         L.orc_bsp_pr.restype = ct.c_int
         L.orc_bsp_pr.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, _PF64, _P64, ct.c_void_p]
+        L.orc_bsp_u32_sched.restype = ct.c_int
+        L.orc_bsp_u32_sched.argtypes = [ct.c_void_p, ct.c_int, ct.c_uint32, ct.c_int, ct.c_uint32, ct.c_int,
+                                        ct.c_int, ct.c_int, _P32, _P64, ct.c_void_p, ct.c_void_p]
+        L.orc_bsp_pr_sched.restype = ct.c_int
+        L.orc_bsp_pr_sched.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _PF64, _P64,
+                                       ct.c_void_p, ct.c_void_p]
         L.orc_bfs.restype = None
         L.orc_bfs.argtypes = [ct.c_uint32, _P32, _P32, ct.c_uint32, _P32]
         L.orc_dijkstra.restype = None
@@ -104,6 +110,25 @@ def er(n, mean_degree, seed):
     return Graph(rp, col[:m].copy())
 
 
+def parity_doubled(g):
+    """Two copies of g on the even and the odd gids (u -> v becomes 2u -> 2v
+    and 2u+1 -> 2v+1): with C = 2 every arc stays on its master, so every
+    vertex is covered."""
+    deg = g.out_degree()
+    src = np.repeat(np.arange(g.n, dtype=np.int64), deg)
+    rank = np.arange(g.m, dtype=np.int64) - g.rp.astype(np.int64)[src]
+    rp2 = np.zeros(2 * g.n + 1, dtype=np.int64)
+    rp2[1:] = np.cumsum(np.repeat(deg, 2))
+    col2 = np.zeros(2 * g.m, dtype=np.uint32)
+    w2 = None if g.w is None else np.zeros(2 * g.m, dtype=np.uint32)
+    for par in (0, 1):
+        idx = rp2[2 * src + par] + rank
+        col2[idx] = 2 * g.col.astype(np.int64) + par
+        if w2 is not None:
+            w2[idx] = g.w
+    return Graph(rp2.astype(np.uint32), col2, w2)
+
+
 def from_edges(n, edges):
     """CSR from an (src, dst) edge list: sorted, self-loops and duplicates dropped."""
     keys = np.array([(int(s) << 32) | int(d) for s, d in edges] or [0], dtype=np.uint64)
@@ -140,32 +165,42 @@ class Partition:
         lib().orc_part_query(self.h, what, i, j, out.ctypes.data)
         return out[:cnt]
 
-    def cc(self, max_iter=1000, nthreads=1, trace=False):
+    def coverage(self):
+        """covered[gid] (1 = every out-neighbour shares the master)."""
+        return self.query(10).astype(np.uint8)
+
+    def _u32(self, algo, source, weighted, delta, max_iter, nthreads, trace, sched):
         out = np.zeros(self.g.n, dtype=np.uint32)
         st = np.zeros(4, dtype=np.uint64)
         tr = np.zeros(max_iter + 1, dtype=np.uint32) if trace else None
-        rc = lib().orc_bsp_u32(self.h, ORC_CC, 0, 0, 0, max_iter, nthreads, out, st,
-                               None if tr is None else tr.ctypes.data)
+        votes = np.zeros(max_iter + 1, dtype=np.uint8)
+        rc = lib().orc_bsp_u32_sched(self.h, algo, source, 1 if weighted else 0, delta, max_iter, nthreads, sched,
+                                     out, st, None if tr is None else tr.ctypes.data, votes.ctypes.data)
         assert rc == 0
-        return (out, st, tr[: int(st[0])]) if trace else (out, st)
+        it = int(st[0])
+        res = (out, st, tr[:it]) if trace else (out, st)
+        return res + (votes[:it],) if sched else res
 
-    def sssp(self, source=0, weighted=False, delta=0, max_iter=100000, nthreads=1, trace=False):
-        out = np.zeros(self.g.n, dtype=np.uint32)
-        st = np.zeros(4, dtype=np.uint64)
-        tr = np.zeros(max_iter + 1, dtype=np.uint32) if trace else None
-        rc = lib().orc_bsp_u32(self.h, ORC_SSSP, source, 1 if weighted else 0, delta, max_iter, nthreads, out, st,
-                               None if tr is None else tr.ctypes.data)
-        assert rc == 0
-        return (out, st, tr[: int(st[0])]) if trace else (out, st)
+    def cc(self, max_iter=1000, nthreads=1, trace=False, sched=0):
+        """sched 0: offloaded superstep every iteration; 1: NDPEngine vote;
+        2: edge shipping whenever the frontier is covered.  With sched, the
+        per-iteration shipping votes are appended to the result."""
+        return self._u32(ORC_CC, 0, False, 0, max_iter, nthreads, trace, sched)
 
-    def pr(self, fp64=False, max_iter=1000, nthreads=1, trace=False):
+    def sssp(self, source=0, weighted=False, delta=0, max_iter=100000, nthreads=1, trace=False, sched=0):
+        return self._u32(ORC_SSSP, source, weighted, delta, max_iter, nthreads, trace, sched)
+
+    def pr(self, fp64=False, max_iter=1000, nthreads=1, trace=False, sched=0):
         out = np.zeros(self.g.n, dtype=np.float64)
         st = np.zeros(4, dtype=np.uint64)
         tr = np.zeros(max_iter + 1, dtype=np.uint32) if trace else None
-        rc = lib().orc_bsp_pr(self.h, 1 if fp64 else 0, max_iter, nthreads, out, st,
-                              None if tr is None else tr.ctypes.data)
+        votes = np.zeros(max_iter + 1, dtype=np.uint8)
+        rc = lib().orc_bsp_pr_sched(self.h, 1 if fp64 else 0, max_iter, nthreads, sched, out, st,
+                                    None if tr is None else tr.ctypes.data, votes.ctypes.data)
         assert rc == 0
-        return (out, st, tr[: int(st[0])]) if trace else (out, st)
+        it = int(st[0])
+        res = (out, st, tr[:it]) if trace else (out, st)
+        return res + (votes[:it],) if sched else res
 
 
 def bfs(g, src):
