@@ -345,6 +345,8 @@ static grd_status ship_t(grd_state *s)
                      s->ship_tile, s->cnt);
   const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(ntiles, (uint32_t)ctx->num_cus * 8));
   const uint32_t *wt = weighted ? g->w : nullptr;
+  const bool timed = s->prm.time_kernel != 0;
+  if (timed) GRD_HIP(hipEventRecord(s->kev[0], st));
   hipLaunchKernelGGL(k_ship_send, dim3(grid), dim3(kBlock), 0, st, s->AL, off, s->ship_tile, (uint32_t)k,
                      (uint32_t)total, ntiles, g->rp, g->col, wt, g->proxy_slot, g->slot_gid, s->ship_buf,
                      weighted ? s->ship_w : nullptr);
@@ -352,6 +354,15 @@ static grd_status ship_t(grd_state *s)
                      (uint32_t)total, ntiles, s->ship_buf, weighted ? s->ship_w : nullptr, g->gid_slot,
                      (const V *)s->prop, (V *)s->upd, s->Dc);
   GRD_HIP(hipGetLastError());
+  if (timed) {
+    // per arc: col, the two gid lookups, wire write + read, gid_slot, the
+    // atomic read-modify-write (+ weight read, write, read); per record:
+    // active list, row bounds, offsets (twice), gid lookups, header write +
+    // read, gid_slot, property
+    GRD_HIP(hipEventRecord(s->kev[1], st));
+    s->kern_pending = 1;
+    s->kern_pending_bytes = (weighted ? 44 : 32) * E + 44 * k;
+  }
   s->edges += E;
   // wire bytes (MPI.hpp:44 counts count * sizeof(GNode) per send): one gid
   // per record header and per arc, plus the weights of a weighted SSSP

@@ -86,6 +86,9 @@ typedef struct {
   uint32_t fixed_option; /* option for GRD_SCHED_FIXED */
   uint32_t time_iters;   /* record per-iteration device time in the trace */
   uint32_t time_kernel;  /* bracket the dominant traversal kernel with HIP events */
+  uint32_t offload_only; /* heuristic / learned: never turn a push superstep into
+                            edge shipping (by default they ship whenever the
+                            frontier is covered, the cheaper push on one GPU) */
 } grd_params;
 
 /* One record per iteration (grd_run trace).  feat holds the selector's 16
@@ -112,8 +115,10 @@ typedef struct {
   uint64_t fallbacks;        /* learned: iterations decided by the heuristic;
                                 fixed edge shipping: iterations run offloaded */
   /* time_kernel: the dominant traversal kernel (pull: k_pull / k_prp_pull,
-   * push: k_push), summed over launches, with its algorithmic HBM bytes:
-   * pull 8 B per edge + 8 B per row, push 12 B per edge + 16 B per vertex */
+   * push: k_push, edge shipping: k_ship_send + k_ship_recv), summed over
+   * launches, with its algorithmic HBM bytes: pull 8 B per edge + 8 B per
+   * row, push 12 B per edge + 16 B per vertex, shipping 32 B per edge
+   * (44 weighted) + 44 B per record */
   double kernel_ms;
   uint64_t kernel_launches;
   uint64_t kernel_bytes;
