@@ -204,11 +204,12 @@ def layout_desc(args, scale, sym):
 def timed_solves(S, args, init_kw, sources=None):
     """Warm-up, then exactly args.steps timed steps; a step is one solve (or
     one solve per source).  Returns (seconds, iterations, kernel stats, last
-    run stats)."""
+    run stats); the last run's stats also carry "timed_edges_examined", the
+    edges the traversals examined over all timed steps (for work GTEPS)."""
     srcs = sources if sources is not None else [None]
 
     def step(time_kernel):
-        acc = [0, 0.0, 0, 0]
+        acc = [0, 0.0, 0, 0, 0]
         st = None
         for s in srcs:
             kw = dict(init_kw)
@@ -220,11 +221,12 @@ def timed_solves(S, args, init_kw, sources=None):
             acc[1] += st["kernel_ms"]
             acc[2] += st["kernel_launches"]
             acc[3] += st["kernel_bytes"]
+            acc[4] += st["edges_examined"]
         return acc, st
 
     for _ in range(args.warmup):
         step(False)
-    iters, kms, kl, kb = 0, 0.0, 0, 0
+    iters, kms, kl, kb, ee = 0, 0.0, 0, 0, 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         acc, last = step(True)
@@ -232,6 +234,8 @@ def timed_solves(S, args, init_kw, sources=None):
         kms += acc[1]
         kl += acc[2]
         kb += acc[3]
+        ee += acc[4]
+    last["timed_edges_examined"] = ee
     return time.perf_counter() - t0, iters, (kms, kl, kb), last
 
 
@@ -324,13 +328,15 @@ def run_pagerank(api, ctx, args):
     algo = api.PR_PULL if pull else api.PR
     S = api.State(G, algo)
     secs, iters, (kms, kl, kb), last = timed_solves(
-        S, args, {"max_iter": 1000, "tolerance": 1e-6, "sched": api.SCHED_HEURISTIC})
+        S, args, {"max_iter": 1000, "tolerance": 1e-6, "sched": api.SCHED_HEURISTIC,
+                  "offload_only": args.offload_only})
     gteps = G.m * iters / secs / 1e9
     log("timed %d steps: %.3f s, %d iterations, %.2f ms/step, %.1f GTEPS"
         % (args.steps, secs, iters, secs * 1e3 / args.steps, gteps))
     kname = "k_prp_pull" if pull else "k_bsp_traverse"
     roof = roofline_of(kms, kl, kb, kname, traffic_from_profiles("%s_rmat%d" % (kname, args.scale)))
-    extra = {}
+    # edges the traversals examined (a pull examines every arc) per second
+    extra = {"work_gteps": round(last["timed_edges_examined"] / secs / 1e9, 3)}
     if pull and not args.no_check:
         ref = fp64_pagerank(rp, col, last["iterations"])
         extra["fp64_l1"] = float(np.abs(S.result().astype(np.float64) - ref).sum())
@@ -373,10 +379,12 @@ def run_cc(api, ctx, args):
     G = api.Graph.from_csr(ctx, rp, col, C=args.compute, M=args.parts)
     log("graph: n=%d m=%d partition+upload %.1fs" % (G.n, G.m, time.perf_counter() - t1))
     S = api.State(G, api.CC)
-    secs, iters, (kms, kl, kb), last = timed_solves(S, args, {"sched": api.SCHED_HEURISTIC})
+    secs, iters, (kms, kl, kb), last = timed_solves(S, args, {"sched": api.SCHED_HEURISTIC,
+                                                              "offload_only": args.offload_only})
     gteps = G.m * args.steps / secs / 1e9
     log("timed %d steps: %.3f s, %.1f GTEPS" % (args.steps, secs, gteps))
-    extra = {}
+    # edges the traversals examined (a pull examines every arc) per second
+    extra = {"work_gteps": round(last["timed_edges_examined"] / secs / 1e9, 3)}
     if not args.no_check:
         extra["labels_bit_exact"] = bool(np.array_equal(S.result(), cc_labels_scipy(rp, col)))
         log("labels bit-exact vs scipy components: %s" % extra["labels_bit_exact"])
@@ -408,12 +416,13 @@ def run_sssp(api, ctx, args):
     G = api.Graph.from_csr(ctx, rp, col, w, C=args.compute, M=args.parts)
     log("graph: n=%d m=%d partition+upload %.1fs" % (G.n, G.m, time.perf_counter() - t1))
     S = api.State(G, api.SSSP)
-    kw = {"weighted": True, "delta": args.delta, "sched": api.SCHED_HEURISTIC}
+    kw = {"weighted": True, "delta": args.delta, "sched": api.SCHED_HEURISTIC, "offload_only": args.offload_only}
     secs, iters, (kms, kl, kb), last = timed_solves(S, args, kw, sources=sources)
     # harmonic mean over sources of arcs / t_s = 16 arcs / sum t_s
     gteps = G.m * len(sources) * args.steps / secs / 1e9
     log("timed %d steps x %d sources: %.3f s, %.1f GTEPS" % (args.steps, len(sources), secs, gteps))
-    extra = {}
+    # edges the traversals examined (a pull examines every arc) per second
+    extra = {"work_gteps": round(last["timed_edges_examined"] / secs / 1e9, 3)}
     if not args.no_check:
         ok = []
         for s in sources:
@@ -556,6 +565,8 @@ def main():
     ap.add_argument("--parts", type=int, default=4)
     ap.add_argument("--compute", type=int, default=1)
     ap.add_argument("--delta", type=int, default=256, help="SSSP delta-stepping bucket width")
+    ap.add_argument("--offload-only", action="store_true",
+                    help="cc/sssp/pr: the heuristic never runs a push superstep as edge shipping")
     ap.add_argument("--workload", default="pr_pull", choices=["pr_pull", "cc", "sssp", "pr", "batch", "learned"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 / exactness check")

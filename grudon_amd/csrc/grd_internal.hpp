@@ -148,7 +148,6 @@ struct grd_graph {
   uint32_t *crp = nullptr, *ccol = nullptr, *cw = nullptr;  // pull CSC over proxies
   uint32_t *xptr = nullptr, *xlist = nullptr;               // [S+1], [X] cross proxies per slot
   // edge shipping (NO_OFFLOAD schedule)
-  uint32_t *gid_slot = nullptr;  // [n] gid -> compute slot (the compute side's getLocalNode)
   uint8_t *word_c = nullptr;     // [S/64] compute partition of each slot word
   uint64_t *uncov = nullptr;     // [S/64] slots with an out-neighbour of another master (not covered)
   uint64_t n_uncovered = 0;      // vertices not covered

@@ -246,18 +246,27 @@ __global__ void k_push_csr(uint64_t m, uint32_t n, const uint32_t *__restrict__ 
   }
 }
 
-// Row pointers of rows [0, R) from sorted row keys: ptr[r] = first position
-// with key >= r (ptr[R] = count).  Each row is written by exactly one thread:
-// the one at the first key of the gap that ends at it.
-__global__ void k_rowptr_sorted(uint64_t count, const uint32_t *__restrict__ key, uint32_t R, uint32_t *__restrict__ ptr)
+// Row pointers from sorted row keys, ptr[r] = number of keys below r: the
+// first key of each run records its start, the last one the run's length,
+// and the lengths (zero for rows without keys) are scanned.  Every pass is a
+// flat grid over the keys or the rows, so long runs of empty rows (the
+// zero-degree tail of a degree-ordered region) cost nothing extra.
+__global__ void k_run_start(uint64_t count, const uint32_t *__restrict__ key, uint32_t *__restrict__ start)
 {
   GRID_LOOP(e, count)
   {
     const uint32_t k = key[e];
-    const uint64_t lo = e ? (uint64_t)key[e - 1] + 1 : 0;
-    for (uint64_t r = lo; r <= k; r++) ptr[r] = (uint32_t)e;
-    if (e + 1 == count)
-      for (uint64_t r = (uint64_t)k + 1; r <= R; r++) ptr[r] = (uint32_t)count;
+    if (e == 0 || key[e - 1] != k) start[k] = (uint32_t)e;
+  }
+}
+
+__global__ void k_run_len(uint64_t count, const uint32_t *__restrict__ key, const uint32_t *__restrict__ start,
+                          uint32_t *__restrict__ len)
+{
+  GRID_LOOP(e, count)
+  {
+    const uint32_t k = key[e];
+    if (e + 1 == count || key[e + 1] != k) len[k] = (uint32_t)(e + 1) - start[k];
   }
 }
 
@@ -294,6 +303,21 @@ grd_status exclusive_scan_u32(DevMem &mem, hipStream_t st, const T *in, T *out, 
   GRD_TRY(mem.alloc(&tmp, tb));
   GRD_HIP(rocprim::exclusive_scan(tmp, tb, in, out, T(0), (size_t)n, rocprim::plus<T>(), st));
   return GRD_OK;
+}
+
+// ptr[0..R] for `count` keys sorted ascending, each below R
+grd_status rowptr_sorted(DevMem &mem, hipStream_t st, uint64_t count, const uint32_t *key, uint32_t R, uint32_t *ptr)
+{
+  uint32_t *start = nullptr, *len = nullptr;
+  GRD_TRY(mem.alloc_n(&start, std::max<uint32_t>(R, 1)));
+  GRD_TRY(mem.alloc_n(&len, (uint64_t)R + 1));
+  hipLaunchKernelGGL(k_fill32, dim3(grid_for((uint64_t)R + 1)), dim3(kBlock), 0, st, len, (uint64_t)R + 1, 0u);
+  if (count) {
+    hipLaunchKernelGGL(k_run_start, dim3(grid_for(count)), dim3(kBlock), 0, st, count, key, start);
+    hipLaunchKernelGGL(k_run_len, dim3(grid_for(count)), dim3(kBlock), 0, st, count, key, start, len);
+  }
+  GRD_HIP(hipGetLastError());
+  return exclusive_scan_u32(mem, st, len, ptr, (uint64_t)R + 1);
 }
 
 template <typename K, typename V>
@@ -528,8 +552,7 @@ grd_status build_dev_impl(grd_ctx *ctx, uint32_t n, const uint32_t *h_rp, const 
   if (m) GRD_HIP(hipMemcpyAsync(ckey, g->col, m * 4, hipMemcpyDeviceToDevice, st));
   if (m) GRD_TRY(sort_pairs(tmp, st, ckey, pval, m, bits_of(P)));
   GRD_TRY(gm.alloc_n(&g->crp, (uint64_t)P + 1));
-  if (m) hipLaunchKernelGGL(k_rowptr_sorted, dim3(grid_for(m)), dim3(kBlock), 0, st, m, ckey, P, g->crp);
-  else hipLaunchKernelGGL(k_fill32, dim3(grid_for(P + 1)), dim3(kBlock), 0, st, g->crp, (uint64_t)P + 1, 0u);
+  GRD_TRY(rowptr_sorted(tmp, st, m, ckey, P, g->crp));
   GRD_TRY(gm.alloc_n(&g->ccol, std::max<uint64_t>(m, 1)));
   if (h_w) GRD_TRY(gm.alloc_n(&g->cw, std::max<uint64_t>(m, 1)));
   hipLaunchKernelGGL(k_csc_unpack, dim3(grid_for(m)), dim3(kBlock), 0, st, m, pval, g->ccol, g->cw);
@@ -544,9 +567,9 @@ grd_status build_dev_impl(grd_ctx *ctx, uint32_t n, const uint32_t *h_rp, const 
                        d_xstart, g->proxy_slot, lkey, lval);
     GRD_TRY(sort_pairs(tmp, st, lkey, lval, X, bits_of(S)));
     GRD_HIP(hipMemcpyAsync(g->xlist, lval, X * 4, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_rowptr_sorted, dim3(grid_for(X)), dim3(kBlock), 0, st, X, lkey, S, g->xptr);
+    GRD_TRY(rowptr_sorted(tmp, st, X, lkey, S, g->xptr));
   } else {
-    hipLaunchKernelGGL(k_fill32, dim3(grid_for(S + 1)), dim3(kBlock), 0, st, g->xptr, (uint64_t)S + 1, 0u);
+    GRD_TRY(rowptr_sorted(tmp, st, 0, nullptr, S, g->xptr));
   }
   GRD_HIP(hipGetLastError());
   // host copies

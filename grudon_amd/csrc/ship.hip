@@ -10,12 +10,16 @@
 //
 // On one GPU the wire is one buffer of records in active-list order (the
 // active list is grouped by (compute, memory) pair, so each pair's records
-// are contiguous):
+// are contiguous).  A record names the source and its out-neighbours by
+// compute slot, the receiver's local id: the sender translates each proxy
+// with one lookup of the proxy -> slot table (the table the offloaded reduce
+// uses too), where gids would cost proxy -> gid on the way out and gid ->
+// slot on the way in.  An entry is 4 bytes either way.
 //   k_ship_len    record length (1 + out-degree) of every active proxy
 //   (rocPRIM exclusive scan: record offsets)
 //   k_ship_tiles  the record holding the first position of every tile
-//   k_ship_send   memory side: writes the records, gids through the tables
-//   k_ship_recv   compute side: gid -> slot, aggregate() of the source value
+//   k_ship_send   memory side: writes the records
+//   k_ship_recv   compute side: aggregate() of the source value
 // Both wire passes are merge-path tiled: a tile of kShipTile positions stages
 // the offsets of its records in LDS and each thread finds the record of its
 // position by binary search there, so the work per thread is even whatever
@@ -160,15 +164,14 @@ __device__ __forceinline__ uint32_t ship_owner(const uint32_t *s_off, uint32_t n
 }
 
 // Memory side (GraphAlgorithm.cpp:426-474): position 0 of a record is the
-// source gid, position r > 0 the gid of its r-th out-neighbour (with the arc
-// weight in wbuf for weighted SSSP; the header's weight slot is 0).
+// source's slot, position r > 0 the slot of its r-th out-neighbour (with the
+// arc weight in wbuf for weighted SSSP; the header's weight slot is 0).
 __global__ void __launch_bounds__(kBlock) k_ship_send(const uint32_t *__restrict__ AL, const uint32_t *__restrict__ off,
                                                       const uint32_t *__restrict__ tile, uint32_t k, uint32_t total,
                                                       uint32_t ntiles, const uint32_t *__restrict__ rp,
                                                       const uint32_t *__restrict__ col, const uint32_t *__restrict__ wt,
                                                       const uint32_t *__restrict__ proxy_slot,
-                                                      const uint32_t *__restrict__ slot_gid, uint32_t *__restrict__ buf,
-                                                      uint32_t *__restrict__ wbuf)
+                                                      uint32_t *__restrict__ buf, uint32_t *__restrict__ wbuf)
 {
   __shared__ uint32_t s_off[kShipTile + 2];
   __shared__ uint32_t s_beg[kShipTile + 1];
@@ -180,7 +183,7 @@ __global__ void __launch_bounds__(kBlock) k_ship_send(const uint32_t *__restrict
     for (uint32_t i = threadIdx.x; i < nr; i += kBlock) {
       const uint32_t p = AL[a + i];
       s_beg[i] = rp[p];
-      s_src[i] = slot_gid[proxy_slot[p]];
+      s_src[i] = proxy_slot[p];
     }
     __syncthreads();
     const uint32_t q0 = t * kShipTile, q1 = min(total, q0 + kShipTile);
@@ -193,7 +196,7 @@ __global__ void __launch_bounds__(kBlock) k_ship_send(const uint32_t *__restrict
           x = s_src[i];
         } else {
           const uint32_t e = s_beg[i] + r - 1;
-          x = slot_gid[proxy_slot[col[e]]];
+          x = proxy_slot[col[e]];
           if (wbuf) wx = wt[e];
         }
         buf[q] = x;
@@ -203,8 +206,9 @@ __global__ void __launch_bounds__(kBlock) k_ship_send(const uint32_t *__restrict
   }
 }
 
-// Compute side (GraphAlgorithm.cpp:582-637): getLocalNode of every gid, then
-// aggregate() of the source's property along each listed arc.  The reference
+// Compute side (GraphAlgorithm.cpp:582-637): aggregate() of the source's
+// property along each listed arc (the entries are already local ids, what
+// getLocalNode returns there).  The reference
 // applies addUpdate here whatever the algorithm (a TODO at :624); this applies
 // the algorithm's own aggregate, which is what makes the schedule give the
 // same result as the offloaded one.
@@ -213,7 +217,6 @@ __global__ void __launch_bounds__(kBlock) k_ship_recv(const uint32_t *__restrict
                                                       const uint32_t *__restrict__ tile, uint32_t k, uint32_t total,
                                                       uint32_t ntiles, const uint32_t *__restrict__ buf,
                                                       const uint32_t *__restrict__ wbuf,
-                                                      const uint32_t *__restrict__ gid_slot,
                                                       const typename Op::V *__restrict__ prop, typename Op::V *upd,
                                                       uint64_t *Dc)
 {
@@ -226,15 +229,14 @@ __global__ void __launch_bounds__(kBlock) k_ship_recv(const uint32_t *__restrict
     const uint32_t nr = ship_tile_records(off, tile, k, t, ntiles, s_off, &a);
     (void)a;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < nr; i += kBlock) s_val[i] = prop[gid_slot[buf[s_off[i]]]];
+    for (uint32_t i = threadIdx.x; i < nr; i += kBlock) s_val[i] = prop[buf[s_off[i]]];
     __syncthreads();
     const uint32_t q0 = t * kShipTile, q1 = min(total, q0 + kShipTile);
     if (nr)
       for (uint32_t q = q0 + threadIdx.x; q < q1; q += kBlock) {
         const uint32_t i = ship_owner(s_off, nr, q);
         if (q == s_off[i]) continue;  // the record's header
-        const uint32_t d = gid_slot[buf[q]];
-        relax<Op, true>(upd, Dc, d, Op::edge(s_val[i], wbuf ? wbuf[q] : 1u));
+        relax<Op, true>(upd, Dc, buf[q], Op::edge(s_val[i], wbuf ? wbuf[q] : 1u));
       }
     __syncthreads();
   }
@@ -253,8 +255,6 @@ grd_status build_ship_tables(grd_graph *g)
 {
   hipStream_t st = g->ctx->stream;
   const uint32_t nwS = g->S / 64;
-  GRD_TRY(g->mem.alloc_n(&g->gid_slot, std::max<uint32_t>(g->n, 1)));
-  GRD_HIP(hipMemcpyAsync(g->gid_slot, g->h_gid_slot.data(), (uint64_t)g->n * 4, hipMemcpyHostToDevice, st));
   // compute partition of every slot word: pair (c, m) owns its own count
   // rounded up to whole words from cbase[c * M + m]
   std::vector<uint8_t> wc(std::max<uint32_t>(nwS, 1), 0);
@@ -348,23 +348,23 @@ static grd_status ship_t(grd_state *s)
   const bool timed = s->prm.time_kernel != 0;
   if (timed) GRD_HIP(hipEventRecord(s->kev[0], st));
   hipLaunchKernelGGL(k_ship_send, dim3(grid), dim3(kBlock), 0, st, s->AL, off, s->ship_tile, (uint32_t)k,
-                     (uint32_t)total, ntiles, g->rp, g->col, wt, g->proxy_slot, g->slot_gid, s->ship_buf,
+                     (uint32_t)total, ntiles, g->rp, g->col, wt, g->proxy_slot, s->ship_buf,
                      weighted ? s->ship_w : nullptr);
   hipLaunchKernelGGL((k_ship_recv<Op>), dim3(grid), dim3(kBlock), 0, st, off, s->ship_tile, (uint32_t)k,
-                     (uint32_t)total, ntiles, s->ship_buf, weighted ? s->ship_w : nullptr, g->gid_slot,
-                     (const V *)s->prop, (V *)s->upd, s->Dc);
+                     (uint32_t)total, ntiles, s->ship_buf, weighted ? s->ship_w : nullptr, (const V *)s->prop,
+                     (V *)s->upd, s->Dc);
   GRD_HIP(hipGetLastError());
   if (timed) {
-    // per arc: col, the two gid lookups, wire write + read, gid_slot, the
-    // atomic read-modify-write (+ weight read, write, read); per record:
-    // active list, row bounds, offsets (twice), gid lookups, header write +
-    // read, gid_slot, property
+    // per arc: col, the slot lookup, wire write + read, the atomic
+    // read-modify-write (+ weight read, write, read); per record: active
+    // list, row bounds, offsets (twice), slot lookup, header write + read,
+    // property
     GRD_HIP(hipEventRecord(s->kev[1], st));
     s->kern_pending = 1;
-    s->kern_pending_bytes = (weighted ? 44 : 32) * E + 44 * k;
+    s->kern_pending_bytes = (weighted ? 36 : 24) * E + 36 * k;
   }
   s->edges += E;
-  // wire bytes (MPI.hpp:44 counts count * sizeof(GNode) per send): one gid
+  // wire bytes (MPI.hpp:44 counts count * sizeof(GNode) per send): one id
   // per record header and per arc, plus the weights of a weighted SSSP
   s->bytes += 4 * total + (weighted ? 4 * E : 0);
   return GRD_OK;

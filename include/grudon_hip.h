@@ -48,7 +48,7 @@ typedef enum { GRD_PR = 0, GRD_CC = 1, GRD_SSSP = 2, GRD_PR_PULL = 3 } grd_algo;
  * (else push), bit 1 = dense sync (else sparse bitvector + packed values).
  * GRD_OPT_SHIP_EDGES is the NO_OFFLOAD superstep (GraphAlgorithm.cpp:426-474,
  * 582-637): after the sparse broadcast, the memory partitions ship the
- * frontier's edge lists (gids) to the masters, which apply the updates
+ * frontier's edge lists to the masters, which apply the updates
  * themselves; there is no reduce.  It needs every frontier vertex covered
  * (all out-neighbours on the same master, see grd_graph_coverage). */
 enum {
@@ -117,8 +117,8 @@ typedef struct {
   /* time_kernel: the dominant traversal kernel (pull: k_pull / k_prp_pull,
    * push: k_push, edge shipping: k_ship_send + k_ship_recv), summed over
    * launches, with its algorithmic HBM bytes: pull 8 B per edge + 8 B per
-   * row, push 12 B per edge + 16 B per vertex, shipping 32 B per edge
-   * (44 weighted) + 44 B per record */
+   * row, push 12 B per edge + 16 B per vertex, shipping 24 B per edge
+   * (36 weighted) + 36 B per record */
   double kernel_ms;
   uint64_t kernel_launches;
   uint64_t kernel_bytes;
