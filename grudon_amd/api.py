@@ -25,7 +25,7 @@ EXPORTS = [
     "grd_graph_table", "grd_graph_out_degree", "grd_graph_coverage",
     "grd_state_create", "grd_state_free", "grd_init", "grd_broadcast", "grd_traverse", "grd_reduce",
     "grd_update", "grd_aggregate_one", "grd_run", "grd_select_option", "grd_state_stats", "grd_download",
-    "grd_features", "grd_snapshot",
+    "grd_features", "grd_snapshot", "grd_pair_exchange",
     "grd_restore", "grd_selector_load", "grd_selector_train", "grd_selector_infer", "grd_selector_grad",
     "grd_batch_create", "grd_batch_free", "grd_batch_tune", "grd_batch_autotune", "grd_batch_pagerank",
     "grd_batch_cc", "grd_batch_download", "grd_probe_gather",
@@ -98,6 +98,7 @@ def lib():
             "grd_select_option": [vp, ct.POINTER(ct.c_int)], "grd_state_stats": [vp, ct.POINTER(RunStats)],
             "grd_state_create": [vp, ct.c_int, pp], "grd_state_free": [vp], "grd_init": [vp, ct.POINTER(Params)],
             "grd_broadcast": [vp, ct.c_int], "grd_traverse": [vp, ct.c_int], "grd_reduce": [vp, ct.c_int],
+            "grd_pair_exchange": [vp, ct.c_int, ct.c_int, ct.c_int, vp, vp, ct.POINTER(ct.c_uint64)],
             "grd_update": [vp, ct.POINTER(ct.c_uint64)], "grd_aggregate_one": [vp, ct.c_uint32, vp],
             "grd_run": [vp, ct.POINTER(RunStats), vp, ct.c_uint32], "grd_download": [vp, vp],
             "grd_features": [vp, vp], "grd_snapshot": [vp], "grd_restore": [vp],
@@ -326,6 +327,13 @@ class State:
             return st.as_dict(), [tr[i] for i in range(n)]
         return st.as_dict()
 
+    def step_update(self):
+        """grd_update alone (after grd_broadcast / grd_traverse / grd_reduce);
+        returns the next frontier size."""
+        f = ct.c_uint64()
+        _check(lib().grd_update(self.h, ct.byref(f)), "grd_update")
+        return f.value
+
     def step(self, option):
         _check(lib().grd_broadcast(self.h, option), "grd_broadcast")
         _check(lib().grd_traverse(self.h, option), "grd_traverse")
@@ -339,6 +347,17 @@ class State:
         out = np.zeros(self.graph.n, dtype=dt)
         _check(lib().grd_download(self.h, _ptr(out)), "grd_download")
         return out
+
+    def pair_exchange(self, direction, c, m):
+        """(gids ascending, uint32 value bits) of pair (c, m)'s broadcast
+        (direction 0) or reduce (direction 1) in the current superstep."""
+        cnt = ct.c_uint64()
+        _check(lib().grd_pair_exchange(self.h, direction, c, m, None, None, ct.byref(cnt)), "grd_pair_exchange")
+        gids = np.zeros(max(cnt.value, 1), dtype=np.uint32)
+        vals = np.zeros(max(cnt.value, 1), dtype=np.uint32)
+        _check(lib().grd_pair_exchange(self.h, direction, c, m, _ptr(gids), _ptr(vals), ct.byref(cnt)),
+               "grd_pair_exchange")
+        return gids[: cnt.value], vals[: cnt.value]
 
     def features(self):
         f = np.zeros(16, dtype=np.float32)

@@ -811,6 +811,52 @@ grd_status grd_graph_write_gr(const char *path, uint32_t n, const uint32_t *row_
   return write_gr(path, n, row_ptr, col, weights);
 }
 
+grd_status grd_pair_exchange(grd_state *s, int dir, int c, int m, uint32_t *gids, void *values, uint64_t *count)
+{
+  if (!s || !count) return fail(GRD_ERR_ARG, "grd_pair_exchange: NULL argument");
+  if (!s->initialized) return fail(GRD_ERR_STATE, "grd_pair_exchange: state not initialized");
+  const grd_graph *g = s->g;
+  if (c < 0 || m < 0 || c >= g->C || m >= g->M || (dir != 0 && dir != 1))
+    return fail(GRD_ERR_ARG, "grd_pair_exchange: bad pair or direction");
+  if (s->algo == GRD_PR_PULL)
+    return fail(GRD_ERR_STATE, "grd_pair_exchange: the pull PageRank has no sparse exchange");
+  const int dense = dir == 0 ? s->xb_dense : s->xr_dense;
+  if (dense < 0) return fail(GRD_ERR_STATE, "grd_pair_exchange: no such exchange in the current superstep");
+  GRD_HIP(hipSetDevice(g->ctx->device));
+  hipStream_t st = g->ctx->stream;
+  const uint64_t nbits = dir == 0 ? g->S : g->P;
+  std::vector<uint64_t> bits(nbits / 64);
+  std::vector<uint32_t> vals(nbits);
+  GRD_HIP(hipMemcpyAsync(bits.data(), dir == 0 ? s->bvB : s->bvR, nbits / 8, hipMemcpyDeviceToHost, st));
+  GRD_HIP(hipMemcpyAsync(vals.data(), dir == 0 ? s->valB : s->valR, nbits * 4, hipMemcpyDeviceToHost, st));
+  GRD_HIP(hipStreamSynchronize(st));
+  // packing order of the sparse form: set bits of the whole wire, word by word
+  std::vector<uint64_t> rank(bits.size() + 1, 0);
+  for (size_t w = 0; w < bits.size(); w++) rank[w + 1] = rank[w] + (uint64_t)__builtin_popcountll(bits[w]);
+  std::vector<std::pair<uint32_t, uint32_t>> out;  // (gid, value)
+  auto take = [&](uint64_t i, uint64_t proxy) {    // bit i of the wire, the proxy it names
+    const uint64_t w = i >> 6, b = i & 63;
+    if (!((bits[w] >> b) & 1ull)) return;
+    const uint64_t r = rank[w] + (uint64_t)__builtin_popcountll(bits[w] & ((1ull << b) - 1));
+    out.emplace_back(g->h_proxy_gid[proxy], vals[dense ? i : r]);
+  };
+  const size_t cm = (size_t)c * g->M + m, mc = (size_t)m * g->C + c;
+  if (dir == 0) {
+    // own slots of (c, m); the own proxies of (m, c) repeat their order
+    for (uint32_t k = 0; k < g->ccnt[cm]; k++) take((uint64_t)g->cbase[cm] + k, (uint64_t)g->obase[mc] + k);
+  } else {
+    for (uint32_t k = 0; k < g->ocnt[mc]; k++) take((uint64_t)g->obase[mc] + k, (uint64_t)g->obase[mc] + k);
+    for (uint32_t k = 0; k < g->xcnt[mc]; k++) take((uint64_t)g->xbase[mc] + k, (uint64_t)g->xbase[mc] + k);
+  }
+  std::sort(out.begin(), out.end());
+  *count = out.size();
+  for (size_t i = 0; i < out.size(); i++) {
+    if (gids) gids[i] = out[i].first;
+    if (values) memcpy((uint8_t *)values + 4 * i, &out[i].second, 4);
+  }
+  return GRD_OK;
+}
+
 grd_status grd_download(grd_state *s, void *out)
 {
   if (!s || !out || !s->initialized) return fail(GRD_ERR_STATE, "grd_download: bad state");

@@ -692,6 +692,7 @@ grd_status state_init(grd_state *s)
   s->edges = 0;
   s->bcast_option = -1;
   s->sel_queued = false;
+  s->xb_dense = s->xr_dense = -1;
   if (g->C > 1) {  // per-partition frontier sizes for the first NDPEngine vote
     GRD_TRY(op_frontier_parts(s));
     GRD_TRY(op_read_counters(s));
@@ -719,6 +720,8 @@ grd_status op_broadcast(grd_state *s, int option)
   const uint32_t nwS = g->S / 64;
   const bool pull = option & 1, dense = option & 2;
   const bool need_rank = !dense || !pull;
+  s->xb_dense = dense ? 1 : 0;
+  s->xr_dense = -1;
   if (need_rank) GRD_TRY(word_ranks(st, s->F, nwS, s->blk, s->wrankB, s->totals + 0));
   const uint32_t grid = group_grid(nwS);
   hipLaunchKernelGGL(k_bcast_send<uint32_t>, dim3(grid), dim3(kBlock), 0, st, nwS, s->F, (const uint32_t *)s->prop,
@@ -789,6 +792,7 @@ static grd_status reduce_t(grd_state *s, int option)
   using V = typename Op::V;
   const uint32_t nwP = g->P / 64;
   const bool dense = option & 2;
+  s->xr_dense = dense ? 1 : 0;
   if (!dense) GRD_TRY(word_ranks(st, s->Du, nwP, s->blk, s->wrankR, s->totals + 1));
   const uint32_t grid = group_grid(nwP);
   hipLaunchKernelGGL(k_reduce_send<V>, dim3(grid), dim3(kBlock), 0, st, nwP, s->Du, (V *)s->mupd, s->wrankR, (int)dense,

@@ -169,6 +169,7 @@ struct grd_state {
   uint64_t hi = ~0ull;  // SSSP delta-stepping bucket bound
   int last_option = 0;
   int bcast_option = -1;  // option of the last broadcast (the traversal must use the same direction)
+  int xb_dense = -1, xr_dense = -1;  // last broadcast / reduce of this superstep: 0 sparse, 1 dense, -1 none
   bool sel_queued = false;  // learned: the next decision was computed on the device with the counters
   uint64_t last_frontier = 0, prev_frontier = 0, last_frontier_edges = 0, last_dirty = 0;
   uint32_t last_maxdeg = 0;

@@ -228,6 +228,16 @@ grd_status grd_select_option(grd_state *s, int *option);
 grd_status grd_state_stats(const grd_state *s, grd_run_stats *out);
 /* Result by gid: CC labels / SSSP distances (uint32), PR values (float). */
 grd_status grd_download(grd_state *s, void *out_by_gid);
+/* The current superstep's exchange of one pair, as the reference's public
+ * Worker::bitCommVector and the packed propertyBuffers hold it
+ * (include/Workers.hpp:30-45; filled at GraphAlgorithm.cpp:199-274 and
+ * :348-425): dir 0 = the broadcast c -> m (after grd_broadcast), dir 1 = the
+ * reduce m -> c (after grd_reduce).  gids[] receives the vertices whose bit
+ * is set, ascending (the device's bit order is the degree order), values[]
+ * the 4-byte property or update packed for each; *count their number.
+ * gids and values may be NULL.  GRD_ERR_STATE when that exchange has not
+ * run in this superstep (an edge-shipping superstep has no reduce). */
+grd_status grd_pair_exchange(grd_state *s, int dir, int c, int m, uint32_t *gids, void *values, uint64_t *count);
 /* Selector features of the current state (16 floats). */
 grd_status grd_features(grd_state *s, float *out16);
 /* Copy / restore every mutable array of the state (used for labelling). */

@@ -506,6 +506,13 @@ typedef struct {
   uint32_t T;
 } bv_t;
 
+/* one pair's exchange captured at one iteration (orc_bsp_pair_exchange) */
+typedef struct {
+  int iter, dir, c, m;
+  uint32_t *gids, *vals; /* may be NULL */
+  int64_t count;         /* -1 until captured */
+} orc_cap;
+
 static inline void bv_set(bv_t *b, uint32_t i) { b->bits[i >> 6] |= 1ull << (i & 63); }
 static inline uint64_t bv_words(uint32_t T) { return ((uint64_t)T + 63) / 64; }
 /* DynamicBitSet::getOffsets(): ascending set-bit indices */
@@ -561,7 +568,7 @@ int orc_bsp_u32(orc_part *p, int algo, uint32_t source, int weighted, uint32_t d
   if (algo != ORC_CC && algo != ORC_SSSP) return -1;
   if (algo == ORC_SSSP && source >= p->n) return -2;
   double *dummy = NULL;
-  return bsp_run_u32(p, algo, source, weighted, delta, max_iter, nthreads, 0, out, dummy, stats, trace, NULL);
+  return bsp_run_u32(p, algo, source, weighted, delta, max_iter, nthreads, 0, out, dummy, stats, trace, NULL, NULL);
 }
 
 int orc_bsp_pr(orc_part *p, int fp64, int max_iter, int nthreads, double *out, uint64_t *stats, uint32_t *trace)
@@ -575,15 +582,53 @@ int orc_bsp_u32_sched(orc_part *p, int algo, uint32_t source, int weighted, uint
   if (algo != ORC_CC && algo != ORC_SSSP) return -1;
   if (algo == ORC_SSSP && source >= p->n) return -2;
   if (sched < 0 || sched > 2) return -3;
-  return bsp_run_u32(p, algo, source, weighted, delta, max_iter, nthreads, sched, out, NULL, stats, trace, votes);
+  return bsp_run_u32(p, algo, source, weighted, delta, max_iter, nthreads, sched, out, NULL, stats, trace, votes,
+                     NULL);
+}
+
+static int cmp_pair_u64(const void *a, const void *b)
+{
+  uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+  return x < y ? -1 : x > y;
+}
+
+int64_t orc_bsp_pair_exchange(orc_part *p, int algo, uint32_t source, int weighted, uint32_t delta, int iter, int dir,
+                              int c, int m, uint32_t *gids, uint32_t *vals)
+{
+  if (algo != ORC_CC && algo != ORC_SSSP) return -1;
+  if (algo == ORC_SSSP && source >= p->n) return -2;
+  if (c < 0 || m < 0 || c >= p->C || m >= p->M || (dir != 0 && dir != 1) || iter < 0) return -3;
+  orc_cap cap = {iter, dir, c, m, NULL, NULL, -1};
+  uint64_t T = p->T[(size_t)c * p->M + m];
+  uint64_t *kv = (uint64_t *)xcalloc(T ? T : 1, sizeof(uint64_t));
+  cap.gids = (uint32_t *)xcalloc(T ? T : 1, sizeof(uint32_t));
+  cap.vals = (uint32_t *)xcalloc(T ? T : 1, sizeof(uint32_t));
+  uint32_t *out = (uint32_t *)xcalloc(p->n, sizeof(uint32_t));
+  uint64_t stats[4];
+  bsp_run_u32(p, algo, source, weighted, delta, iter + 1, 1, 0, out, NULL, stats, NULL, NULL, &cap);
+  int64_t k = cap.count < 0 ? 0 : cap.count;
+  if (k > 0) {
+    /* ascending gid, values alongside (gid << 32 | value) */
+    for (int64_t i = 0; i < k; i++) kv[i] = ((uint64_t)cap.gids[i] << 32) | cap.vals[i];
+    qsort(kv, (size_t)k, sizeof(uint64_t), cmp_pair_u64);
+    for (int64_t i = 0; i < k; i++) {
+      if (gids) gids[i] = (uint32_t)(kv[i] >> 32);
+      if (vals) vals[i] = (uint32_t)kv[i];
+    }
+  }
+  free(kv);
+  free(cap.gids);
+  free(cap.vals);
+  free(out);
+  return k;
 }
 
 int orc_bsp_pr_sched(orc_part *p, int fp64, int max_iter, int nthreads, int sched, double *out, uint64_t *stats,
                      uint32_t *trace, uint8_t *votes)
 {
   if (sched < 0 || sched > 2) return -3;
-  if (fp64) return bsp_run_f64(p, ORC_PR, 0, 0, 0, max_iter, nthreads, sched, NULL, out, stats, trace, votes);
-  return bsp_run_f32(p, ORC_PR, 0, 0, 0, max_iter, nthreads, sched, NULL, out, stats, trace, votes);
+  if (fp64) return bsp_run_f64(p, ORC_PR, 0, 0, 0, max_iter, nthreads, sched, NULL, out, stats, trace, votes, NULL);
+  return bsp_run_f32(p, ORC_PR, 0, 0, 0, max_iter, nthreads, sched, NULL, out, stats, trace, votes, NULL);
 }
 
 /* ------------------------------------------------------------------ */

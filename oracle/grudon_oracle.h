@@ -128,6 +128,16 @@ int orc_bsp_u32_sched(orc_part *p, int algo, uint32_t source, int weighted,
 int orc_bsp_pr_sched(orc_part *p, int fp64, int max_iter, int nthreads, int sched,
                      double *out, uint64_t *stats, uint32_t *trace, uint8_t *votes);
 
+/* One pair's exchange at iteration `iter` of the offloaded CC / SSSP run:
+ * dir 0 = the broadcast bitvector c -> m with its packed properties
+ * (GraphAlgorithm.cpp:199-274), dir 1 = the reduce bitvector m -> c with its
+ * packed updates (:348-425).  Fills gids (ascending) of the set bits and the
+ * value sent for each; returns their number, or < 0 for a bad argument
+ * (0 also when the run ended before `iter`). */
+int64_t orc_bsp_pair_exchange(orc_part *p, int algo, uint32_t source, int weighted,
+                              uint32_t delta, int iter, int dir, int c, int m,
+                              uint32_t *gids, uint32_t *vals);
+
 /* ---- closed forms ---- */
 
 void orc_bfs(uint32_t n, const uint32_t *row_ptr, const uint32_t *col,

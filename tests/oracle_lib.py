@@ -56,6 +56,9 @@ def lib():
         L.orc_bsp_pr_sched.restype = ct.c_int
         L.orc_bsp_pr_sched.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _PF64, _P64,
                                        ct.c_void_p, ct.c_void_p]
+        L.orc_bsp_pair_exchange.restype = ct.c_int64
+        L.orc_bsp_pair_exchange.argtypes = [ct.c_void_p, ct.c_int, ct.c_uint32, ct.c_int, ct.c_uint32, ct.c_int,
+                                            ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p]
         L.orc_bfs.restype = None
         L.orc_bfs.argtypes = [ct.c_uint32, _P32, _P32, ct.c_uint32, _P32]
         L.orc_dijkstra.restype = None
@@ -164,6 +167,18 @@ class Partition:
         out = np.zeros(max(cnt, 1), dtype=np.uint32)
         lib().orc_part_query(self.h, what, i, j, out.ctypes.data)
         return out[:cnt]
+
+    def pair_exchange(self, algo, it, direction, c, m, source=0, weighted=False, delta=0):
+        """(gids ascending, uint32 values) of pair (c, m)'s broadcast
+        (direction 0) or reduce (direction 1) at iteration `it` of the
+        offloaded CC / SSSP run."""
+        T = int(self.query(0, c, m, fetch=False))
+        gids = np.zeros(max(T, 1), dtype=np.uint32)
+        vals = np.zeros(max(T, 1), dtype=np.uint32)
+        k = lib().orc_bsp_pair_exchange(self.h, algo, source, 1 if weighted else 0, delta, it, direction, c, m,
+                                        gids.ctypes.data, vals.ctypes.data)
+        assert k >= 0
+        return gids[:k], vals[:k]
 
     def coverage(self):
         """covered[gid] (1 = every out-neighbour shares the master)."""
