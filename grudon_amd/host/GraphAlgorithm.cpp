@@ -92,5 +92,17 @@ std::vector<VP> GraphAlgorithm<VP>::result()
   return out;
 }
 
+template <typename VP>
+void GraphAlgorithm<VP>::exchangeView(int dir, uint32_t c, uint32_t m, std::vector<GNode> &gids,
+                                      std::vector<VP> &values)
+{
+  static_assert(sizeof(VP) == 4, "the exchange packs 4-byte properties");
+  uint64_t k = 0;
+  check(grd_pair_exchange(state, dir, (int)c, (int)m, nullptr, nullptr, &k), "exchange read-back");
+  gids.resize(k);
+  values.resize(k);
+  check(grd_pair_exchange(state, dir, (int)c, (int)m, gids.data(), values.data(), &k), "exchange read-back");
+}
+
 template class GraphAlgorithm<float>;
 template class GraphAlgorithm<uint32_t>;

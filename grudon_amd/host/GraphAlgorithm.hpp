@@ -3,10 +3,12 @@
 // Same constructor arguments and pure virtuals as the reference; the
 // MPICore& argument becomes the on-device Exchange&.  run() is the BSP
 // superstep loop of src/GraphAlgorithm.cpp:145-701: per iteration the
-// schedule decision (the offload vote of :187-195, here push/pull x
-// sparse/dense), the compute -> memory broadcast, gen_updates on the memory
-// partitions, the memory -> compute reduce with aggregate() applied in bulk
-// on the device, update_frontier + apply_updates, and the termination check.
+// schedule decision (the offload vote of :187-195: push/pull x sparse/dense
+// for the offloaded superstep, or NO_OFFLOAD edge shipping), the compute ->
+// memory broadcast, gen_updates on the memory partitions (or the shipping of
+// the frontier's edge lists), the memory -> compute reduce with aggregate()
+// applied in bulk on the device, update_frontier + apply_updates, and the
+// termination check.
 #ifndef GRUDON_GRAPHALGORITHM_HPP
 #define GRUDON_GRAPHALGORITHM_HPP
 
@@ -48,6 +50,10 @@ class GraphAlgorithm
   const grd_run_stats &stats() const { return last_stats; }
   std::vector<VertexProperty> result();  // by gid
   const DistributedGraph &graph() const { return *dgraph; }
+  // read-back of the current superstep's bitCommVector + propertyBuffers of
+  // pair (c, m): dir 0 = broadcast c -> m, dir 1 = reduce m -> c; the
+  // vertices whose bit is set, ascending gid, with the values packed for them
+  void exchangeView(int dir, uint32_t c, uint32_t m, std::vector<GNode> &gids, std::vector<VertexProperty> &values);
 
  protected:
   std::string algorithm_name;

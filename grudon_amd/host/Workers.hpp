@@ -10,6 +10,7 @@
 #define GRUDON_WORKERS_HPP
 
 #include <cstdint>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -39,6 +40,9 @@ class Worker
     num_edges = h.num_edges();
     out_degrees.resize(h.n);
     for (uint32_t v = 0; v < h.n; v++) out_degrees[v] = h.row_ptr[v + 1] - h.row_ptr[v];
+    coverage_vector.resize(h.n);
+    if (grd_graph_coverage(graph->handle(), coverage_vector.data(), nullptr) != GRD_OK)
+      throw std::runtime_error(std::string("coverage vector: ") + grd_last_error());
   }
   virtual ~Worker() = default;
 
@@ -51,6 +55,9 @@ class Worker
   uint64_t total_vertices = 0;
   uint64_t num_edges = 0;
   std::vector<uint64_t> out_degrees;  // by gid
+  // 1 when every out-neighbour shares the vertex's master (the vector the
+  // NDPEngine vote reads, DistributedGraph.cpp:286-370), by gid
+  std::vector<uint8_t> coverage_vector;
   Exchange &net;
   size_t num_compute;
   size_t num_memory;

@@ -29,3 +29,14 @@ def test_bf16_rounding_known_values():
     # BF16 keeps 7 fraction bits: spacing 2^-7 near 1.  1 + 2^-8 is a tie ->
     # even (1.0); 1 + 3*2^-8 is a tie -> even (1 + 2^-6); 1 + 2^-7 exact
     assert r.tolist() == [1.0, 1.0, 1.015625, 1.0078125, -2.5, 3.140625]
+
+
+def test_class_costs_charge_shipping_to_push():
+    from grudon_amd.train_selector import class_costs
+
+    c5 = np.array([[5.0, 3.0, 4.0, 2.5, 1.0], [1.0, 2.0, 1.5, 3.0, 9.0]], dtype=np.float32)
+    cc = class_costs(c5)
+    assert cc.tolist() == [[1.0, 3.0, 1.0, 2.5], [9.0, 2.0, 9.0, 3.0]]
+    assert cc.argmin(axis=1).tolist() == [0, 1]
+    c4 = c5[:, :4]
+    assert class_costs(c4) is c4
