@@ -107,3 +107,34 @@ def test_pr_shipping_close_to_offloaded():
     # FP64 sums in another order: equal to rounding
     assert np.abs(out - ref).sum() <= 1e-9 * np.abs(ref).sum()
     assert int(st2[0]) == int(st[0]) and votes.all()
+
+
+def _random_graph(rng, n, p):
+    edges = [(u, v) for u in range(n) for v in range(n) if u != v and rng.random() < p]
+    return O.from_edges(n, edges)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_small_graphs_all_schedules(seed):
+    """Brute force on small random graphs: every schedule gives the BFS
+    distances (hop SSSP) and the union-find components (CC on the
+    symmetrized graph), with the offloaded schedule's iteration count."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 40))
+    g = _random_graph(rng, n, float(rng.uniform(0.02, 0.3)))
+    es = [(u, int(v)) for u in range(n) for v in g.col[g.rp[u]:g.rp[u + 1]]]
+    gs = O.from_edges(n, es + [(v, u) for u, v in es])
+    C, M = int(rng.integers(1, 4)), int(rng.integers(1, 4))
+    bfs = np.zeros(n, dtype=np.uint32)
+    O.lib().orc_bfs(n, g.rp, g.col, 0, bfs)
+    lab = np.zeros(n, dtype=np.uint32)
+    O.lib().orc_cc_unionfind(n, gs.rp, gs.col, lab)
+    P, Ps = O.Partition(g, C, M), O.Partition(gs, C, M)
+    d0, st0 = P.sssp(0)
+    c0, cst0 = Ps.cc()
+    assert np.array_equal(d0, bfs) and np.array_equal(c0, lab)
+    for sched in (1, 2):
+        d, st, _ = P.sssp(0, sched=sched)
+        c, cst, _ = Ps.cc(sched=sched)
+        assert np.array_equal(d, bfs) and int(st[0]) == int(st0[0])
+        assert np.array_equal(c, lab) and int(cst[0]) == int(cst0[0])
