@@ -28,6 +28,10 @@ Other workloads (one JSON line each, same fields):
   learned  learned vs heuristic vs plain schedule on RMAT-26 (weights
            [1,255]): reference PageRank, CC, weighted SSSP; value = geometric
            mean speedup of learned over plain
+  schedules  the reference's offloaded push, its NDPEngine vote, edge
+           shipping every superstep and the heuristic on RMAT-24: CC, SSSP
+           hop and weighted; value = geometric mean speedup of edge shipping
+           over the offloaded push
 
     python bench.py [--steps K] [--warmup W] [--workload NAME] [--scale S]
                     [--parts M] [--no-cpu] [--no-check]
@@ -555,6 +559,60 @@ def run_learned(api, ctx, args):
                 cpu, extra)
 
 
+def run_schedules(api, ctx, args):
+    """The reference's two supersteps and its offload vote on RMAT-`scale`
+    (C compute + M memory partitions): plain (the offloaded push every
+    superstep, as the reference runs), the NDPEngine vote, edge shipping
+    every superstep, and the heuristic; CC on the symmetrized graph, hop and
+    weighted delta-stepping SSSP from the highest-degree vertex.  Best of
+    `steps` solves each; results must be identical across schedules."""
+    import numpy as np
+
+    scheds = [("plain", api.SCHED_PLAIN, 0), ("ndp_engine", api.SCHED_NDP_ENGINE, 0),
+              ("shipping", api.SCHED_FIXED, api.OPT_SHIP_EDGES), ("heuristic", api.SCHED_HEURISTIC, 0)]
+    out = {}
+    for name, sym, algo, kw in [("cc", True, api.CC, {}), ("sssp_hop", False, api.SSSP, {}),
+                                ("sssp_weighted", False, api.SSSP, {"weighted": True, "delta": args.delta})]:
+        G = api.Graph.rmat(ctx, args.scale, args.ef, seed=1, symmetrize=sym, wseed=2 if kw.get("weighted") else 0,
+                           C=args.compute, M=args.parts)
+        if algo == api.SSSP:
+            kw = dict(kw, source=int(np.argmax(G.out_degree())))
+        row = {"arcs": G.m, "ms": {}, "options": {}, "bytes_exchanged": {}}
+        ref = None
+        same = True
+        for label, sc, fo in scheds:
+            best = None
+            for _ in range(max(1, args.warmup) + max(1, args.steps)):
+                S = api.State(G, algo)
+                S.init(sched=sc, fixed_option=fo, max_iter=100000, **kw)
+                st = S.run()
+                r = S.result()
+                S.free()
+                best = st["ms"] if best is None else min(best, st["ms"])
+            ref = r if ref is None else ref
+            same = same and bool(np.array_equal(r, ref))
+            row["ms"][label] = round(best, 3)
+            row["options"][label] = st["option_counts"]
+            row["bytes_exchanged"][label] = st["bytes_exchanged"]
+        row["shipping_vs_plain"] = round(row["ms"]["plain"] / row["ms"]["shipping"], 3)
+        row["ndp_engine_vs_plain"] = round(row["ms"]["plain"] / row["ms"]["ndp_engine"], 3)
+        row["results_identical"] = same
+        log("schedules %s: %s" % (name, json.dumps(row)))
+        out[name] = row
+        G.free()
+    ratios = [out[k]["shipping_vs_plain"] for k in out]
+    geo = math.exp(sum(math.log(r) for r in ratios) / len(ratios))
+    config = {"workload": "offloaded push vs NDPEngine vote vs edge shipping vs heuristic: CC (symmetrized), SSSP "
+                          "hop and weighted (delta %d) from the highest-degree vertex, %s"
+                          % (args.delta, layout_desc(args, args.scale, False)),
+              "schedules": [s[0] for s in scheds]}
+    cpu = {"value": None, "unit": "x", "cores": 0, "kind": "port",
+           "sample": "none: the value is a GPU-to-GPU schedule ratio"}
+    ms = sum(out[k]["ms"]["shipping"] for k in out)
+    return line(args, geo, "x (edge shipping / plain, geometric mean of CC, SSSP hop, SSSP weighted)", ms, "u32",
+                config, None, cpu, {"schedules": out})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -567,7 +625,8 @@ def main():
     ap.add_argument("--delta", type=int, default=256, help="SSSP delta-stepping bucket width")
     ap.add_argument("--offload-only", action="store_true",
                     help="cc/sssp/pr: the heuristic never runs a push superstep as edge shipping")
-    ap.add_argument("--workload", default="pr_pull", choices=["pr_pull", "cc", "sssp", "pr", "batch", "learned"])
+    ap.add_argument("--workload", default="pr_pull", choices=["pr_pull", "cc", "sssp", "pr", "batch", "learned",
+                                                                  "schedules"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 / exactness check")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -589,7 +648,7 @@ def main():
 
     ctx = api.Context(0)
     run = {"pr_pull": run_pagerank, "pr": run_pagerank, "cc": run_cc, "sssp": run_sssp, "batch": run_batch,
-           "learned": run_learned}[args.workload]
+           "learned": run_learned, "schedules": run_schedules}[args.workload]
     out = run(api, ctx, args)
     print(json.dumps(out), flush=True)
 

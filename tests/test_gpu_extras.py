@@ -169,6 +169,15 @@ def test_cli_runs_all_algorithms(ctx, tmp_path):
     ref3, st3 = O.Partition(g, 2, 3, mirror=mirror.astype(np.uint32)).cc()
     assert np.array_equal(np.fromfile(res3, dtype=np.uint32), ref3)
     assert "STAT, CC, Iterations, %d" % int(st3[0]) in p.stdout
+    # the reference's offload vote, and edge shipping every superstep
+    for sched in ("ndp", "fixed:4"):
+        res4 = tmp_path / ("cc_%s.bin" % sched.replace(":", ""))
+        p = subprocess.run([cli, "-g", "rmat:11:8:3:sym", "-c", "1", "-m", "2", "-a", "cc", "--sched", sched,
+                            "--verify", "--result", str(res4)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=120)
+        assert p.returncode == 0, p.stdout
+        assert np.array_equal(np.fromfile(res4, dtype=np.uint32), ref)
+        assert "STAT, CC, Iterations, %d" % int(st[0]) in p.stdout
 
 
 def test_gather_probe_runs(ctx):
