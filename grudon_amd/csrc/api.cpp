@@ -26,12 +26,19 @@ grd_status fail(grd_status st, const std::string &msg)
 // Direction-optimizing heuristic (plain fallback and judge of the learned
 // choice): pull when the frontier's out-edges exceed a quarter of the arcs
 // (a push edge costs an atomic read-modify-write, a pull edge a gather of a
-// bit and a value), dense sync when more than 1/16 of the vertices or of the
-// proxies are moving.
+// bit and a value); dense sync when more than 1/16 of the vertices or of the
+// proxies are moving.  When the push would run as edge shipping the
+// crossover moves (measured per superstep and per solve on RMAT-24): for
+// min-combines shipping beats the dense pull up to about a third of the
+// arcs (1.3 vs 2.2 ms at 0.16 m, 3.1 vs 2.7 ms at 0.38 m), since its receive
+// skips the atomic where a value cannot improve; PageRank's float add takes
+// an atomic on every arc, and pulling from a sixth of the arcs is faster.
 int heuristic_option(const grd_state *s)
 {
   const grd_graph *g = s->g;
-  const bool pull = (double)s->last_frontier_edges * 4.0 > (double)g->m;
+  const bool ships = !s->prm.offload_only && s->last_uncovered == 0;
+  const double k = ships ? (s->algo == GRD_PR ? 6.0 : 3.0) : 4.0;
+  const bool pull = (double)s->last_frontier_edges * k > (double)g->m;
   const bool dense = (double)s->last_frontier > (double)g->n / 16.0 || (double)s->last_dirty > (double)g->P / 16.0;
   return (pull ? 1 : 0) | (dense ? 2 : 0);
 }
