@@ -6,6 +6,8 @@ equal iteration counts; the option of every iteration equal to the oracle's
 vote; exchange bytes equal to the oracle's count; reference PageRank within
 the FP32-vs-FP64 bound of the offloaded schedule's test.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -13,6 +15,7 @@ import oracle_lib as O
 from grudon_amd import api
 
 pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _say(*a):
@@ -169,4 +172,27 @@ def test_pr_shipping_reference_semantics(ctx, C, M):
     assert l1 <= max(1e-6, 4 * l1_32)
     assert abs(stats["iterations"] - int(st[0])) <= 3
     assert stats["option_counts"][api.OPT_SHIP_EDGES] == stats["iterations"]
+    S.free()
+
+
+@pytest.mark.parametrize("sched", [api.SCHED_HEURISTIC, api.SCHED_LEARNED])
+def test_heuristic_and_learned_ship_unless_offload_only(ctx, sched):
+    _say("ship refine", sched)
+    g = graph("doubled_dir_w")
+    w = np.fromfile(os.path.join(ROOT, "grudon_amd", "selector_bf16.bin"), dtype="<u2")
+    ctx.selector_load(w, min_margin=0.0)
+    ref = O.dijkstra(g, 0)
+    G = api.Graph.from_csr(ctx, g.rp, g.col, g.w, C=1, M=2)
+    S = api.State(G, api.SSSP)
+    S.init(source=0, weighted=True, delta=32, max_iter=100000, sched=sched)
+    st = S.run()
+    assert np.array_equal(S.result(), ref)
+    # every push decision ran as edge shipping (C = 1: always covered)
+    assert st["option_counts"][api.OPT_PUSH_SPARSE] == st["option_counts"][api.OPT_PUSH_DENSE] == 0
+    if sched == api.SCHED_HEURISTIC:  # the first frontier is one vertex: a push
+        assert st["option_counts"][api.OPT_SHIP_EDGES] > 0
+    S.init(source=0, weighted=True, delta=32, max_iter=100000, sched=sched, offload_only=True)
+    st = S.run()
+    assert np.array_equal(S.result(), ref)
+    assert st["option_counts"][api.OPT_SHIP_EDGES] == 0
     S.free()
