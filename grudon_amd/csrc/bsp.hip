@@ -332,25 +332,28 @@ __global__ void __launch_bounds__(kBlock) k_reduce_send(uint32_t nw, uint64_t *_
                                                         const uint32_t *__restrict__ wrank, int dense, int clear_vals,
                                                         uint64_t *__restrict__ bv, V *__restrict__ val, Counters *cnt)
 {
-  const uint32_t base = group_base();
   const int l = lane_id();
-  const bool ok = l < (int)kGroupWords && base + l < nw;
-  const uint64_t mine = ok ? Du[base + l] : 0ull;
-  if (ok) {
-    bv[base + l] = mine;
-    if (mine) Du[base + l] = 0;
-  }
-  uint32_t ndirty = (uint32_t)__popcll(mine);
-  uint64_t todo = dense ? __ballot(ok) : __ballot(mine != 0);
-  while (todo) {
-    const int j = __ffsll((unsigned long long)todo) - 1;
-    todo &= todo - 1;
-    const uint64_t word = shfl_u64(mine, j);
-    const uint32_t w = base + j, p = w * 64 + l;
-    const bool bit = (word >> l) & 1ull;
-    if (dense) val[p] = mupd[p];
-    else if (bit) val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())] = mupd[p];
-    if (clear_vals && bit) mupd[p] = V(0);  // PropertyList::clear (DistributedGraph.hpp:140-166)
+  uint32_t ndirty = 0;
+  // capped grid, wave-uniform loop over the word groups (fewer counter atomics)
+  for (uint32_t base = group_base(); base < nw; base += gridDim.x * kWavesPerBlock * kGroupWords) {
+    const bool ok = l < (int)kGroupWords && base + l < nw;
+    const uint64_t mine = ok ? Du[base + l] : 0ull;
+    if (ok) {
+      bv[base + l] = mine;
+      if (mine) Du[base + l] = 0;
+    }
+    ndirty += (uint32_t)__popcll(mine);
+    uint64_t todo = dense ? __ballot(ok) : __ballot(mine != 0);
+    while (todo) {
+      const int j = __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1;
+      const uint64_t word = shfl_u64(mine, j);
+      const uint32_t w = base + j, p = w * 64 + l;
+      const bool bit = (word >> l) & 1ull;
+      if (dense) val[p] = mupd[p];
+      else if (bit) val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())] = mupd[p];
+      if (clear_vals && bit) mupd[p] = V(0);  // PropertyList::clear (DistributedGraph.hpp:140-166)
+    }
   }
   uint32_t t = block_sum(ndirty);
   if (threadIdx.x == 0 && t) atomicAdd(&cnt->dirty, (unsigned long long)t);
@@ -396,57 +399,60 @@ __global__ void __launch_bounds__(kBlock) k_update(uint32_t nw, uint64_t *__rest
                                                    uint64_t *__restrict__ F, uint64_t *__restrict__ Pend,
                                                    unsigned long long hi, int use_delta, Counters *cnt)
 {
-  const uint32_t base = group_base();
   const int l = lane_id();
-  const bool ok = l < (int)kGroupWords && base + l < nw;
-  const uint64_t mine = ok ? Dc[base + l] : 0ull;
-  if (ok && !mine) F[base + l] = 0;  // nothing changed: empty next-frontier word
   uint64_t f_cnt = 0, f_edges = 0, improved = 0;
   uint32_t f_max = 0;
-  uint64_t todo = __ballot(mine != 0);
-  while (todo) {
-    const int j = __ffsll((unsigned long long)todo) - 1;
-    todo &= todo - 1;
-    const uint64_t d = shfl_u64(mine, j);
-    const uint32_t w = base + j, s = w * 64 + l;
-    bool f = false, pend = false, imp = false;
-    if ((d >> l) & 1ull) {
-      if constexpr (ALGO == GRD_PR) {
-        float *prop = (float *)prop_, *upd = (float *)upd_;
-        float u = upd[s];
-        if ((double)u > 1e-6) {  // pr.cpp:133-142 and :76-83 (prev is always 0 here)
-          f = true;
-          pr[s] = pr[s] + u;
-          prop[s] = (float)(0.85 * (double)u / (double)deg[s]);
-          upd[s] = 0.0f;
-        }
-      } else {
-        uint32_t *prop = (uint32_t *)prop_, *upd = (uint32_t *)upd_;
-        uint32_t u = upd[s];
-        if (prop[s] > u) {  // cc.cpp:100-106, sssp.cpp:102-108
-          prop[s] = u;
-          imp = true;
-          if (use_delta && (unsigned long long)u >= hi) pend = true;
-          else f = true;
+  // a capped grid walks the word groups (the loop is wave-uniform), so the
+  // block's counter atomics below are issued by fewer blocks
+  for (uint32_t base = group_base(); base < nw; base += gridDim.x * kWavesPerBlock * kGroupWords) {
+    const bool ok = l < (int)kGroupWords && base + l < nw;
+    const uint64_t mine = ok ? Dc[base + l] : 0ull;
+    if (ok && !mine) F[base + l] = 0;  // nothing changed: empty next-frontier word
+    uint64_t todo = __ballot(mine != 0);
+    while (todo) {
+      const int j = __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1;
+      const uint64_t d = shfl_u64(mine, j);
+      const uint32_t w = base + j, s = w * 64 + l;
+      bool f = false, pend = false, imp = false;
+      if ((d >> l) & 1ull) {
+        if constexpr (ALGO == GRD_PR) {
+          float *prop = (float *)prop_, *upd = (float *)upd_;
+          float u = upd[s];
+          if ((double)u > 1e-6) {  // pr.cpp:133-142 and :76-83 (prev is always 0 here)
+            f = true;
+            pr[s] = pr[s] + u;
+            prop[s] = (float)(0.85 * (double)u / (double)deg[s]);
+            upd[s] = 0.0f;
+          }
+        } else {
+          uint32_t *prop = (uint32_t *)prop_, *upd = (uint32_t *)upd_;
+          uint32_t u = upd[s];
+          if (prop[s] > u) {  // cc.cpp:100-106, sssp.cpp:102-108
+            prop[s] = u;
+            imp = true;
+            if (use_delta && (unsigned long long)u >= hi) pend = true;
+            else f = true;
+          }
         }
       }
+      const uint64_t fm = __ballot(f);
+      if (l == 0) {
+        F[w] = fm;
+        Dc[w] = 0;
+        f_cnt += (uint64_t)__popcll(fm);
+      }
+      if (use_delta) {
+        const uint64_t pm = __ballot(pend);
+        if (l == 0 && (pm | fm)) Pend[w] = (Pend[w] | pm) & ~fm;
+      }
+      if (f) {
+        uint32_t dg = deg[s];
+        f_edges += dg;
+        f_max = dg > f_max ? dg : f_max;
+      }
+      improved += imp;
     }
-    const uint64_t fm = __ballot(f);
-    if (l == 0) {
-      F[w] = fm;
-      Dc[w] = 0;
-      f_cnt += (uint64_t)__popcll(fm);
-    }
-    if (use_delta) {
-      const uint64_t pm = __ballot(pend);
-      if (l == 0 && (pm | fm)) Pend[w] = (Pend[w] | pm) & ~fm;
-    }
-    if (f) {
-      uint32_t dg = deg[s];
-      f_edges += dg;
-      f_max = dg > f_max ? dg : f_max;
-    }
-    improved += imp;
   }
   uint64_t a = block_sum(f_cnt);
   uint64_t b = block_sum(f_edges);
@@ -795,8 +801,9 @@ static grd_status reduce_t(grd_state *s, int option)
   s->xr_dense = dense ? 1 : 0;
   if (!dense) GRD_TRY(word_ranks(st, s->Du, nwP, s->blk, s->wrankR, s->totals + 1));
   const uint32_t grid = group_grid(nwP);
-  hipLaunchKernelGGL(k_reduce_send<V>, dim3(grid), dim3(kBlock), 0, st, nwP, s->Du, (V *)s->mupd, s->wrankR, (int)dense,
-                     (int)(s->algo == GRD_PR), s->bvR, (V *)s->valR, s->cnt);
+  const uint32_t grid_send = std::min<uint32_t>(grid, (uint32_t)g->ctx->num_cus * 4);
+  hipLaunchKernelGGL(k_reduce_send<V>, dim3(grid_send), dim3(kBlock), 0, st, nwP, s->Du, (V *)s->mupd, s->wrankR,
+                     (int)dense, (int)(s->algo == GRD_PR), s->bvR, (V *)s->valR, s->cnt);
   hipLaunchKernelGGL((k_reduce_recv<Op>), dim3(grid), dim3(kBlock), 0, st, nwP, s->bvR, (const V *)s->valR, s->wrankR,
                      g->proxy_slot, (int)dense, (V *)s->upd, s->Dc);
   GRD_HIP(hipGetLastError());
@@ -818,7 +825,7 @@ grd_status op_update(grd_state *s)
   grd_graph *g = s->g;
   hipStream_t st = g->ctx->stream;
   const uint32_t nwS = g->S / 64;
-  const uint32_t grid = group_grid(nwS);
+  const uint32_t grid = std::min<uint32_t>(group_grid(nwS), (uint32_t)g->ctx->num_cus * 4);
   const int use_delta = (s->algo == GRD_SSSP && s->prm.delta) ? 1 : 0;
   switch (s->algo) {
   case GRD_PR:
