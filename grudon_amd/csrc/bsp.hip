@@ -129,6 +129,16 @@ static inline uint32_t group_grid(uint64_t nwords)
   return (uint32_t)std::max<uint64_t>(1, (nwords + per - 1) / per);
 }
 
+// Passes that end in same-address counter atomics (update, reduce pack,
+// bucket selection) loop over the groups from a grid of this many
+// workgroups per CU, so that few blocks issue those atomics.
+constexpr uint32_t kLoopBlocksPerCU = 8;
+
+static inline uint32_t loop_grid(const grd_ctx *ctx, uint64_t nwords)
+{
+  return std::min<uint32_t>(group_grid(nwords), (uint32_t)ctx->num_cus * kLoopBlocksPerCU);
+}
+
 // ------------------------------------------------------------------------
 // init
 // ------------------------------------------------------------------------
@@ -470,21 +480,23 @@ __global__ void __launch_bounds__(kBlock) k_update(uint32_t nw, uint64_t *__rest
 __global__ void __launch_bounds__(kBlock) k_pend_min(uint32_t nw, const uint64_t *__restrict__ Pend,
                                                      const uint32_t *__restrict__ prop, Counters *cnt)
 {
-  const uint32_t base = group_base();
   const int l = lane_id();
-  const bool ok = l < (int)kGroupWords && base + l < nw;
-  const uint64_t mine = ok ? Pend[base + l] : 0ull;
   uint32_t mn = 0xFFFFFFFFu;
   uint64_t c = 0;
-  uint64_t todo = __ballot(mine != 0);
-  while (todo) {
-    const int j = __ffsll((unsigned long long)todo) - 1;
-    todo &= todo - 1;
-    const uint64_t pm = shfl_u64(mine, j);
-    if ((pm >> l) & 1ull) {
-      uint32_t v = prop[(base + j) * 64 + l];
-      mn = v < mn ? v : mn;
-      c++;
+  // capped grid, wave-uniform loop over the word groups
+  for (uint32_t base = group_base(); base < nw; base += gridDim.x * kWavesPerBlock * kGroupWords) {
+    const bool ok = l < (int)kGroupWords && base + l < nw;
+    const uint64_t mine = ok ? Pend[base + l] : 0ull;
+    uint64_t todo = __ballot(mine != 0);
+    while (todo) {
+      const int j = __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1;
+      const uint64_t pm = shfl_u64(mine, j);
+      if ((pm >> l) & 1ull) {
+        uint32_t v = prop[(base + j) * 64 + l];
+        mn = v < mn ? v : mn;
+        c++;
+      }
     }
   }
   mn = block_min(mn);
@@ -500,30 +512,32 @@ __global__ void __launch_bounds__(kBlock) k_select(uint32_t nw, uint64_t *__rest
                                                    const uint32_t *__restrict__ prop, const uint32_t *__restrict__ deg,
                                                    unsigned long long hi, uint64_t *__restrict__ F, Counters *cnt)
 {
-  const uint32_t base = group_base();
   const int l = lane_id();
-  const bool ok = l < (int)kGroupWords && base + l < nw;
-  const uint64_t mine = ok ? Pend[base + l] : 0ull;
-  if (ok && !mine) F[base + l] = 0;
   uint64_t f_cnt = 0, f_edges = 0;
   uint32_t f_max = 0;
-  uint64_t todo = __ballot(mine != 0);
-  while (todo) {
-    const int j = __ffsll((unsigned long long)todo) - 1;
-    todo &= todo - 1;
-    const uint64_t pm = shfl_u64(mine, j);
-    const uint32_t w = base + j, s = w * 64 + l;
-    const bool f = ((pm >> l) & 1ull) && (unsigned long long)prop[s] < hi;
-    const uint64_t fm = __ballot(f);
-    if (l == 0) {
-      F[w] = fm;
-      if (fm) Pend[w] = pm & ~fm;
-      f_cnt += (uint64_t)__popcll(fm);
-    }
-    if (f) {
-      uint32_t dg = deg[s];
-      f_edges += dg;
-      f_max = dg > f_max ? dg : f_max;
+  // capped grid, wave-uniform loop over the word groups
+  for (uint32_t base = group_base(); base < nw; base += gridDim.x * kWavesPerBlock * kGroupWords) {
+    const bool ok = l < (int)kGroupWords && base + l < nw;
+    const uint64_t mine = ok ? Pend[base + l] : 0ull;
+    if (ok && !mine) F[base + l] = 0;
+    uint64_t todo = __ballot(mine != 0);
+    while (todo) {
+      const int j = __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1;
+      const uint64_t pm = shfl_u64(mine, j);
+      const uint32_t w = base + j, s = w * 64 + l;
+      const bool f = ((pm >> l) & 1ull) && (unsigned long long)prop[s] < hi;
+      const uint64_t fm = __ballot(f);
+      if (l == 0) {
+        F[w] = fm;
+        if (fm) Pend[w] = pm & ~fm;
+        f_cnt += (uint64_t)__popcll(fm);
+      }
+      if (f) {
+        uint32_t dg = deg[s];
+        f_edges += dg;
+        f_max = dg > f_max ? dg : f_max;
+      }
     }
   }
   uint64_t a = block_sum(f_cnt), b = block_sum(f_edges);
@@ -801,7 +815,7 @@ static grd_status reduce_t(grd_state *s, int option)
   s->xr_dense = dense ? 1 : 0;
   if (!dense) GRD_TRY(word_ranks(st, s->Du, nwP, s->blk, s->wrankR, s->totals + 1));
   const uint32_t grid = group_grid(nwP);
-  const uint32_t grid_send = std::min<uint32_t>(grid, (uint32_t)g->ctx->num_cus * 4);
+  const uint32_t grid_send = loop_grid(g->ctx, nwP);
   hipLaunchKernelGGL(k_reduce_send<V>, dim3(grid_send), dim3(kBlock), 0, st, nwP, s->Du, (V *)s->mupd, s->wrankR,
                      (int)dense, (int)(s->algo == GRD_PR), s->bvR, (V *)s->valR, s->cnt);
   hipLaunchKernelGGL((k_reduce_recv<Op>), dim3(grid), dim3(kBlock), 0, st, nwP, s->bvR, (const V *)s->valR, s->wrankR,
@@ -825,7 +839,7 @@ grd_status op_update(grd_state *s)
   grd_graph *g = s->g;
   hipStream_t st = g->ctx->stream;
   const uint32_t nwS = g->S / 64;
-  const uint32_t grid = std::min<uint32_t>(group_grid(nwS), (uint32_t)g->ctx->num_cus * 4);
+  const uint32_t grid = loop_grid(g->ctx, nwS);
   const int use_delta = (s->algo == GRD_SSSP && s->prm.delta) ? 1 : 0;
   switch (s->algo) {
   case GRD_PR:
@@ -866,7 +880,7 @@ grd_status op_select_bucket(grd_state *s, bool *any_pending)
   grd_graph *g = s->g;
   hipStream_t st = g->ctx->stream;
   const uint32_t nwS = g->S / 64;
-  const uint32_t grid = group_grid(nwS);
+  const uint32_t grid = loop_grid(g->ctx, nwS);
   *any_pending = false;
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, s->cnt);
   hipLaunchKernelGGL(k_pend_min, dim3(grid), dim3(kBlock), 0, st, nwS, s->Pend, (const uint32_t *)s->prop, s->cnt);
