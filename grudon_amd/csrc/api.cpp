@@ -23,8 +23,8 @@ grd_status fail(grd_status st, const std::string &msg)
 // schedule policies
 // ------------------------------------------------------------------------
 
-// Direction-optimizing heuristic (plain fallback and judge of the learned
-// choice): pull when the frontier's out-edges exceed a quarter of the arcs
+// Direction-optimizing heuristic (the learned choice's fallback and
+// yardstick): pull when the frontier's out-edges exceed a quarter of the arcs
 // (a push edge costs an atomic read-modify-write, a pull edge a gather of a
 // bit and a value); dense sync when more than 1/16 of the vertices or of the
 // proxies are moving.  When the push would run as edge shipping the

@@ -12,9 +12,10 @@
  *   include/graph_algorithms/pr.hpp:6-7 DAMPING_FACTOR 0.85, TOLERANCE 1e-6
  *   include/GraphAlgorithm.hpp:10       MAX_ITERATIONS 1000 (a parameter here)
  *
- * Reference defects that are NOT reproduced (documented): the
- * 8x bitvector message count, the thread-count division by zero and the
- * per-thread bucket overflow.  Exchange bytes are accounted with the correct
+ * Reference defects that are NOT reproduced: the 8x bitvector message count
+ * (count = size_bytes() sent as MPI_UINT64_T, GraphAlgorithm.cpp:255,283,
+ * 412,488), the thread-count division by zero and the per-thread bucket
+ * overflow (generatePerThreadMatrix, GraphAlgorithm.cpp:10-66).  Exchange bytes are accounted with the correct
  * sizes: 8 * ceil(T/64) per bitvector plus sizeof(VP) per packed value.
  */
 #include "grudon_oracle.h"
