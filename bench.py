@@ -78,6 +78,16 @@ def cpu_threads(args):
     return model, (args.cpu_threads or min(cores, 64))
 
 
+def host_desc():
+    """Affinity core count and NUMA nodes of the host the CPU baseline ran on."""
+    _, cores = cpu_info()
+    try:
+        nodes = len([d for d in os.listdir("/sys/devices/system/node") if d.startswith("node") and d[4:].isdigit()])
+    except OSError:
+        nodes = 0
+    return {"affinity_cores": cores, "numa_nodes": nodes}
+
+
 def traffic_from_profiles(kernel_prefix):
     """HBM bytes per launch of the dominant kernel from the committed PMC
     summary (profiles/pmc_<kernel>.md: 'traffic_bytes_per_launch: N')."""
@@ -365,7 +375,7 @@ def run_pagerank(api, ctx, args):
         import oracle_check
         model, nt = cpu_threads(args)
         cpu = oracle_check.cpu_baseline(rp, col, args.compute, args.parts, nt, api.PR)
-        cpu.update(cores=nt, cpu_model=model)
+        cpu.update(cores=nt, cpu_model=model, **host_desc())
         log("cpu baseline: %s" % json.dumps(cpu))
     wl = ("PageRank pull (FP32 Kahan, FP64 dangling+residual, d=0.85, tol 1e-6)" if pull
           else "reference delta-push PageRank (FP32, d=0.85, per-vertex threshold 1e-6)")
@@ -401,7 +411,7 @@ def run_cc(api, ctx, args):
         import oracle_check
         model, nt = cpu_threads(args)
         cpu = oracle_check.cpu_baseline(rp, col, args.compute, args.parts, nt, api.CC)
-        cpu.update(cores=nt, cpu_model=model)
+        cpu.update(cores=nt, cpu_model=model, **host_desc())
     config = {"workload": "CC label propagation (u32 labels), " + layout_desc(args, args.scale, True),
               "n": n, "arcs": m, "iterations_per_step": last["iterations"], "schedule": "heuristic",
               "gteps_definition": "arcs / time of one solve"}
@@ -445,7 +455,7 @@ def run_sssp(api, ctx, args):
         model, nt = cpu_threads(args)
         cpu = oracle_check.cpu_baseline(rp, col, args.compute, args.parts, nt, api.SSSP, supersteps=12, w=w,
                                         source=int(sources[0]))
-        cpu.update(cores=nt, cpu_model=model)
+        cpu.update(cores=nt, cpu_model=model, **host_desc())
     config = {"workload": "SSSP delta-stepping (u32 distances, delta %d), weights [1,255] seed 2, 16 sources "
                           "seed 3 among degree >= 1, %s" % (args.delta, layout_desc(args, args.scale, True)),
               "n": n, "arcs": m, "iterations_per_step": iters // max(1, args.steps), "schedule": "heuristic",
@@ -531,7 +541,7 @@ def run_batch(api, ctx, args):
         import oracle_check
         model, nt = cpu_threads(args)
         cpu = oracle_check.cpu_baseline_pull(graphs, nt)
-        cpu.update(cores=nt, cpu_model=model)
+        cpu.update(cores=nt, cpu_model=model, **host_desc())
     config = {"workload": "batched PageRank pull (FP32 Kahan, FP64 residual, tol 1e-6) + CC over 64 graphs "
                           "(32 RMAT scale 14-20 EF 8-32, 32 Erdos-Renyi n 2^14-2^20 degree 4-64, seed 4) in "
                           "persistent launches with hipGraph-captured iterations",
