@@ -196,3 +196,24 @@ def test_heuristic_and_learned_ship_unless_offload_only(ctx, sched):
     assert np.array_equal(S.result(), ref)
     assert st["option_counts"][api.OPT_SHIP_EDGES] == 0
     S.free()
+
+
+@pytest.mark.parametrize("osched,sched", [(2, api.SCHED_FIXED), (1, api.SCHED_NDP_ENGINE)])
+def test_shipping_with_partition_file(ctx, osched, sched):
+    """Mirrors from a partition file (not gid % M), two compute partitions,
+    every vertex covered: the shipping schedules against the oracle."""
+    _say("ship partition file", sched)
+    g = graph("doubled_sym")
+    mirror = ((np.arange(g.n, dtype=np.uint64) * 2654435761) % 3).astype(np.uint32)
+    P = O.Partition(g, 2, 3, mirror=mirror)
+    ref, st, votes = P.cc(sched=osched)
+    G = api.Graph.from_csr(ctx, g.rp, g.col, C=2, M=3, mirror=mirror)
+    assert G.coverage()[1] == 0
+    S = api.State(G, api.CC)
+    S.init(sched=sched, fixed_option=api.OPT_SHIP_EDGES)
+    stats, tr = S.run(trace_cap=1000)
+    assert np.array_equal(S.result(), ref)
+    assert stats["iterations"] == int(st[0]) and stats["bytes_exchanged"] == int(st[1])
+    want = np.where(votes == 1, api.OPT_SHIP_EDGES, api.OPT_PUSH_SPARSE)
+    assert np.array_equal(_options(tr), want)
+    S.free()
