@@ -67,11 +67,10 @@ static bool ndp_engine_ships(const grd_state *s)
 // the updates straight to the compute side, with no memory-side update array
 // and no reduce pass over the proxy bitmaps.  Measured per superstep on
 // RMAT-22/24 (CC, SSSP, reference PageRank) it is faster than the sparse and
-// the dense push wherever the heuristic pushes (frontier arcs below a quarter
-// of the arcs), by up to 2x, except on frontiers of a few thousand arcs where
-// both take under 0.1 ms and differ by under 10 us.  So the heuristic and
-// learned schedules run a push decision as edge shipping when the frontier
-// is covered.
+// the dense push at every frontier size, by up to 2x, except on frontiers of
+// a few thousand arcs where both take under 0.1 ms and differ by under
+// 10 us.  So the heuristic and learned schedules run a push decision as edge
+// shipping when the frontier is covered.
 static int ship_refine(const grd_state *s, int option)
 {
   if (s->prm.offload_only || (option & 1) || s->last_uncovered) return option;
