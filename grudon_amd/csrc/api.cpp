@@ -66,9 +66,10 @@ static bool ndp_engine_ships(const grd_state *s)
 // On one GPU the shipping superstep is the cheaper push: the receive applies
 // the updates straight to the compute side, with no memory-side update array
 // and no reduce pass over the proxy bitmaps.  Measured per superstep on
-// RMAT-22/24 (CC, SSSP, reference PageRank) it is faster than the sparse and
-// the dense push at every frontier size, by up to 2x, except on frontiers of
-// a few thousand arcs where both take under 0.1 ms and differ by under
+// RMAT-22/24 it is faster than the sparse and the dense push, by up to 2x:
+// for CC and SSSP at every frontier size, for reference PageRank (a float
+// add per arc) wherever the heuristic pushes; the exception is frontiers of
+// a few thousand arcs, where both take under 0.1 ms and differ by under
 // 10 us.  So the heuristic and learned schedules run a push decision as edge
 // shipping when the frontier is covered.
 static int ship_refine(const grd_state *s, int option)
