@@ -71,7 +71,7 @@ def records_from(traces, exact):
     return feats, costs
 
 
-def collect(ctx, budget_s, seed=1, max_scale=22):
+def collect(ctx, budget_s, seed=1, max_scale=22, pr_max_scale=20):
     rng = np.random.default_rng(seed)
     feats, costs = [], []
     t0 = time.time()
@@ -108,7 +108,7 @@ def collect(ctx, budget_s, seed=1, max_scale=22):
             s = int(rng.choice(srcs))
             jobs.append((Gs, api.SSSP, {"source": s, "weighted": True, "delta": int(rng.choice([32, 64, 128]))},
                          2000, True))
-        if kind == "rmat" and Gd.n <= (1 << 20):
+        if kind == "rmat" and Gd.n <= (1 << pr_max_scale):
             jobs.append((Gd, api.PR, {}, 60, False))
         for G, algo, kw, mi, exact in jobs:
             S = api.State(G, algo)
@@ -129,6 +129,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--budget", type=float, default=240.0, help="seconds of record collection")
     ap.add_argument("--max-scale", type=int, default=22)
+    ap.add_argument("--pr-max-scale", type=int, default=20, help="largest RMAT scale of the PageRank jobs")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector_bf16.bin"))
     ap.add_argument("--records", default="", help="also save the records (.npz)")
     ap.add_argument("--load-records", default="", help="train from saved records instead of collecting")
@@ -142,7 +143,8 @@ def main():
         feats = np.concatenate([d["feat"] for d in parts])
         costs = np.concatenate([d["cost"] for d in parts])
     else:
-        feats, costs = collect(ctx, args.budget, seed=args.seed, max_scale=args.max_scale)
+        feats, costs = collect(ctx, args.budget, seed=args.seed, max_scale=args.max_scale,
+                               pr_max_scale=args.pr_max_scale)
         if args.records:
             np.savez_compressed(args.records, feat=feats, cost=costs)
         if args.no_train:
