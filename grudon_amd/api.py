@@ -23,12 +23,12 @@ EXPORTS = [
     "grd_generate_rmat", "grd_generate_er", "grd_edge_weights", "grd_graph_from_csr", "grd_graph_rmat",
     "grd_graph_er", "grd_graph_load", "grd_graph_read", "grd_graph_write_gr", "grd_graph_free", "grd_graph_info",
     "grd_graph_table", "grd_graph_out_degree", "grd_graph_coverage",
-    "grd_state_create", "grd_state_free", "grd_init", "grd_broadcast", "grd_traverse", "grd_reduce",
+    "grd_state_create", "grd_state_create_typed", "grd_state_free", "grd_init", "grd_broadcast", "grd_traverse", "grd_reduce",
     "grd_update", "grd_aggregate_one", "grd_run", "grd_select_option", "grd_state_stats", "grd_download",
     "grd_features", "grd_snapshot", "grd_pair_exchange",
     "grd_restore", "grd_selector_load", "grd_selector_train", "grd_selector_infer", "grd_selector_grad",
     "grd_batch_create", "grd_batch_free", "grd_batch_tune", "grd_batch_autotune", "grd_batch_pagerank",
-    "grd_batch_cc", "grd_batch_download", "grd_probe_gather",
+    "grd_batch_cc", "grd_batch_download",
 ]
 
 
@@ -96,10 +96,11 @@ def lib():
                                ct.POINTER(ct.c_int)],
             "grd_graph_write_gr": [ct.c_char_p, ct.c_uint32, vp, vp, vp],
             "grd_select_option": [vp, ct.POINTER(ct.c_int)], "grd_state_stats": [vp, ct.POINTER(RunStats)],
-            "grd_state_create": [vp, ct.c_int, pp], "grd_state_free": [vp], "grd_init": [vp, ct.POINTER(Params)],
+            "grd_state_create": [vp, ct.c_int, pp], "grd_state_create_typed": [vp, ct.c_int, ct.c_uint32, pp],
+            "grd_state_free": [vp], "grd_init": [vp, ct.POINTER(Params)],
             "grd_broadcast": [vp, ct.c_int], "grd_traverse": [vp, ct.c_int], "grd_reduce": [vp, ct.c_int],
             "grd_pair_exchange": [vp, ct.c_int, ct.c_int, ct.c_int, vp, vp, ct.POINTER(ct.c_uint64)],
-            "grd_update": [vp, ct.POINTER(ct.c_uint64)], "grd_aggregate_one": [vp, ct.c_uint32, vp],
+            "grd_update": [vp, ct.POINTER(ct.c_uint64)], "grd_aggregate_one": [vp, ct.c_int, ct.c_uint32, vp],
             "grd_run": [vp, ct.POINTER(RunStats), vp, ct.c_uint32], "grd_download": [vp, vp],
             "grd_features": [vp, vp], "grd_snapshot": [vp], "grd_restore": [vp],
             "grd_selector_load": [vp, vp, ct.c_uint64, ct.c_float],
@@ -112,7 +113,6 @@ def lib():
             "grd_batch_pagerank": [vp, ct.c_double, ct.c_double, ct.c_uint32, vp, ct.POINTER(ct.c_double)],
             "grd_batch_cc": [vp, ct.c_uint32, vp, ct.POINTER(ct.c_double)],
             "grd_batch_download": [vp, ct.c_uint32, ct.c_int, vp],
-            "grd_probe_gather": [vp, ct.c_uint64, ct.c_uint64, vp, vp],
         }
         for name, args in sigs.items():
             fn = getattr(L, name, None)
@@ -184,12 +184,6 @@ class Context:
         _check(lib().grd_edge_weights(self.h, len(rp) - 1, _ptr(rp), _ptr(col), wseed, _ptr(w)), "grd_edge_weights")
         return w[: len(col)]
 
-    def probe_gather(self, table_entries, n_gathers=1 << 28):
-        """Random-gather ceiling: (ms per pass, gathers per second)."""
-        ms, rate = ct.c_double(), ct.c_double()
-        _check(lib().grd_probe_gather(self.h, table_entries, n_gathers, ct.byref(ms), ct.byref(rate)),
-               "grd_probe_gather")
-        return ms.value, rate.value
 
     # selector
     def selector_load(self, w_bf16, min_margin=0.0):
@@ -305,12 +299,25 @@ class Graph:
 
 
 class State:
-    def __init__(self, graph, algo):
+    def __init__(self, graph, algo, value_bytes=4):
+        """value_bytes 8: the uint64 property width of CC<uint64_t> /
+        SSSP<uint64_t> (results and exchanged values come back as uint64)."""
         self.graph = graph
         self.algo = algo
+        self.value_bytes = value_bytes
         h = ct.c_void_p()
-        _check(lib().grd_state_create(graph.h, algo, ct.byref(h)), "grd_state_create")
+        _check(lib().grd_state_create_typed(graph.h, algo, value_bytes, ct.byref(h)), "grd_state_create")
         self.h = h
+
+    def _vdtype(self):
+        if self.algo in (PR, PR_PULL):
+            return np.float32
+        return np.uint64 if self.value_bytes == 8 else np.uint32
+
+    def aggregate_one(self, compute_partition, lid, value):
+        """aggregate(lid, value) on compute partition compute_partition."""
+        v = np.array([value], dtype=self._vdtype())
+        _check(lib().grd_aggregate_one(self.h, compute_partition, lid, _ptr(v)), "grd_aggregate_one")
 
     def init(self, source=0, weighted=False, delta=0, max_iter=1000, damping=0.85, tolerance=1e-6,
              sched=SCHED_PLAIN, fixed_option=0, time_iters=False, time_kernel=False, offload_only=False):
@@ -343,18 +350,18 @@ class State:
         return f.value
 
     def result(self):
-        dt = np.float32 if self.algo in (PR, PR_PULL) else np.uint32
-        out = np.zeros(self.graph.n, dtype=dt)
+        out = np.zeros(self.graph.n, dtype=self._vdtype())
         _check(lib().grd_download(self.h, _ptr(out)), "grd_download")
         return out
 
     def pair_exchange(self, direction, c, m):
-        """(gids ascending, uint32 value bits) of pair (c, m)'s broadcast
-        (direction 0) or reduce (direction 1) in the current superstep."""
+        """(gids ascending, value bits) of pair (c, m)'s broadcast (direction
+        0) or reduce (direction 1) in the current superstep: uint32 bits, or
+        uint64 values for an 8-byte state."""
         cnt = ct.c_uint64()
         _check(lib().grd_pair_exchange(self.h, direction, c, m, None, None, ct.byref(cnt)), "grd_pair_exchange")
         gids = np.zeros(max(cnt.value, 1), dtype=np.uint32)
-        vals = np.zeros(max(cnt.value, 1), dtype=np.uint32)
+        vals = np.zeros(max(cnt.value, 1), dtype=np.uint64 if self.value_bytes == 8 else np.uint32)
         _check(lib().grd_pair_exchange(self.h, direction, c, m, _ptr(gids), _ptr(vals), ct.byref(cnt)),
                "grd_pair_exchange")
         return gids[: cnt.value], vals[: cnt.value]
@@ -438,3 +445,45 @@ class Batch:
             self.free()
         except Exception:
             pass
+
+
+# ---- measurement library (bench only) ----
+
+PROBE_PATH = os.path.join(PKG, "libgrudon_probe.so")
+_probe = None
+
+
+def probe_lib():
+    """libgrudon_probe.so: the random-gather ceiling and a streaming read of
+    known size (measurement only; not part of the operator library)."""
+    global _probe
+    if _probe is None:
+        if not os.path.exists(PROBE_PATH):
+            raise GrdError("libgrudon_probe.so is not built (%s); run __graft_entry__.build()" % PROBE_PATH)
+        L = ct.CDLL(PROBE_PATH)
+        L.grdm_last_error.restype = ct.c_char_p
+        L.grdm_probe_gather.restype = ct.c_int
+        L.grdm_probe_gather.argtypes = [ct.c_int, ct.c_uint64, ct.c_uint64, ct.POINTER(ct.c_double),
+                                        ct.POINTER(ct.c_double)]
+        L.grdm_probe_stream.restype = ct.c_int
+        L.grdm_probe_stream.argtypes = [ct.c_int, ct.c_uint64, ct.c_int, ct.POINTER(ct.c_double)]
+        _probe = L
+    return _probe
+
+
+def probe_gather(device, table_entries, n_gathers=1 << 28):
+    """Random-gather ceiling: (ms per pass, gathers per second)."""
+    ms, rate = ct.c_double(), ct.c_double()
+    L = probe_lib()
+    if L.grdm_probe_gather(device, table_entries, n_gathers, ct.byref(ms), ct.byref(rate)) != 0:
+        raise GrdError("grdm_probe_gather: " + L.grdm_last_error().decode())
+    return ms.value, rate.value
+
+
+def probe_stream(device, nbytes, launches=3):
+    """Streaming read of nbytes, launches times: mean ms per launch."""
+    ms = ct.c_double()
+    L = probe_lib()
+    if L.grdm_probe_stream(device, nbytes, launches, ct.byref(ms)) != 0:
+        raise GrdError("grdm_probe_stream: " + L.grdm_last_error().decode())
+    return ms.value

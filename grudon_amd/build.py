@@ -6,6 +6,7 @@
 Outputs (in-tree, git-ignored, shipped with the working tree):
     grudon_amd/libgrudon_hip.so      HIP kernels + C-ABI (include/grudon_hip.h)
     grudon_amd/bin/grudon            CLI (-g -c -m -t -p -a), reference-compatible
+    grudon_amd/libgrudon_probe.so    measurement probes used by bench.py only
     oracle/liboracle.so              CPU oracle (test infrastructure only)
 """
 import concurrent.futures as cf
@@ -68,6 +69,17 @@ def build_library(jobs=8):
     return out
 
 
+def build_probe():
+    """Measurement-only library for bench.py (random-gather ceiling, streaming
+    read of known size); not part of libgrudon_hip.so."""
+    os.makedirs(BUILD, exist_ok=True)
+    src = os.path.join(PKG, "measure", "probe.hip")
+    out = os.path.join(PKG, "libgrudon_probe.so")
+    _run([HIPCC] + HIP_FLAGS + ["-shared", src, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_host():
     """C++ drop-in operator API + CLI, linked against libgrudon_hip.so."""
     if not os.path.isdir(HOST):
@@ -96,7 +108,7 @@ def build_oracle():
 
 
 def build_all(oracle=True):
-    outs = [build_library()]
+    outs = [build_library(), build_probe()]
     h = build_host()
     if h:
         outs.append(h)

@@ -488,15 +488,25 @@ grd_status grd_graph_coverage(const grd_graph *g, uint8_t *covered, uint64_t *n_
   return GRD_OK;
 }
 
-grd_status grd_state_create(grd_graph *g, int algo, grd_state **out)
+grd_status grd_state_create(grd_graph *g, int algo, grd_state **out) { return grd_state_create_typed(g, algo, 4, out); }
+
+grd_status grd_state_create_typed(grd_graph *g, int algo, uint32_t value_bytes, grd_state **out)
 {
   if (!g || !out) return fail(GRD_ERR_ARG, "grd_state_create: NULL argument");
   if (algo < GRD_PR || algo > GRD_PR_PULL) return fail(GRD_ERR_ARG, "grd_state_create: unknown algorithm");
+  if (value_bytes != 4 && value_bytes != 8)
+    return fail(GRD_ERR_ARG, "grd_state_create: vertex properties are 4 or 8 bytes wide");
+  if (value_bytes == 8 && (algo == GRD_PR || algo == GRD_PR_PULL))
+    return fail(GRD_ERR_ARG,
+                "grd_state_create: PageRank<double> (8-byte float properties) is not built; use PageRank<float> "
+                "or the pull form, which keeps FP64 for the dangling mass and the residual");
   GRD_HIP(hipSetDevice(g->ctx->device));
   grd_state *s = new (std::nothrow) grd_state();
   if (!s) return fail(GRD_ERR_OOM, "grd_state_create: host allocation failed");
   s->g = g;
   s->algo = algo;
+  s->vbytes = value_bytes;
+  s->wide = value_bytes == 8;
   grd_status st = state_alloc(s);
   if (st == GRD_OK && (hipEventCreate(&s->kev[0]) != hipSuccess || hipEventCreate(&s->kev[1]) != hipSuccess))
     st = fail(GRD_ERR_HIP, "grd_state_create: hipEventCreate failed");
@@ -528,6 +538,10 @@ grd_status grd_init(grd_state *s, const grd_params *p)
   if (s->algo == GRD_SSSP && p->source >= s->g->n) return fail(GRD_ERR_ARG, "grd_init: SSSP source out of range");
   if (s->algo == GRD_SSSP && p->weighted && !s->g->weighted)
     return fail(GRD_ERR_ARG, "grd_init: weighted SSSP on a graph without weights");
+  // 8-byte SSSP runs in 32 bits: every finite weighted distance must stay
+  // below the unreached value 2^32 - 1
+  if (s->wide && s->algo == GRD_SSSP && p->weighted && 255ull * (uint64_t)(s->g->n - 1) >= 0xFFFFFFFFull)
+    return fail(GRD_ERR_ARG, "grd_init: weighted SSSP<uint64_t> needs 255 * (n - 1) < 2^32 - 1");
   if (p->sched > GRD_SCHED_NDP_ENGINE || p->fixed_option > GRD_OPT_SHIP_EDGES)
     return fail(GRD_ERR_ARG, "grd_init: bad schedule");
   s->prm = *p;
@@ -631,12 +645,27 @@ grd_status grd_update(grd_state *s, uint64_t *frontier_out)
   return GRD_OK;
 }
 
-grd_status grd_aggregate_one(grd_state *s, uint32_t gid, const void *value)
+grd_status grd_aggregate_one(grd_state *s, int compute_partition, uint32_t lid, const void *value)
 {
   if (!s || !value || !s->initialized) return fail(GRD_ERR_STATE, "grd_aggregate_one: bad state");
-  if (gid >= s->g->n) return fail(GRD_ERR_ARG, "grd_aggregate_one: gid out of range");
+  const grd_graph *g = s->g;
+  if (compute_partition < 0 || compute_partition >= g->C)
+    return fail(GRD_ERR_ARG, "grd_aggregate_one: no such compute partition");
+  // compute lid = gid / C on partition gid % C (DistributedGraph.cpp:170-191)
+  const uint64_t gid = (uint64_t)lid * (uint64_t)g->C + (uint64_t)compute_partition;
+  if (gid >= g->n) return fail(GRD_ERR_ARG, "grd_aggregate_one: local id out of range for the compute partition");
   if (s->algo == GRD_PR_PULL) return fail(GRD_ERR_STATE, "grd_aggregate_one: not defined for PR_PULL");
-  return op_aggregate_one(s, s->g->h_gid_slot[gid], value);
+  uint32_t v32 = 0;
+  if (s->wide) {
+    // min-combine of 32-bit labels / distances: a value at or above the
+    // unreached 2^32 - 1 cannot improve any vertex
+    uint64_t v64;
+    memcpy(&v64, value, 8);
+    v32 = v64 >= 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)v64;
+  } else {
+    memcpy(&v32, value, 4);
+  }
+  return op_aggregate_one(s, g->h_gid_slot[gid], &v32);
 }
 
 grd_status grd_run(grd_state *s, grd_run_stats *stats, grd_iter_rec *trace, uint32_t trace_cap)
@@ -859,7 +888,12 @@ grd_status grd_pair_exchange(grd_state *s, int dir, int c, int m, uint32_t *gids
   *count = out.size();
   for (size_t i = 0; i < out.size(); i++) {
     if (gids) gids[i] = out[i].first;
-    if (values) memcpy((uint8_t *)values + 4 * i, &out[i].second, 4);
+    if (values && s->wide) {
+      const uint64_t x = out[i].second == 0xFFFFFFFFu ? ~0ull : (uint64_t)out[i].second;
+      memcpy((uint8_t *)values + 8 * i, &x, 8);
+    } else if (values) {
+      memcpy((uint8_t *)values + 4 * i, &out[i].second, 4);
+    }
   }
   return GRD_OK;
 }
@@ -890,7 +924,7 @@ grd_status grd_snapshot(grd_state *s)
   s->snap_iteration = s->iteration;
   s->snap_hi = s->hi;
   s->snap_host = {s->last_frontier, s->prev_frontier, s->last_frontier_edges, s->last_dirty, s->last_maxdeg,
-                  s->last_uncovered};
+                  s->last_uncovered, s->bytes, s->edges};
   memcpy(s->snap_fpart, s->last_fpart, sizeof(s->snap_fpart));
   return GRD_OK;
 }
@@ -913,7 +947,12 @@ grd_status grd_restore(grd_state *s)
   s->last_dirty = s->snap_host[3];
   s->last_maxdeg = (uint32_t)s->snap_host[4];
   s->last_uncovered = s->snap_host[5];
+  s->bytes = s->snap_host[6];
+  s->edges = s->snap_host[7];
   memcpy(s->last_fpart, s->snap_fpart, sizeof(s->snap_fpart));
+  // the counters restored with the device arrays (the pull PageRank's FP64
+  // dangling mass and residual live there) are the host's copy too
+  GRD_HIP(hipMemcpy(s->hcnt, s->cnt, sizeof(Counters), hipMemcpyDeviceToHost));
   s->sel_queued = false;
   s->bcast_option = -1;
   return GRD_OK;

@@ -646,6 +646,10 @@ grd_status state_alloc(grd_state *s)
     s->snap.push_back({s->pr, nullptr});
     s->snap_bytes.push_back(S * 4);
   }
+  // the counters carry the pull PageRank's FP64 dangling mass of the next
+  // update (ping-pong by iteration parity) and the last residual
+  s->snap.push_back({s->cnt, nullptr});
+  s->snap_bytes.push_back(sizeof(Counters));
   return GRD_OK;
 }
 
@@ -928,6 +932,16 @@ grd_status op_download(grd_state *s, void *out)
   GRD_HIP(hipMemcpyAsync(out, dout, (uint64_t)g->n * 4, hipMemcpyDeviceToHost, st));
   GRD_HIP(hipFreeAsync(dout, st));
   GRD_HIP(hipStreamSynchronize(st));
+  if (s->wide) {
+    // 8-byte properties: widen in place from the top, the unreached 2^32 - 1
+    // to 2^64 - 1 (numeric_limits<uint64_t>::max)
+    const uint32_t *in = (const uint32_t *)out;
+    uint64_t *o = (uint64_t *)out;
+    for (uint64_t i = g->n; i-- > 0;) {
+      const uint32_t x = in[i];
+      o[i] = x == 0xFFFFFFFFu ? ~0ull : (uint64_t)x;
+    }
+  }
   return GRD_OK;
 }
 

@@ -59,6 +59,12 @@ static grd_status read_gr(const std::string &path, uint32_t &n, std::vector<uint
     fclose(f);
     return fail(GRD_ERR_IO, "unsupported .gr file (version/size) " + path);
   }
+  // edge data: none, or the u32 weights of the reference's uint32 edge type
+  // (graph-convert --edgeType=uint32, scripts/convert_and_partition.sh:22-29)
+  if (esz != 0 && esz != 4) {
+    fclose(f);
+    return fail(GRD_ERR_IO, "unsupported .gr edge data of " + std::to_string(esz) + " bytes (0 or 4) in " + path);
+  }
   n = (uint32_t)nn;
   std::vector<uint64_t> ends(nn);
   if (nn && fread(ends.data(), 8, nn, f) != nn) {
@@ -88,7 +94,14 @@ static grd_status read_gr(const std::string &path, uint32_t &n, std::vector<uint
       return fail(GRD_ERR_IO, "truncated .gr edges in " + path);
     }
     dst32.resize(ne);
-    for (uint64_t i = 0; i < ne; i++) dst32[i] = (uint32_t)d64[i];
+    for (uint64_t i = 0; i < ne; i++) {
+      // range-check before narrowing: 2^32 + v must not alias vertex v
+      if (d64[i] >= nn) {
+        fclose(f);
+        return fail(GRD_ERR_IO, "corrupt .gr destination in " + path);
+      }
+      dst32[i] = (uint32_t)d64[i];
+    }
   }
   has_w = (esz == 4);
   if (has_w) {
@@ -107,6 +120,8 @@ static grd_status read_gr(const std::string &path, uint32_t &n, std::vector<uint
       keys[e] = ((uint64_t)v << 32) | dst32[e];
     }
   }
+  // the last node's out-index must close the arc array
+  if (e != ne) return fail(GRD_ERR_IO, "corrupt .gr offsets (the last row ends before the last arc) in " + path);
   return csr_from_pairs(n, keys, has_w ? &wts : nullptr, rp, col, w);
 }
 

@@ -180,7 +180,8 @@ struct grd_state {
   uint64_t kern_pending_bytes = 0;
   double kern_ms = 0.0;
   uint64_t kern_launches = 0, kern_bytes = 0;
-  uint32_t vbytes = 4;  // sizeof(VP)
+  uint32_t vbytes = 4;  // sizeof(VP) of the host instantiation: bytes counted per exchanged value
+  bool wide = false;    // CC / SSSP with 8-byte properties: computed in 32 bits, widened at the boundary
   grd::DevMem mem;
   // compute side [S]
   void *prop = nullptr, *upd = nullptr;
@@ -217,7 +218,7 @@ struct grd_state {
   std::vector<uint64_t> snap_bytes;
   uint32_t snap_iteration = 0;
   uint64_t snap_hi = 0;
-  std::array<uint64_t, 6> snap_host{};  // frontier statistics at the snapshot
+  std::array<uint64_t, 8> snap_host{};  // frontier statistics, bytes and edges at the snapshot
   uint64_t snap_fpart[grd::kMaxParts] = {};
 };
 

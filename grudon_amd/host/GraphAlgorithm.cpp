@@ -9,7 +9,7 @@
 template <typename VP>
 GraphAlgorithm<VP>::GraphAlgorithm(NODE_TYPE node_type, std::string algorithm_name, std::string graph_path,
                                    size_t num_compute, size_t num_memory, uint32_t node_id, Exchange &net,
-                                   std::string &partitioning_scheme_file, int grd_algo)
+                                   std::string &partitioning_scheme_file)
     : algorithm_name(algorithm_name),
       graph_path(graph_path),
       node_type(node_type),
@@ -21,9 +21,15 @@ GraphAlgorithm<VP>::GraphAlgorithm(NODE_TYPE node_type, std::string algorithm_na
   if (num_compute < 1 || num_memory < 1) throw std::runtime_error("need at least one compute and one memory partition");
   HostGraph h = load_host_graph(net, graph_path);
   dgraph.reset(new DistributedGraph(net, h, num_compute, num_memory, partitioning_scheme_file));
-  worker.reset(new UpdateWorker<VP>(dgraph.get(), num_compute, num_memory, node_id, COMPUTE_NODE, net));
-  traverser.reset(new TraverseWorker<VP>(dgraph.get(), num_compute, num_memory, node_id, MEMORY_NODE, net));
-  check(grd_state_create(dgraph->handle(), grd_algo, &state), "allocating the algorithm state");
+  // one process holds both roles: node_id names a compute partition
+  // (node_id < C) or a memory partition (node_id - C); the other role's
+  // worker takes partition 0 (src/main.cpp:92-93)
+  const uint32_t C = (uint32_t)num_compute;
+  const uint32_t cnode = node_id < C ? node_id : 0;
+  const uint32_t mnode = node_id >= C && node_id < C + num_memory ? node_id : C;
+  worker.reset(new UpdateWorker<VP>(dgraph.get(), num_compute, num_memory, cnode, COMPUTE_NODE, net));
+  traverser.reset(new TraverseWorker<VP>(dgraph.get(), num_compute, num_memory, mnode, MEMORY_NODE, net));
+  aggregate_partition = cnode;
   params.max_iter = MAX_ITERATIONS;
   params.damping = 0.85;
   params.tolerance = 1e-6;
@@ -40,6 +46,15 @@ template <typename VP>
 void GraphAlgorithm<VP>::check(grd_status st, const char *what)
 {
   if (st != GRD_OK) throw std::runtime_error(std::string(algorithm_name) + ": " + what + ": " + grd_last_error());
+}
+
+template <typename VP>
+void GraphAlgorithm<VP>::bindDevice(int grd_algo)
+{
+  if (state) grd_state_free(state);
+  state = nullptr;
+  check(grd_state_create_typed(dgraph->handle(), grd_algo, (uint32_t)sizeof(VP), &state),
+        "allocating the algorithm state");
 }
 
 template <typename VP>
@@ -96,7 +111,7 @@ template <typename VP>
 void GraphAlgorithm<VP>::exchangeView(int dir, uint32_t c, uint32_t m, std::vector<GNode> &gids,
                                       std::vector<VP> &values)
 {
-  static_assert(sizeof(VP) == 4, "the exchange packs 4-byte properties");
+  static_assert(sizeof(VP) == 4 || sizeof(VP) == 8, "the exchange packs 4- or 8-byte properties");
   uint64_t k = 0;
   check(grd_pair_exchange(state, dir, (int)c, (int)m, nullptr, nullptr, &k), "exchange read-back");
   gids.resize(k);
@@ -105,4 +120,6 @@ void GraphAlgorithm<VP>::exchangeView(int dir, uint32_t c, uint32_t m, std::vect
 }
 
 template class GraphAlgorithm<float>;
+template class GraphAlgorithm<double>;
+template class GraphAlgorithm<uint64_t>;
 template class GraphAlgorithm<uint32_t>;

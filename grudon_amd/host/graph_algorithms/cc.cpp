@@ -8,9 +8,10 @@ template <typename VP>
 CC<VP>::CC(NODE_TYPE node_type, std::string algorithm_name, std::string graph_path, size_t num_compute,
            size_t num_memory, uint32_t node_id, Exchange &net, std::string partitioning_scheme_file)
     : GraphAlgorithm<VP>(node_type, algorithm_name, graph_path, num_compute, num_memory, node_id, net,
-                         partitioning_scheme_file, GRD_CC)
+                         partitioning_scheme_file)
 {
   this->algorithm_name = "CC";
+  this->bindDevice(GRD_CC);
 }
 
 // cc.cpp:22-39: label = gid, update = MAX, every vertex in the frontier
@@ -40,11 +41,12 @@ void CC<VP>::update_frontier()
   this->check(grd_update(this->state, &this->frontier_count), "update_frontier");
 }
 
-// cc.cpp:109-113: minUpdate on the compute vertex (lid = global id here)
+// cc.cpp:109-113: minUpdate on the vertex with local id lid of the compute partition the
+// worker serves
 template <typename VP>
 void CC<VP>::aggregate(GNode &lid, const VP &buffer_val)
 {
-  this->check(grd_aggregate_one(this->state, lid, &buffer_val), "aggregate");
+  this->check(grd_aggregate_one(this->state, (int)this->aggregate_partition, lid, &buffer_val), "aggregate");
 }
 
 template <typename VP>
@@ -57,7 +59,7 @@ template <typename VP>
 void CC<VP>::printState()
 {
   std::vector<VP> r = this->result();
-  for (size_t v = 0; v < r.size(); v++) std::printf("[Proc 0] Vertex/Dist: %zu/%u\n", v, (unsigned)r[v]);
+  for (size_t v = 0; v < r.size(); v++) std::printf("[Proc 0] Vertex/Dist: %zu/%llu\n", v, (unsigned long long)r[v]);
 }
 
 // cc.cpp:139-177: every edge must join equal labels (symmetric input)
@@ -71,11 +73,12 @@ void CC<VP>::verify()
     for (uint32_t x = g.row_ptr[u]; x < g.row_ptr[u + 1]; x++)
       if (r[u] != r[g.col[x]]) {
         if (violations < 10)
-          std::printf("[Proc 0] Vertex %u and %u have different components: %u and %u\n", u, g.col[x], (unsigned)r[u],
-                      (unsigned)r[g.col[x]]);
+          std::printf("[Proc 0] Vertex %u and %u have different components: %llu and %llu\n", u, g.col[x],
+                      (unsigned long long)r[u], (unsigned long long)r[g.col[x]]);
         violations++;
       }
   std::printf("CC verify: %llu edges join different labels\n", (unsigned long long)violations);
 }
 
 template class CC<uint32_t>;
+template class CC<uint64_t>;

@@ -15,14 +15,18 @@ PageRank<VP>::PageRank(NODE_TYPE node_type, std::string algorithm_name, std::str
 {
 }
 
+// PageRank<double> (pr.hpp:39) is instantiated, and bindDevice refuses it
+// with the library's GRD_ERR_ARG message: FP64 vertex properties are not built
+
 template <typename VP>
 PageRank<VP>::PageRank(NODE_TYPE node_type, std::string algorithm_name, std::string graph_path, size_t num_compute,
                        size_t num_memory, uint32_t node_id, Exchange &net, std::string partitioning_scheme_file,
                        int grd_algo)
     : GraphAlgorithm<VP>(node_type, algorithm_name, graph_path, num_compute, num_memory, node_id, net,
-                         partitioning_scheme_file, grd_algo)
+                         partitioning_scheme_file)
 {
   this->algorithm_name = grd_algo == GRD_PR ? "PageRank" : "PageRankPull";
+  this->bindDevice(grd_algo);
 }
 
 // pr.cpp:20-61: prop = d(1-d)/deg, pr = 1-d for vertices with out-edges
@@ -54,11 +58,12 @@ void PageRank<VP>::update_frontier()
   this->check(grd_update(this->state, &this->frontier_count), "update_frontier");
 }
 
-// pr.cpp:149-153: addUpdate on the compute vertex (lid = global id here)
+// pr.cpp:149-153: addUpdate on the vertex with local id lid of the compute
+// partition the worker serves
 template <typename VP>
 void PageRank<VP>::aggregate(GNode &lid, const VP &buffer_val)
 {
-  this->check(grd_aggregate_one(this->state, lid, &buffer_val), "aggregate");
+  this->check(grd_aggregate_one(this->state, (int)this->aggregate_partition, lid, &buffer_val), "aggregate");
 }
 
 template <typename VP>
@@ -114,4 +119,5 @@ void PageRankPull<VP>::verify()
 }
 
 template class PageRank<float>;
+template class PageRank<double>;
 template class PageRankPull<float>;

@@ -9,9 +9,10 @@ template <typename VP>
 SSSP<VP>::SSSP(NODE_TYPE node_type, std::string algorithm_name, std::string graph_path, size_t num_compute,
                size_t num_memory, uint32_t node_id, Exchange &net, std::string partitioning_scheme_file)
     : GraphAlgorithm<VP>(node_type, algorithm_name, graph_path, num_compute, num_memory, node_id, net,
-                         partitioning_scheme_file, GRD_SSSP)
+                         partitioning_scheme_file)
 {
   this->algorithm_name = "SSSP";
+  this->bindDevice(GRD_SSSP);
 }
 
 // sssp.cpp:18-47: every distance MAX, the source 0 and in the frontier
@@ -40,11 +41,12 @@ void SSSP<VP>::update_frontier()
   this->check(grd_update(this->state, &this->frontier_count), "update_frontier");
 }
 
-// sssp.cpp:115-119 (lid = global id here)
+// sssp.cpp:115-119: minUpdate on the vertex with local id lid of the compute partition the
+// worker serves
 template <typename VP>
 void SSSP<VP>::aggregate(GNode &lid, const VP &buffer_val)
 {
-  this->check(grd_aggregate_one(this->state, lid, &buffer_val), "aggregate");
+  this->check(grd_aggregate_one(this->state, (int)this->aggregate_partition, lid, &buffer_val), "aggregate");
 }
 
 template <typename VP>
@@ -57,7 +59,7 @@ template <typename VP>
 void SSSP<VP>::printState()
 {
   std::vector<VP> r = this->result();
-  for (size_t v = 0; v < r.size(); v++) std::printf("[Proc 0] Vertex/Dist: %zu/%u\n", v, (unsigned)r[v]);
+  for (size_t v = 0; v < r.size(); v++) std::printf("[Proc 0] Vertex/Dist: %zu/%llu\n", v, (unsigned long long)r[v]);
 }
 
 // sssp.cpp:144-189: no edge may offer a shorter path; max and visited count
@@ -78,7 +80,8 @@ void SSSP<VP>::verify()
       uint64_t cand = (uint64_t)sd + (weighted ? g.weights[x] : 1u);
       if ((uint64_t)r[g.col[x]] > cand) {
         if (violations < 10)
-          std::printf("Wrong label: %u on node: %u, correct label from src node %u is %llu\n", (unsigned)r[g.col[x]],
+          std::printf("Wrong label: %llu on node: %u, correct label from src node %u is %llu\n",
+                      (unsigned long long)r[g.col[x]],
                       g.col[x], u, (unsigned long long)cand);
         violations++;
       }
@@ -86,8 +89,10 @@ void SSSP<VP>::verify()
     visited++;
     if (sd > maxd) maxd = sd;
   }
-  std::printf("Max Dist: %u\nTotal Visited Dist: %llu\nSSSP verify: %llu violating edges\n", (unsigned)maxd,
+  std::printf("Max Dist: %llu\nTotal Visited Dist: %llu\nSSSP verify: %llu violating edges\n",
+              (unsigned long long)maxd,
               (unsigned long long)visited, (unsigned long long)violations);
 }
 
 template class SSSP<uint32_t>;
+template class SSSP<uint64_t>;

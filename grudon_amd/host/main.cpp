@@ -15,6 +15,8 @@
 #include <vector>
 
 #include "Exchange.hpp"
+#include <type_traits>
+
 #include "GraphAlgorithm.hpp"
 #include "graph_algorithms/cc.hpp"
 #include "graph_algorithms/pr.hpp"
@@ -45,12 +47,14 @@ const char *kHelp =
     "      --print-state              print every vertex value\n"
     "      --verify                   run the algorithm's invariant check\n"
     "      --write-gr arg             also write the input graph as a Galois .gr file\n"
-    "      --result arg               write the result by gid (raw 32-bit values)\n";
+    "      --value-bytes arg          vertex property width: 4 (default) or 8 (CC<uint64_t>,\n"
+    "                                 SSSP<uint64_t>; PageRank<double> is refused)\n"
+    "      --result arg               write the result by gid (raw values of the property width)\n";
 
 struct Options {
   std::string graph_path, pfile, algo = "pr", sched = "plain", selector, write_gr, result;
   size_t num_compute = 0, num_memory = 0, threads = 1;
-  uint32_t source = 0, delta = 0, max_iter = 1000;
+  uint32_t source = 0, delta = 0, max_iter = 1000, value_bytes = 4;
   bool weighted = false, print_state = false, verify = false, help = false;
   bool has_g = false, has_c = false, has_m = false;
 };
@@ -87,6 +91,7 @@ Options parse(int argc, char **argv)
     else if (a == "--verify") o.verify = true;
     else if (a == "--write-gr") o.write_gr = need(i, a);
     else if (a == "--result") o.result = need(i, a);
+    else if (a == "--value-bytes") o.value_bytes = (uint32_t)num(need(i, a), a);
     else throw std::runtime_error("Option '" + a + "' does not exist");
   }
   return o;
@@ -106,6 +111,55 @@ int run_algorithm(GraphAlgorithm<VP> &alg, const Options &o)
     if (!out) throw std::runtime_error("cannot write " + o.result);
   }
   return 0;
+}
+
+template <typename VP>
+void write_input_gr(GraphAlgorithm<VP> &alg, const Options &o)
+{
+  if (o.write_gr.empty()) return;
+  const HostGraph &g = alg.graph().host();
+  if (grd_graph_write_gr(o.write_gr.c_str(), g.n, g.row_ptr.data(), g.col.data(),
+                         g.weights.empty() ? nullptr : g.weights.data()) != GRD_OK)
+    throw std::runtime_error(grd_last_error());
+}
+
+template <typename VP>
+int run_cc(const Options &o, Exchange &net, grd_sched sched, int fixed, uint32_t node_id, std::string &pfile)
+{
+  CC<VP> alg(COMPUTE_NODE, "", o.graph_path, o.num_compute, o.num_memory, node_id, net, pfile);
+  alg.setSchedule(sched, fixed);
+  alg.setMaxIterations(o.max_iter);
+  write_input_gr(alg, o);
+  return run_algorithm(alg, o);
+}
+
+template <typename VP>
+int run_sssp(const Options &o, Exchange &net, grd_sched sched, int fixed, uint32_t node_id, std::string &pfile)
+{
+  SSSP<VP> alg(COMPUTE_NODE, "", o.graph_path, o.num_compute, o.num_memory, node_id, net, pfile);
+  alg.setSchedule(sched, fixed);
+  alg.setMaxIterations(o.max_iter);
+  alg.setSource(o.source);
+  alg.setWeighted(o.weighted);
+  alg.setDelta(o.delta);
+  write_input_gr(alg, o);
+  return run_algorithm(alg, o);
+}
+
+template <typename VP>
+int run_pr(const Options &o, Exchange &net, grd_sched sched, int fixed, uint32_t node_id, std::string &pfile)
+{
+  std::unique_ptr<PageRank<VP>> alg;
+  if constexpr (std::is_same<VP, float>::value) {
+    if (o.algo == "prpull")
+      alg.reset(new PageRankPull<VP>(COMPUTE_NODE, "", o.graph_path, o.num_compute, o.num_memory, node_id, net, pfile));
+  }
+  if (!alg)
+    alg.reset(new PageRank<VP>(COMPUTE_NODE, "", o.graph_path, o.num_compute, o.num_memory, node_id, net, pfile));
+  alg->setSchedule(sched, fixed);
+  alg->setMaxIterations(o.max_iter);
+  write_input_gr(*alg, o);
+  return run_algorithm(*alg, o);
 }
 
 }  // namespace
@@ -137,6 +191,10 @@ int main(int argc, char **argv)
   }
   if (o.algo != "pr" && o.algo != "cc" && o.algo != "sssp" && o.algo != "prpull") {
     std::cerr << "Error parsing options: unknown algorithm '" << o.algo << "' (pr | cc | sssp | prpull)" << std::endl;
+    return 1;
+  }
+  if (o.value_bytes != 4 && o.value_bytes != 8) {
+    std::cerr << "Error parsing options: --value-bytes must be 4 or 8" << std::endl;
     return 1;
   }
   if (o.num_compute < 1 || o.num_memory < 1) {
@@ -173,47 +231,16 @@ int main(int argc, char **argv)
     }
     const uint32_t node_id = 0;
     std::string pfile = o.pfile;
-    if (o.algo == "cc") {
-      CC<uint32_t> alg(COMPUTE_NODE, "", o.graph_path, o.num_compute, o.num_memory, node_id, net, pfile);
-      alg.setSchedule(sched, fixed);
-      alg.setMaxIterations(o.max_iter);
-      if (!o.write_gr.empty()) {
-        const HostGraph &g = alg.graph().host();
-        if (grd_graph_write_gr(o.write_gr.c_str(), g.n, g.row_ptr.data(), g.col.data(),
-                               g.weights.empty() ? nullptr : g.weights.data()) != GRD_OK)
-          throw std::runtime_error(grd_last_error());
-      }
-      return run_algorithm(alg, o);
-    }
-    if (o.algo == "sssp") {
-      SSSP<uint32_t> alg(COMPUTE_NODE, "", o.graph_path, o.num_compute, o.num_memory, node_id, net, pfile);
-      alg.setSchedule(sched, fixed);
-      alg.setMaxIterations(o.max_iter);
-      alg.setSource(o.source);
-      alg.setWeighted(o.weighted);
-      alg.setDelta(o.delta);
-      if (!o.write_gr.empty()) {
-        const HostGraph &g = alg.graph().host();
-        if (grd_graph_write_gr(o.write_gr.c_str(), g.n, g.row_ptr.data(), g.col.data(),
-                               g.weights.empty() ? nullptr : g.weights.data()) != GRD_OK)
-          throw std::runtime_error(grd_last_error());
-      }
-      return run_algorithm(alg, o);
-    }
-    std::unique_ptr<PageRank<float>> alg;
-    if (o.algo == "prpull")
-      alg.reset(new PageRankPull<float>(COMPUTE_NODE, "", o.graph_path, o.num_compute, o.num_memory, node_id, net, pfile));
-    else
-      alg.reset(new PageRank<float>(COMPUTE_NODE, "", o.graph_path, o.num_compute, o.num_memory, node_id, net, pfile));
-    alg->setSchedule(sched, fixed);
-    alg->setMaxIterations(o.max_iter);
-    if (!o.write_gr.empty()) {
-      const HostGraph &g = alg->graph().host();
-      if (grd_graph_write_gr(o.write_gr.c_str(), g.n, g.row_ptr.data(), g.col.data(),
-                             g.weights.empty() ? nullptr : g.weights.data()) != GRD_OK)
-        throw std::runtime_error(grd_last_error());
-    }
-    return run_algorithm(*alg, o);
+    const bool wide = o.value_bytes == 8;
+    if (o.algo == "cc")
+      return wide ? run_cc<uint64_t>(o, net, sched, fixed, node_id, pfile)
+                  : run_cc<uint32_t>(o, net, sched, fixed, node_id, pfile);
+    if (o.algo == "sssp")
+      return wide ? run_sssp<uint64_t>(o, net, sched, fixed, node_id, pfile)
+                  : run_sssp<uint32_t>(o, net, sched, fixed, node_id, pfile);
+    if (o.algo == "prpull" && wide) throw std::runtime_error("the pull PageRank keeps 4-byte (float) ranks");
+    return wide ? run_pr<double>(o, net, sched, fixed, node_id, pfile)
+                : run_pr<float>(o, net, sched, fixed, node_id, pfile);
   } catch (const std::exception &e) {
     std::cerr << "Error: " << e.what() << std::endl;
     return 2;

@@ -199,7 +199,23 @@ grd_status grd_graph_coverage(const grd_graph *g, uint8_t *covered, uint64_t *n_
 grd_status grd_graph_digest(const grd_graph *g, uint64_t *digest16);
 
 /* ---- algorithm state: GraphAlgorithm<VP> (GraphAlgorithm.hpp:24-86) ---- */
+/* A state with 4-byte vertex properties: GraphAlgorithm<float> (GRD_PR,
+ * GRD_PR_PULL) or GraphAlgorithm<uint32_t> (GRD_CC, GRD_SSSP). */
 grd_status grd_state_create(grd_graph *g, int algo, grd_state **out);
+/* A state for the property width value_bytes of the reference's
+ * instantiation (GraphAlgorithm.hpp:88-92: float, double, uint64_t,
+ * uint32_t; CC<uint64_t> cc.hpp:32, SSSP<uint64_t> sssp.hpp:33,
+ * PageRank<double> pr.hpp:39).  value_bytes 4 is grd_state_create.
+ * value_bytes 8 is accepted for GRD_CC and GRD_SSSP: labels and distances
+ * are gids and path lengths below 2^32 - 1 (weighted SSSP needs
+ * 255 * (n - 1) < 2^32 - 1), so the device computes them in 32 bits and
+ * grd_download / grd_pair_exchange return them widened to uint64 (the
+ * unreached value 2^32 - 1 becomes 2^64 - 1, numeric_limits<uint64_t>::max,
+ * sssp.cpp:25); the exchanged bytes count 8 per value, as the reference's
+ * MPI datatype would.  value_bytes 8 for GRD_PR / GRD_PR_PULL
+ * (PageRank<double>) returns GRD_ERR_ARG: FP64 vertex properties are not
+ * built (the pull form keeps FP64 for the dangling mass and the residual). */
+grd_status grd_state_create_typed(grd_graph *g, int algo, uint32_t value_bytes, grd_state **out);
 grd_status grd_state_free(grd_state *s);
 /* init(): pr.cpp:20-61 / cc.cpp:18-40 / sssp.cpp:18-47 */
 grd_status grd_init(grd_state *s, const grd_params *p);
@@ -214,10 +230,18 @@ grd_status grd_broadcast(grd_state *s, int option);
 grd_status grd_traverse(grd_state *s, int option);
 grd_status grd_reduce(grd_state *s, int option);
 grd_status grd_update(grd_state *s, uint64_t *frontier_out);
-/* Per-element aggregate(lid, value) of the reference API
- * (GraphAlgorithm.hpp:42) on a compute vertex given by gid; slow path kept
- * for API compatibility.  value points to a uint32 or float. */
-grd_status grd_aggregate_one(grd_state *s, uint32_t gid, const void *value);
+/* Per-element aggregate(lid, buffer_val) of the reference API
+ * (GraphAlgorithm.hpp:42, dispatched by UpdateWorker::aggregate,
+ * src/Workers.cpp:71-75): combine value into vertex_updates of the vertex
+ * with local id lid on compute partition compute_partition, as run() calls it
+ * with rTranslationTable[m][bit] for every set bit of a reduce message
+ * (src/GraphAlgorithm.cpp:565-578).  Compute local ids follow the reference
+ * (DistributedGraph.cpp:170-191): lid = gid / C on partition gid % C.  PR
+ * adds (addUpdate), CC and SSSP take the minimum (minUpdate); the vertex is
+ * marked updated.  value points to a float (PR), a uint32 or, for a state
+ * created with value_bytes 8, a uint64.  Safe to call from one host thread;
+ * the bulk form is grd_reduce. */
+grd_status grd_aggregate_one(grd_state *s, int compute_partition, uint32_t lid, const void *value);
 /* Whole loop with the chosen schedule policy; trace (may be NULL) receives
  * min(iterations, trace_cap) records. */
 grd_status grd_run(grd_state *s, grd_run_stats *stats, grd_iter_rec *trace, uint32_t trace_cap);
@@ -226,7 +250,8 @@ grd_status grd_run(grd_state *s, grd_run_stats *stats, grd_iter_rec *trace, uint
 grd_status grd_select_option(grd_state *s, int *option);
 /* Counters of the iterations run so far (step-wise use). */
 grd_status grd_state_stats(const grd_state *s, grd_run_stats *out);
-/* Result by gid: CC labels / SSSP distances (uint32), PR values (float). */
+/* Result by gid: CC labels / SSSP distances (uint32, or uint64 for a state
+ * created with value_bytes 8), PR values (float). */
 grd_status grd_download(grd_state *s, void *out_by_gid);
 /* The current superstep's exchange of one pair, as the reference's public
  * Worker::bitCommVector and the packed propertyBuffers hold it
@@ -234,7 +259,8 @@ grd_status grd_download(grd_state *s, void *out_by_gid);
  * :348-425): dir 0 = the broadcast c -> m (after grd_broadcast), dir 1 = the
  * reduce m -> c (after grd_reduce).  gids[] receives the vertices whose bit
  * is set, ascending (the device's bit order is the degree order), values[]
- * the 4-byte property or update packed for each; *count their number.
+ * the property or update packed for each (4 bytes, or 8 for a state created
+ * with value_bytes 8); *count their number.
  * gids and values may be NULL.  GRD_ERR_STATE when that exchange has not
  * run in this superstep (an edge-shipping superstep has no reduce). */
 grd_status grd_pair_exchange(grd_state *s, int dir, int c, int m, uint32_t *gids, void *values, uint64_t *count);
@@ -282,15 +308,6 @@ grd_status grd_batch_pagerank(grd_batch *b, double damping, double tolerance, ui
 grd_status grd_batch_cc(grd_batch *b, uint32_t max_iter, uint32_t *iters_out, double *ms_out);
 /* Results of graph i: PR (float[n]) or CC labels (uint32[n]). */
 grd_status grd_batch_download(grd_batch *b, uint32_t i, int algo, void *out);
-
-/* ---- measurement ---- */
-/* No counterpart in the reference: the random-gather ceiling of the device,
- * against which the pull kernels' gather rate is read.  Times n_gathers 4-byte
- * gathers at uniform random positions of a table of table_entries floats,
- * with the pull's access structure (8 coalesced index loads, then 8
- * independent gathers per thread); allocates and frees its own buffers. */
-grd_status grd_probe_gather(grd_ctx *ctx, uint64_t table_entries, uint64_t n_gathers, double *ms_out,
-                            double *gathers_per_s);
 
 #ifdef __cplusplus
 }

@@ -181,11 +181,13 @@ def test_cli_runs_all_algorithms(ctx, tmp_path):
 
 
 def test_gather_probe_runs(ctx):
+    """The bench's measurement library (not part of libgrudon_hip.so)."""
     _say("gather probe")
-    ms, rate = ctx.probe_gather(1 << 16, 1 << 20)
+    ms, rate = api.probe_gather(0, 1 << 16, 1 << 20)
     assert ms > 0 and rate > 1e8
     with pytest.raises(api.GrdError):
-        ctx.probe_gather(0, 1 << 20)
+        api.probe_gather(0, 0, 1 << 20)
+    assert api.probe_stream(0, 1 << 24, 2) > 0
 
 
 def test_learned_run_decisions_match_host_features(ctx):
