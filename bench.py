@@ -1,37 +1,46 @@
 #!/usr/bin/env python3
 """Benchmark of the per-partition graph operator path on one MI355X.
 
-Default workload (the configuration the headline GTEPS is quoted on):
-PageRank, pull form, on RMAT scale 24 (edge factor 16, a/b/c/d =
-.57/.19/.19/.05, seed 1, duplicates and self-loops removed), one compute
-partition and four memory partitions resident in HBM, with the on-device
-bitvector master/mirror exchange; FP32 ranks with Kahan-compensated
-contributions, FP64 dangling mass and residual, damping 0.85, converged when
-the L1 change drops below 1e-6.
+Default workload (`headline`, the configuration the metric is quoted on):
+RMAT scale 26 (67,108,864 vertices, edge factor 16, a/b/c/d =
+.57/.19/.19/.05, seed 1, duplicates and self-loops removed, 1.06 B arcs),
+one compute partition and four memory partitions resident in HBM, with the
+on-device bitvector master/mirror exchange.
 
-A step is one complete PageRank solve (init + iterations to convergence) on
-the resident graph; GTEPS = arcs x iterations / time.  The dominant kernel
-(the pull over the memory partitions' CSC) is bracketed by HIP events on the
-context's stream inside the timed steps for the roofline figure.
+  value    PageRank, pull form: FP32 ranks with Kahan-compensated
+           contributions, FP64 dangling mass and residual, damping 0.85,
+           converged when the L1 change drops below 1e-6.  A step is one
+           complete solve (init + iterations) on the resident graph; GTEPS =
+           arcs x iterations / time.  Checked: L1 against the oracle's FP64
+           power iteration run for the same number of iterations.
+  legs     on the same scale, same partitioning, one JSON line:
+           cc        CC on RMAT-26 symmetrized (2.1 B arcs); GTEPS = arcs / time
+                     of one solve; labels checked against a parallel union-find
+           sssp      weighted SSSP (weights [1,255] seed 2, delta 256) from 16
+                     sources (seed 3, degree >= 1) on the same graph; GTEPS =
+                     harmonic mean of arcs / time; every distance vector
+                     checked against the Bellman equations
+           learned_vs_plain  plain vs heuristic vs learned schedule on the
+                     directed graph with weights: reference PageRank, CC,
+                     weighted SSSP from gid 0; best of 3 solves each
+  roofline the pull kernel of the PageRank solves (HIP events on the
+           context's stream around every launch); each leg carries its own
+  cpu_baseline  the oracle's threaded FP64 pull PageRank on the same graph
+           (2 iterations per run, median of 5 after a warm-up), and beside it
+           the restatement of the reference's delta-push superstep
 
-Other workloads (one JSON line each, same fields):
-  cc       CC on RMAT-24 symmetrized; step = one solve; GTEPS = arcs / time;
-           labels checked bit for bit against scipy connected components
-  sssp     weighted SSSP (weights [1,255], seed 2) from 16 sources (seed 3,
-           degree >= 1) on RMAT-24 symmetrized, delta-stepping; step = the 16
-           solves; GTEPS = harmonic mean of arcs / time over the sources;
-           distances checked against the Bellman equations
-  pr       the reference's delta-push PageRank (FP32) on RMAT-24
+Other workloads (one JSON line each, same fields), e.g. for profiling:
+  pr_pull  the headline's PageRank alone (default scale 24 unless --scale)
+  cc       CC on RMAT-S symmetrized; labels checked
+  sssp     weighted SSSP from 16 sources on RMAT-S symmetrized; checked
+  pr       the reference's delta-push PageRank (FP32) on RMAT-S
   batch    64 graphs of uneven size (32 RMAT scale 14-20 EF 8-32, 32
            Erdos-Renyi n 2^14-2^20 mean degree 4-64, seed 4) in one batch,
            autotuned; step = PageRank + CC on all of them
-  learned  learned vs heuristic vs plain schedule on RMAT-26 (weights
-           [1,255]): reference PageRank, CC, weighted SSSP; value = geometric
-           mean speedup of learned over plain
+  learned  learned vs heuristic vs plain schedule (RMAT-26 by default)
   schedules  the reference's offloaded push, its NDPEngine vote, edge
-           shipping every superstep and the heuristic on RMAT-24: CC, SSSP
-           hop and weighted; value = geometric mean speedup of edge shipping
-           over the offloaded push
+           shipping every superstep and the heuristic: CC, SSSP hop and
+           weighted; value = geometric mean speedup of edge shipping
 
     python bench.py [--steps K] [--warmup W] [--workload NAME] [--scale S]
                     [--parts M] [--no-cpu] [--no-check]
@@ -253,7 +262,7 @@ def timed_solves(S, args, init_kw, sources=None):
     return time.perf_counter() - t0, iters, (kms, kl, kb), last
 
 
-def schedule_speedup(api, ctx, scale, parts, ef=16, weighted=False, reps=3, sym_cc=True):
+def schedule_speedup(api, ctx, scale, parts, ef=16, weighted=False, reps=3, sym_cc=True, graph=None):
     """Learned vs plain (and heuristic) schedule on RMAT-`scale`: CC (on the
     symmetrized graph when sym_cc), SSSP from gid 0 (weighted or hop count)
     and the reference delta PageRank on the directed graph.  Speedups are
@@ -269,7 +278,7 @@ def schedule_speedup(api, ctx, scale, parts, ef=16, weighted=False, reps=3, sym_
     scheds = [("plain", api.SCHED_PLAIN), ("heuristic", api.SCHED_HEURISTIC), ("learned", api.SCHED_LEARNED)]
     jobs = [(sym_cc, api.CC, "cc", {}), (False, api.SSSP, "sssp", {"source": 0, "weighted": weighted}),
             (False, api.PR, "pr", {"max_iter": 1000})]
-    G = None
+    G = graph
     G_sym = None
     for sym, algo, name, kw in jobs:
         if sym:
@@ -318,7 +327,7 @@ def schedule_speedup(api, ctx, scale, parts, ef=16, weighted=False, reps=3, sym_
         log("schedule %s: %s" % (name, json.dumps(out[name])))
         S.free()
     for g in (G, G_sym):
-        if g is not None:
+        if g is not None and g is not graph:
             g.free()
     return out
 
@@ -351,8 +360,12 @@ def run_pagerank(api, ctx, args):
     roof = roofline_of(kms, kl, kb, kname, traffic_from_profiles("%s_rmat%d" % (kname, args.scale)))
     # edges the traversals examined (a pull examines every arc) per second
     extra = {"work_gteps": round(last["timed_edges_examined"] / secs / 1e9, 3)}
+    plan = None
+    if pull and (not args.no_check or not args.no_cpu):
+        import oracle_lib as O
+        plan = O.PullPlan(rp, col, cpu_threads(args)[1])
     if pull and not args.no_check:
-        ref = fp64_pagerank(rp, col, last["iterations"])
+        ref, _, _, _ = plan.run(iters=last["iterations"])
         extra["fp64_l1"] = float(np.abs(S.result().astype(np.float64) - ref).sum())
         log("FP64 L1 = %.3e (bar 1e-6)" % extra["fp64_l1"])
     n, m, bytes_x = G.n, G.m, last["bytes_exchanged"]
@@ -364,7 +377,7 @@ def run_pagerank(api, ctx, args):
         # L2-resident table (4 MB)
         ceil = {"pull_gathers_per_s": round(m / (kms / kl * 1e-3), 1)}
         for name, entries in (("partition_table", -(-n // args.parts)), ("l2_table", 1 << 20)):
-            _, rate = ctx.probe_gather(entries)
+            _, rate = api.probe_gather(ctx.device, entries)
             ceil[name] = {"entries": entries, "uniform_random_gathers_per_s": round(rate, 1)}
         extra["gather_ceiling"] = ceil
         log("gather ceiling: %s" % json.dumps(ceil))
@@ -374,9 +387,14 @@ def run_pagerank(api, ctx, args):
     if not args.no_cpu:
         import oracle_check
         model, nt = cpu_threads(args)
-        cpu = oracle_check.cpu_baseline(rp, col, args.compute, args.parts, nt, api.PR)
+        if pull:
+            cpu = oracle_check.cpu_baseline_pull(plan, nt)
+        else:
+            cpu = oracle_check.cpu_baseline(rp, col, args.compute, args.parts, nt, api.PR)
         cpu.update(cores=nt, cpu_model=model, **host_desc())
         log("cpu baseline: %s" % json.dumps(cpu))
+    if plan is not None:
+        plan.free()
     wl = ("PageRank pull (FP32 Kahan, FP64 dangling+residual, d=0.85, tol 1e-6)" if pull
           else "reference delta-push PageRank (FP32, d=0.85, per-vertex threshold 1e-6)")
     config = {"workload": wl + ", " + layout_desc(args, args.scale, False), "n": n, "arcs": m,
@@ -540,7 +558,7 @@ def run_batch(api, ctx, args):
     if not args.no_cpu:
         import oracle_check
         model, nt = cpu_threads(args)
-        cpu = oracle_check.cpu_baseline_pull(graphs, nt)
+        cpu = oracle_check.cpu_baseline_batch(graphs, nt)
         cpu.update(cores=nt, cpu_model=model, **host_desc())
     config = {"workload": "batched PageRank pull (FP32 Kahan, FP64 residual, tol 1e-6) + CC over 64 graphs "
                           "(32 RMAT scale 14-20 EF 8-32, 32 Erdos-Renyi n 2^14-2^20 degree 4-64, seed 4) in "
@@ -550,8 +568,130 @@ def run_batch(api, ctx, args):
     return line(args, gteps, "GTEPS", secs * 1e3 / args.steps, "f32", config, None, cpu, extra)
 
 
+def _timed_leg(api, S, args, init_kw, arcs, sources=None):
+    """Timed solves of one leg: (GTEPS, ms per step, iterations per step,
+    roofline of its dominant traversal kernel, last run stats)."""
+    secs, iters, (kms, kl, kb), last = timed_solves(S, args, init_kw, sources=sources)
+    nsrc = 1 if sources is None else len(sources)
+    gteps = arcs * nsrc * args.steps / secs / 1e9
+    return gteps, secs * 1e3 / args.steps, iters // max(1, args.steps), (kms, kl, kb), last, secs
+
+
+def run_headline(api, ctx, args):
+    """PageRank pull on RMAT-S (value), CC and weighted SSSP on RMAT-S
+    symmetrized and the learned-vs-plain schedule speedup on RMAT-S (legs),
+    each checked at full size; the CPU baseline of the same pull algorithm."""
+    import numpy as np
+
+    import oracle_check
+    import oracle_lib as O
+
+    scale = args.scale
+    model, nt = cpu_threads(args)
+    # ---- directed graph: the PageRank value and the learned leg ----
+    t0 = time.perf_counter()
+    rp, col = ctx.generate_rmat(scale, args.ef, seed=1)
+    w = ctx.edge_weights(rp, col, 2)
+    t_gen = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    G = api.Graph.from_csr(ctx, rp, col, w, C=args.compute, M=args.parts)
+    t_part = time.perf_counter() - t1
+    log("graph: n=%d m=%d proxies=%d cross=%d gen %.1fs partition+upload %.1fs device %.2f GB"
+        % (G.n, G.m, G.proxies, G.cross, t_gen, t_part, G.device_bytes / 1e9))
+    S = api.State(G, api.PR_PULL)
+    gteps, ms_step, iters, (kms, kl, kb), last, secs = _timed_leg(
+        api, S, args, {"max_iter": 1000, "tolerance": 1e-6}, G.m)
+    pr_iters = last["iterations"]
+    log("PR pull: %d steps, %.2f ms/step, %d iterations, %.1f GTEPS" % (args.steps, ms_step, pr_iters, gteps))
+    roof = roofline_of(kms, kl, kb, "k_prp_pull", traffic_from_profiles("k_prp_pull_rmat%d" % scale))
+    extra = {"work_gteps": round(last["timed_edges_examined"] / secs / 1e9, 3)}
+    plan = None
+    if not args.no_check or not args.no_cpu:
+        t2 = time.perf_counter()
+        plan = O.PullPlan(rp, col, nt)
+        log("CPU pull plan (transpose) %.1fs" % (time.perf_counter() - t2))
+    if not args.no_check:
+        ref, it, _, _ = plan.run(iters=pr_iters)
+        extra["fp64_l1"] = float(np.abs(S.result().astype(np.float64) - ref).sum())
+        log("FP64 L1 = %.3e (bar 1e-6)" % extra["fp64_l1"])
+    S.free()
+    cpu = None
+    if not args.no_cpu:
+        cpu = oracle_check.cpu_baseline_pull(plan, nt)
+        cpu.update(cores=nt, cpu_model=model, **host_desc())
+        log("cpu baseline (pull): %s" % json.dumps(cpu))
+    if plan is not None:
+        plan.free()
+        plan = None
+    legs = {}
+    if args.speedup_scale:
+        legs["learned_vs_plain"] = schedule_speedup(api, ctx, scale, args.parts, ef=args.ef, weighted=True,
+                                                    reps=3, sym_cc=False, graph=G)
+    n, m, bytes_x, proxies = G.n, G.m, last["bytes_exchanged"], G.proxies
+    G.free()
+    del rp, col, w
+    # ---- symmetrized graph: CC and weighted SSSP ----
+    t0 = time.perf_counter()
+    rps, cols = ctx.generate_rmat(scale, args.ef, seed=1, symmetrize=True)
+    ws = ctx.edge_weights(rps, cols, 2)
+    Gs = api.Graph.from_csr(ctx, rps, cols, ws, C=args.compute, M=args.parts)
+    log("symmetrized graph: m=%d, gen + partition %.1fs" % (Gs.m, time.perf_counter() - t0))
+    Sc = api.State(Gs, api.CC)
+    g_cc, ms_cc, it_cc, kcc, last_cc, secs_cc = _timed_leg(api, Sc, args, {"sched": api.SCHED_HEURISTIC}, Gs.m)
+    leg = {"gteps": round(g_cc, 3), "ms_per_step": round(ms_cc, 3), "arcs": Gs.m, "iterations": it_cc,
+           "work_gteps": round(last_cc["timed_edges_examined"] / secs_cc / 1e9, 3),
+           "roofline": roofline_of(*kcc, "k_bsp_traverse", None),
+           "workload": "CC label propagation (u32), RMAT-%d EF%d seed 1 symmetrized, heuristic schedule, "
+                       "1 step = 1 solve" % (scale, args.ef)}
+    if not args.no_check:
+        leg["labels_exact"] = oracle_check.cc_exact(rps, cols, Sc.result(), nt)
+    log("CC leg: %s" % json.dumps(leg))
+    legs["cc"] = leg
+    Sc.free()
+    deg = np.diff(rps.astype(np.int64))
+    rng = np.random.default_rng(3)
+    sources = rng.choice(np.flatnonzero(deg > 0), size=16, replace=False)
+    Ss = api.State(Gs, api.SSSP)
+    kw = {"weighted": True, "delta": args.delta, "sched": api.SCHED_HEURISTIC, "max_iter": 100000}
+    g_ss, ms_ss, it_ss, kss, last_ss, secs_ss = _timed_leg(api, Ss, args, kw, Gs.m, sources=sources)
+    leg = {"gteps": round(g_ss, 3), "ms_per_step": round(ms_ss, 3), "arcs": Gs.m, "iterations_per_step": it_ss,
+           "work_gteps": round(last_ss["timed_edges_examined"] / secs_ss / 1e9, 3),
+           "roofline": roofline_of(*kss, "k_bsp_traverse", None),
+           "sources": [int(x) for x in sources],
+           "workload": "SSSP delta-stepping (u32, delta %d), weights [1,255] seed 2, 16 sources seed 3 among "
+                       "degree >= 1, RMAT-%d EF%d seed 1 symmetrized, heuristic schedule, 1 step = the 16 "
+                       "solves; GTEPS = harmonic mean over the sources of arcs / time" % (args.delta, scale, args.ef)}
+    if not args.no_check:
+        ok = []
+        for x in sources:
+            Ss.init(source=int(x), **kw)
+            Ss.run()
+            ok.append(oracle_check.sssp_exact(rps, cols, ws, Ss.result(), int(x), nt))
+        leg["distances_exact"] = bool(all(ok))
+    log("SSSP leg: %s" % json.dumps(leg))
+    legs["sssp"] = leg
+    Ss.free()
+    Gs.free()
+    del rps, cols, ws
+    if not args.no_cpu:
+        # the reference's own superstep (delta-push PageRank) beside the pull
+        # baseline, on a bounded sample: RMAT-22, C=1 M=4, 2 supersteps
+        rp22, col22 = ctx.generate_rmat(min(scale, 22), args.ef, seed=1)
+        dp = oracle_check.cpu_baseline(rp22, col22, args.compute, args.parts, nt, api.PR)
+        dp.update(cores=nt)
+        extra["cpu_baseline_delta_push"] = dp
+        log("cpu baseline (delta push): %s" % json.dumps(dp))
+    extra["legs"] = legs
+    wl = ("PageRank pull (FP32 Kahan, FP64 dangling+residual, d=0.85, tol 1e-6), " + layout_desc(args, scale, False)
+          + "; legs: CC and weighted SSSP on RMAT-%d symmetrized, learned vs plain schedule" % scale)
+    config = {"workload": wl, "n": n, "arcs": m, "proxies": proxies, "iterations_per_step": pr_iters,
+              "exchange_bytes_per_step": bytes_x,
+              "gteps_definition": "PR: arcs x iterations / time of one solve (init included)"}
+    return line(args, gteps, "GTEPS", ms_step, "f32", config, roof, cpu, extra)
+
+
 def run_learned(api, ctx, args):
-    scale = args.scale if args.scale != 24 else 26
+    scale = args.scale
     sp = schedule_speedup(api, ctx, scale, args.parts, ef=args.ef, weighted=True, reps=max(1, args.steps),
                           sym_cc=False)
     ratios = [sp[k]["learned_vs_plain"] for k in ("pr", "cc", "sssp")]
@@ -628,26 +768,35 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--scale", type=int, default=0, help="RMAT scale (default 26 for headline/learned, else 24)")
     ap.add_argument("--ef", type=int, default=16)
     ap.add_argument("--parts", type=int, default=4)
     ap.add_argument("--compute", type=int, default=1)
     ap.add_argument("--delta", type=int, default=256, help="SSSP delta-stepping bucket width")
     ap.add_argument("--offload-only", action="store_true",
                     help="cc/sssp/pr: the heuristic never runs a push superstep as edge shipping")
-    ap.add_argument("--workload", default="pr_pull", choices=["pr_pull", "cc", "sssp", "pr", "batch", "learned",
-                                                                  "schedules"])
+    ap.add_argument("--workload", default="headline", choices=["headline", "pr_pull", "cc", "sssp", "pr", "batch",
+                                                                   "learned", "schedules"])
+    ap.add_argument("--pull", default="tiled", choices=["tiled", "plain"],
+                    help="pull PageRank: LDS-tiled pull (default) or the plain row-block pull")
+    ap.add_argument("--tier-bits", type=int, default=0, help="tiled pull: 2^bits sources per tier (0 = 15)")
+    ap.add_argument("--tile-arcs", type=int, default=0, help="tiled pull: arcs per tile (0 = 65536)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 / exactness check")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--speedup-scale", type=int, default=22,
-                    help="RMAT scale of the learned-vs-plain schedule leg of pr_pull (0 = skip)")
+    ap.add_argument("--speedup-scale", type=int, default=-1,
+                    help="pr_pull: RMAT scale of a learned-vs-plain leg (0 = skip, default); headline: 0 skips "
+                         "its learned-vs-plain leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a cold device copy of MIB MiB (PMC counter calibration)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
+    if not args.scale:
+        args.scale = 26 if args.workload in ("headline", "learned") else 24
+    if args.speedup_scale < 0:
+        args.speedup_scale = args.scale if args.workload == "headline" else 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU (partitions are resident in one HBM); "
@@ -657,7 +806,8 @@ def main():
     from grudon_amd import api
 
     ctx = api.Context(0)
-    run = {"pr_pull": run_pagerank, "pr": run_pagerank, "cc": run_cc, "sssp": run_sssp, "batch": run_batch,
+    ctx.set_pull_tiling(args.pull == "tiled", args.tier_bits, args.tile_arcs)
+    run = {"headline": run_headline, "pr_pull": run_pagerank, "pr": run_pagerank, "cc": run_cc, "sssp": run_sssp, "batch": run_batch,
            "learned": run_learned, "schedules": run_schedules}[args.workload]
     out = run(api, ctx, args)
     print(json.dumps(out), flush=True)

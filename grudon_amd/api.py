@@ -5,6 +5,7 @@ built or a call fails; there is no CPU fallback.
 """
 import ctypes as ct
 import os
+import weakref
 
 import numpy as np
 
@@ -19,7 +20,7 @@ MLP_PARAMS = 5508
 
 # every extern "C" symbol declared in include/grudon_hip.h
 EXPORTS = [
-    "grd_last_error", "grd_create", "grd_destroy", "grd_set_host_threads", "grd_set_layout", "grd_graph_digest",
+    "grd_last_error", "grd_create", "grd_destroy", "grd_set_host_threads", "grd_set_layout", "grd_set_pull_tiling", "grd_graph_digest",
     "grd_generate_rmat", "grd_generate_er", "grd_edge_weights", "grd_graph_from_csr", "grd_graph_rmat",
     "grd_graph_er", "grd_graph_load", "grd_graph_read", "grd_graph_write_gr", "grd_graph_free", "grd_graph_info",
     "grd_graph_table", "grd_graph_out_degree", "grd_graph_coverage",
@@ -79,6 +80,7 @@ def lib():
         sigs = {
             "grd_create": [ct.c_int, pp], "grd_destroy": [vp], "grd_set_host_threads": [vp, ct.c_int],
             "grd_set_layout": [vp, ct.c_int], "grd_graph_digest": [vp, vp],
+            "grd_set_pull_tiling": [vp, ct.c_int, ct.c_uint32, ct.c_uint32],
             "grd_generate_rmat": [vp, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_double, ct.c_uint64,
                                   ct.c_int, vp, vp, ct.c_uint64, ct.POINTER(ct.c_uint64)],
             "grd_generate_er": [vp, ct.c_uint32, ct.c_double, ct.c_uint64, vp, vp, ct.c_uint64,
@@ -138,9 +140,21 @@ class Context:
         h = ct.c_void_p()
         _check(lib().grd_create(device, ct.byref(h)), "grd_create")
         self.h = h
+        self.device = device
+        self._deps = []  # weak references to the graphs, states and batches made on it
+
+    def _track(self, obj):
+        self._deps.append(weakref.ref(obj))
 
     def close(self):
+        """Frees what is still alive on the context (newest first: states
+        before their graphs), then the context."""
         if self.h:
+            for ref in reversed(self._deps):
+                obj = ref()
+                if obj is not None:
+                    obj.free()
+            self._deps = []
             lib().grd_destroy(self.h)
             self.h = None
 
@@ -152,6 +166,11 @@ class Context:
 
     def set_host_threads(self, n):
         _check(lib().grd_set_host_threads(self.h, n), "grd_set_host_threads")
+
+    def set_pull_tiling(self, enable=True, tier_bits=0, tile_arcs=0):
+        """Pull PageRank states created afterwards: tiled pull (tier of
+        2^tier_bits sources, tiles of about tile_arcs arcs) or the plain pull."""
+        _check(lib().grd_set_pull_tiling(self.h, int(enable), tier_bits, tile_arcs), "grd_set_pull_tiling")
 
     def set_layout(self, on_device):
         """Build partitioned layouts with device kernels (True) or host threads."""
@@ -223,6 +242,7 @@ class Graph:
     def __init__(self, ctx, h):
         self.ctx = ctx
         self.h = h
+        ctx._track(self)
         info = np.zeros(16, dtype=np.uint64)
         _check(lib().grd_graph_info(self.h, _ptr(info)), "grd_graph_info")
         self.n, self.m, self.C, self.M = int(info[0]), int(info[1]), int(info[2]), int(info[3])
@@ -308,6 +328,7 @@ class State:
         h = ct.c_void_p()
         _check(lib().grd_state_create_typed(graph.h, algo, value_bytes, ct.byref(h)), "grd_state_create")
         self.h = h
+        graph.ctx._track(self)
 
     def _vdtype(self):
         if self.algo in (PR, PR_PULL):
@@ -407,6 +428,7 @@ class Batch:
         _check(lib().grd_batch_create(ctx.h, cnt, _ptr(self.n), _ptr(self.m), rps, cols, ct.byref(h)),
                "grd_batch_create")
         self.h = h
+        ctx._track(self)
         self.count = cnt
 
     def tune(self, tile=0, grid_per_cu=0, K=0):

@@ -275,6 +275,17 @@ grd_status grd_destroy(grd_ctx *c)
   return GRD_OK;
 }
 
+grd_status grd_set_pull_tiling(grd_ctx *c, int enable, uint32_t tier_bits, uint32_t tile_arcs)
+{
+  if (!c) return fail(GRD_ERR_ARG, "grd_set_pull_tiling: NULL context");
+  if (tier_bits && (tier_bits < 4 || tier_bits > kTierBitsMax))
+    return fail(GRD_ERR_ARG, "grd_set_pull_tiling: tier bits must be 0 (default) or 4..15");
+  c->pull_tiled = enable != 0;
+  c->tier_bits = tier_bits;
+  c->tile_arcs = tile_arcs;
+  return GRD_OK;
+}
+
 grd_status grd_set_layout(grd_ctx *c, int on_device)
 {
   if (!c) return fail(GRD_ERR_ARG, "grd_set_layout: NULL context");
@@ -507,7 +518,16 @@ grd_status grd_state_create_typed(grd_graph *g, int algo, uint32_t value_bytes, 
   s->algo = algo;
   s->vbytes = value_bytes;
   s->wide = value_bytes == 8;
-  grd_status st = state_alloc(s);
+  grd_status st = GRD_OK;
+  if (algo == GRD_PR_PULL && g->ctx->pull_tiled) {
+    try {
+      st = build_tiers(g);
+    } catch (const std::bad_alloc &) {
+      st = fail(GRD_ERR_OOM, "tiled pull: host allocation failed");
+    }
+    s->tiled = st == GRD_OK;
+  }
+  if (st == GRD_OK) st = state_alloc(s);
   if (st == GRD_OK && (hipEventCreate(&s->kev[0]) != hipSuccess || hipEventCreate(&s->kev[1]) != hipSuccess))
     st = fail(GRD_ERR_HIP, "grd_state_create: hipEventCreate failed");
   if (st != GRD_OK) {

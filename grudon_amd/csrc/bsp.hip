@@ -634,6 +634,10 @@ grd_status state_alloc(grd_state *s)
   GRD_TRY(m.alloc_n(&s->heavy, 2 * (g->m / kHeavyDeg) + 64));
   GRD_TRY(m.alloc(&s->partial, (uint64_t)std::max<uint32_t>(1, g->n_pblk) * 8));
   if (s->algo == GRD_PR_PULL) GRD_TRY(m.alloc_n(&s->red, 2ull * prp_grid(g)));
+  if (s->tiled) {
+    GRD_TRY(m.alloc_n(&s->tpart, std::max<uint64_t>(g->n_pairs, 1)));
+    GRD_TRY(m.alloc_n(&s->tseg, std::max<uint32_t>(g->n_tblk, 1)));
+  }
   GRD_TRY(m.alloc_n(&s->cnt, 1));
   GRD_HIP(hipHostMalloc((void **)&s->hcnt, sizeof(Counters), hipHostMallocDefault));
   memset(s->hcnt, 0, sizeof(Counters));

@@ -96,12 +96,20 @@ struct PullBlock {
 struct PullLong {
   uint32_t row, first, nseg, pad;
 };
+// tiled PageRank pull: one tile = a run of row blocks of one (memory
+// partition, source tier) group, whose tier window mprop[wbase, wbase + wlen)
+// is staged in LDS once
+struct TierTile {
+  uint32_t b0, nb, wbase, wlen;
+};
 constexpr uint32_t kPullNnz = 2048;     // arcs per block (8 per thread)
 constexpr uint32_t kPullMaxRows = 1024;  // rows per block
 constexpr uint32_t kPullHot = 2048;      // PR pull: hottest sources of a partition kept in LDS
 constexpr uint32_t kHeavyDeg = 4096;     // push: out-degree handled by k_push_heavy, in chunks of this many arcs
 constexpr uint32_t kShipTile = 1024;     // edge shipping: wire positions per workgroup tile
 constexpr int kMaxParts = 64;            // largest C and M
+constexpr uint32_t kTierBitsMax = 15;    // tiled pull: largest tier, 2^15 sources (128 KB of LDS)
+constexpr uint32_t kTierTileArcs = 65536;  // tiled pull: default arcs per tile
 
 }  // namespace grd
 
@@ -111,6 +119,9 @@ struct grd_ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int host_threads = 0;
   int num_cus = 256;
+  bool pull_tiled = true;   // pull PageRank states use the tiled pull (tiers.hip)
+  uint32_t tier_bits = 0;   // its tunables for graphs tiled later (0 = default)
+  uint32_t tile_arcs = 0;
   bool layout_on_device = true;  // build the partitioned layout with device kernels (else on the host)
   // learned selector weights (BF16, device) and threshold
   uint16_t *sel_w = nullptr;
@@ -158,6 +169,25 @@ struct grd_graph {
   uint32_t n_pblk = 0;
   grd::PullLong *plong = nullptr;
   uint32_t n_plong = 0;
+  // tiled PageRank pull (tiers.hip), built with the first pull-PageRank
+  // state: the CSC regrouped by (memory partition, source tier, row), a
+  // (row, tier) "pair" per row and tier it has arcs in
+  bool tiers_built = false;
+  uint32_t tier_bits = 0;          // a tier is 2^tier_bits consecutive proxies of a partition
+  uint32_t tile_arcs = 0;
+  uint64_t n_pairs = 0;
+  uint16_t *tcol = nullptr;        // [m] source offset within the pair's tier window
+  uint32_t *tptr = nullptr;        // [pairs+1] arcs of each pair, pairs in tier order
+  uint32_t *trow = nullptr;        // [pairs] proxy row of the pair | 1 << 31 when the row has one pair
+  uint32_t *trptr = nullptr;       // [P+1] pairs of each proxy row, in (row, tier) order
+  uint32_t *tinv = nullptr;        // [pairs] tier-order index of the pairs in (row, tier) order
+  grd::PullBlock *tblk = nullptr;  // row blocks over pairs (r0 = first pair); nrows 0 = segment of a long pair
+  uint32_t n_tblk = 0;
+  grd::PullLong *tlong = nullptr;  // long pairs (row = pair index)
+  uint32_t n_tlong = 0;
+  grd::TierTile *ttile = nullptr;
+  uint32_t n_ttile = 0;
+  grd::DevMem tmem;
 };
 
 struct grd_state {
@@ -198,6 +228,9 @@ struct grd_state {
   uint32_t *AL = nullptr;
   uint2 *heavy = nullptr;  // push: (proxy, first arc offset) of each heavy-vertex chunk
   void *partial = nullptr;  // pull: one 8-byte partial per row block (long-row segments)
+  bool tiled = false;       // PR_PULL: the tiled pull runs (else the plain row-block pull)
+  float2 *tpart = nullptr;  // tiled pull: Kahan partial per pair, then per tier-block segment
+  float2 *tseg = nullptr;
   double *red = nullptr;    // PR pull update: (residual, dangling) per workgroup
   // edge shipping wire (grown on demand): record lengths / offsets, the first
   // record of every tile, the records (gids) and their edge weights
@@ -260,6 +293,9 @@ void take_frontier_parts(grd_state *s);
 grd_status build_ship_tables(grd_graph *g);
 grd_status op_frontier_parts(grd_state *s);  // per-partition frontier counts into the counters (C > 1)
 grd_status op_ship(grd_state *s);            // edge shipping: memory send + compute apply
+// tiers.hip
+grd_status build_tiers(grd_graph *g);
+grd_status prp_pull_tiled(grd_state *s);  // pull + long-pair fix-up + merge into valR
 // prpull.hip
 grd_status prp_init(grd_state *s);
 uint32_t prp_grid(const grd_graph *g);  // workgroups of the PR pull update

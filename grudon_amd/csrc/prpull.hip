@@ -144,13 +144,18 @@ grd_status prp_traverse(grd_state *s)
   hipStream_t st = g->ctx->stream;
   const bool timed = s->prm.time_kernel != 0;
   if (timed) GRD_HIP(hipEventRecord(s->kev[0], st));
-  GRD_TRY(prp_pull_launch(s));
+  if (s->tiled) {
+    // tiled pull, long-pair fix-up and the per-row merge: the whole pull
+    GRD_TRY(prp_pull_tiled(s));
+  } else {
+    GRD_TRY(prp_pull_launch(s));
+  }
   if (timed) {
     GRD_HIP(hipEventRecord(s->kev[1], st));
     s->kern_pending = 1;
     s->kern_pending_bytes = 8 * g->m + 8ull * g->P;
   }
-  GRD_TRY(prp_pull_fixup(s));
+  if (!s->tiled) GRD_TRY(prp_pull_fixup(s));
   s->edges += g->m;
   s->bytes += ((uint64_t)g->n + g->X) * 4;
   return GRD_OK;

@@ -135,6 +135,17 @@ grd_status grd_set_host_threads(grd_ctx *ctx, int nthreads);
  * with device kernels, 0 with host threads.  Both give the same layout. */
 grd_status grd_set_layout(grd_ctx *ctx, int on_device);
 
+/* No counterpart in the reference: how pull-PageRank states (GRD_PR_PULL)
+ * created afterwards run their pull.  enable != 0 (the default): the tiled
+ * pull, which regroups a graph's CSC by (memory partition, source tier, row)
+ * when its first pull-PageRank state is created and serves every gather from
+ * a tier window staged in LDS; a tier is 2^tier_bits consecutive proxies
+ * (4..15, 0 = 15) and a tile about tile_arcs arcs (0 = 65536).  A graph
+ * keeps the tiling it was first given.  enable == 0: the plain row-block
+ * pull over the CSC.  Both compute the same sums within FP32 rounding (Kahan
+ * partials merged in a fixed order). */
+grd_status grd_set_pull_tiling(grd_ctx *ctx, int enable, uint32_t tier_bits, uint32_t tile_arcs);
+
 /* ---- graphs ---- */
 
 /* Seeded device generators; fill caller host arrays with the global CSR

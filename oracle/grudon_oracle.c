@@ -22,6 +22,7 @@
 
 #include <math.h>
 #include <pthread.h>
+#include <time.h>
 #include <stdatomic.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -838,4 +839,226 @@ int orc_pr_pull_mt(uint32_t n, const uint32_t *rp, const uint32_t *col, double d
   free(contrib);
   free(nx);
   return it;
+}
+
+/* ------------------------------------------------------------------ */
+/* full-size checks and the timed CPU pull (bench.py)                  */
+/* ------------------------------------------------------------------ */
+
+struct orc_pull_plan {
+  uint32_t n;
+  int nthreads;
+  uint32_t *trp, *tcol, *deg;
+  double *contrib, *nx;
+};
+
+typedef struct {
+  const uint32_t *rp, *col;
+  uint32_t *trp, *deg;
+  _Atomic uint32_t *cur;
+  uint32_t *tcol;
+} tr_ctx;
+
+static void tr_count(void *vc, uint64_t b, uint64_t e)
+{
+  tr_ctx *c = (tr_ctx *)vc;
+  for (uint64_t u = b; u < e; u++) {
+    c->deg[u] = c->rp[u + 1] - c->rp[u];
+    for (uint32_t x = c->rp[u]; x < c->rp[u + 1]; x++) atomic_fetch_add_explicit(&c->cur[c->col[x]], 1u, memory_order_relaxed);
+  }
+}
+
+static void tr_fill(void *vc, uint64_t b, uint64_t e)
+{
+  tr_ctx *c = (tr_ctx *)vc;
+  for (uint64_t u = b; u < e; u++)
+    for (uint32_t x = c->rp[u]; x < c->rp[u + 1]; x++)
+      c->tcol[atomic_fetch_add_explicit(&c->cur[c->col[x]], 1u, memory_order_relaxed)] = (uint32_t)u;
+}
+
+static int cmp_u32(const void *a, const void *b)
+{
+  uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+  return (x > y) - (x < y);
+}
+
+static void tr_sort(void *vc, uint64_t b, uint64_t e)
+{
+  tr_ctx *c = (tr_ctx *)vc;
+  for (uint64_t v = b; v < e; v++) {
+    uint32_t lo = c->trp[v], hi = c->trp[v + 1];
+    if (hi - lo > 1) qsort(c->tcol + lo, hi - lo, sizeof(uint32_t), cmp_u32);
+  }
+}
+
+orc_pull_plan *orc_pull_plan_build(uint32_t n, const uint32_t *rp, const uint32_t *col, int nthreads)
+{
+  orc_pull_plan *p = (orc_pull_plan *)xcalloc(1, sizeof(orc_pull_plan));
+  const uint64_t m = rp[n];
+  p->n = n;
+  p->nthreads = nthreads;
+  p->trp = (uint32_t *)xcalloc((size_t)n + 1, sizeof(uint32_t));
+  p->tcol = (uint32_t *)xcalloc(m, sizeof(uint32_t));
+  p->deg = (uint32_t *)xcalloc(n, sizeof(uint32_t));
+  p->contrib = (double *)xcalloc(n, sizeof(double));
+  p->nx = (double *)xcalloc(n, sizeof(double));
+  _Atomic uint32_t *cur = (_Atomic uint32_t *)xcalloc(n, sizeof(uint32_t));
+  tr_ctx c = {rp, col, p->trp, p->deg, cur, p->tcol};
+  parallel_for(nthreads, n, 16384, tr_count, &c);
+  /* in-degree counts -> row starts; the cursors restart at the row starts */
+  uint32_t acc = 0;
+  for (uint32_t v = 0; v < n; v++) {
+    uint32_t k = atomic_load_explicit(&cur[v], memory_order_relaxed);
+    p->trp[v] = acc;
+    atomic_store_explicit(&cur[v], acc, memory_order_relaxed);
+    acc += k;
+  }
+  p->trp[n] = acc;
+  parallel_for(nthreads, n, 16384, tr_fill, &c);
+  /* ascending sources per row: the summation order of orc_pr_pull_mt */
+  parallel_for(nthreads, n, 4096, tr_sort, &c);
+  free((void *)cur);
+  return p;
+}
+
+int orc_pull_plan_run(orc_pull_plan *p, double d, int iters, double tol, double *r, double *last_residual,
+                      double *seconds)
+{
+  struct timespec t0, t1;
+  const uint32_t n = p->n;
+  for (uint32_t i = 0; i < n; i++) r[i] = 1.0 / (double)n;
+  prmt_ctx c = {n, p->trp, p->tcol, p->deg, r, p->contrib, p->nx, 0.0, d};
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  int it = 0, cap = iters > 0 ? iters : 10000;
+  double res = 0.0;
+  while (it < cap) {
+    double D = 0.0;
+    for (uint32_t u = 0; u < n; u++)
+      if (!p->deg[u]) D += r[u];
+    c.base = (1.0 - d) / (double)n + d * D / (double)n;
+    parallel_for(p->nthreads, n, 65536, prmt_contrib, &c);
+    parallel_for(p->nthreads, n, 16384, prmt_pull, &c);
+    res = 0.0;
+    for (uint32_t v = 0; v < n; v++) {
+      res += fabs(p->nx[v] - r[v]);
+      r[v] = p->nx[v];
+    }
+    it++;
+    if (iters <= 0 && res < tol) break;
+  }
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  if (last_residual) *last_residual = res;
+  if (seconds) *seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+  return it;
+}
+
+uint64_t orc_pull_plan_arcs(const orc_pull_plan *p) { return p->trp[p->n]; }
+
+void orc_pull_plan_free(orc_pull_plan *p)
+{
+  if (!p) return;
+  free(p->trp);
+  free(p->tcol);
+  free(p->deg);
+  free(p->contrib);
+  free(p->nx);
+  free(p);
+}
+
+/* parallel union-find: a root is linked only under a smaller root, so the
+ * root of every set is its smallest gid */
+typedef struct {
+  const uint32_t *rp, *col;
+  _Atomic uint32_t *pa;
+  uint32_t *label;
+} uf_ctx;
+
+static uint32_t ufm_find(_Atomic uint32_t *pa, uint32_t x)
+{
+  for (;;) {
+    uint32_t p = atomic_load_explicit(&pa[x], memory_order_relaxed);
+    if (p == x) return x;
+    uint32_t gp = atomic_load_explicit(&pa[p], memory_order_relaxed);
+    if (gp != p) atomic_compare_exchange_weak_explicit(&pa[x], &p, gp, memory_order_relaxed, memory_order_relaxed);
+    x = gp;
+  }
+}
+
+static void uf_union_range(void *vc, uint64_t b, uint64_t e)
+{
+  uf_ctx *c = (uf_ctx *)vc;
+  for (uint64_t u = b; u < e; u++)
+    for (uint32_t x = c->rp[u]; x < c->rp[u + 1]; x++) {
+      uint32_t a = (uint32_t)u, v = c->col[x];
+      for (;;) {
+        uint32_t ra = ufm_find(c->pa, a), rb = ufm_find(c->pa, v);
+        if (ra == rb) break;
+        uint32_t hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
+        uint32_t expect = hi;
+        if (atomic_compare_exchange_strong_explicit(&c->pa[hi], &expect, lo, memory_order_relaxed,
+                                                    memory_order_relaxed))
+          break;
+        a = hi;
+        v = lo;
+      }
+    }
+}
+
+static void uf_label_range(void *vc, uint64_t b, uint64_t e)
+{
+  uf_ctx *c = (uf_ctx *)vc;
+  for (uint64_t v = b; v < e; v++) c->label[v] = ufm_find(c->pa, (uint32_t)v);
+}
+
+void orc_cc_unionfind_mt(uint32_t n, const uint32_t *rp, const uint32_t *col, uint32_t *label, int nthreads)
+{
+  _Atomic uint32_t *pa = (_Atomic uint32_t *)xcalloc(n, sizeof(uint32_t));
+  for (uint32_t i = 0; i < n; i++) atomic_init(&pa[i], i);
+  uf_ctx c = {rp, col, pa, label};
+  parallel_for(nthreads, n, 4096, uf_union_range, &c);
+  parallel_for(nthreads, n, 65536, uf_label_range, &c);
+  free((void *)pa);
+}
+
+/* Bellman equations of a distance vector (positive weights, or +1 per arc
+ * when w is NULL): dist[src] = 0; no arc u -> v offers dist[u] + w < dist[v]
+ * (an unreached 2^32 - 1 below a reached vertex counts); every reached
+ * v != src has an arc u -> v with dist[u] + w == dist[v].  These hold for
+ * the shortest-path distances and for no other vector. */
+typedef struct {
+  const uint32_t *rp, *col, *w, *dist;
+  uint8_t *tight;
+  _Atomic uint64_t bad;
+} bc_ctx;
+
+static void bc_arcs(void *vc, uint64_t b, uint64_t e)
+{
+  bc_ctx *c = (bc_ctx *)vc;
+  uint64_t bad = 0;
+  for (uint64_t u = b; u < e; u++) {
+    const uint32_t du = c->dist[u];
+    if (du == 0xFFFFFFFFu) continue;
+    for (uint32_t x = c->rp[u]; x < c->rp[u + 1]; x++) {
+      const uint32_t v = c->col[x];
+      const uint64_t cand = (uint64_t)du + (c->w ? c->w[x] : 1u);
+      if ((uint64_t)c->dist[v] > cand) bad++;
+      else if ((uint64_t)c->dist[v] == cand) c->tight[v] = 1;
+    }
+  }
+  if (bad) atomic_fetch_add(&c->bad, bad);
+}
+
+uint64_t orc_sssp_check_mt(uint32_t n, const uint32_t *rp, const uint32_t *col, const uint32_t *w,
+                           const uint32_t *dist, uint32_t src, int nthreads)
+{
+  uint8_t *tight = (uint8_t *)xcalloc(n, 1);
+  bc_ctx c = {rp, col, w, dist, tight, 0};
+  atomic_init(&c.bad, 0);
+  parallel_for(nthreads, n, 4096, bc_arcs, &c);
+  uint64_t bad = atomic_load(&c.bad);
+  if (src >= n || dist[src] != 0) bad++;
+  for (uint32_t v = 0; v < n; v++)
+    if (v != src && dist[v] != 0xFFFFFFFFu && !tight[v]) bad++;
+  free(tight);
+  return bad;
 }

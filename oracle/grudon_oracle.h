@@ -161,6 +161,26 @@ int orc_pr_pull_mt(uint32_t n, const uint32_t *row_ptr, const uint32_t *col,
                    double damping, int iters, double tol, double *r,
                    double *last_residual, int nthreads);
 
+/* ---- full-size checks and the timed CPU pull (bench.py) ---- */
+
+/* The threaded FP64 pull of orc_pr_pull_mt with its transposed graph built
+ * once (in parallel; sources ascending per row, as orc_pr_pull_mt sums
+ * them), so that solves can be repeated and timed: orc_pull_plan_run starts
+ * from r0 = 1/n and sets *seconds to the wall time of the iteration loop. */
+typedef struct orc_pull_plan orc_pull_plan;
+orc_pull_plan *orc_pull_plan_build(uint32_t n, const uint32_t *row_ptr, const uint32_t *col, int nthreads);
+int orc_pull_plan_run(orc_pull_plan *p, double damping, int iters, double tol, double *r,
+                      double *last_residual, double *seconds);
+uint64_t orc_pull_plan_arcs(const orc_pull_plan *p);
+void orc_pull_plan_free(orc_pull_plan *p);
+/* min gid per weakly connected component by lock-free parallel union-find */
+void orc_cc_unionfind_mt(uint32_t n, const uint32_t *row_ptr, const uint32_t *col, uint32_t *label,
+                         int nthreads);
+/* Number of violated Bellman conditions of dist from src (0: dist holds the
+ * shortest-path distances; w NULL = +1 per arc; unreached = 2^32 - 1). */
+uint64_t orc_sssp_check_mt(uint32_t n, const uint32_t *row_ptr, const uint32_t *col, const uint32_t *w,
+                           const uint32_t *dist, uint32_t src, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

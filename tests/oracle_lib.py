@@ -68,6 +68,19 @@ def lib():
         L.orc_pr_pull.restype = ct.c_int
         L.orc_pr_pull.argtypes = [ct.c_uint32, _P32, _P32, ct.c_double, ct.c_int, ct.c_double, _PF64,
                                   ct.POINTER(ct.c_double)]
+        L.orc_pull_plan_build.restype = ct.c_void_p
+        L.orc_pull_plan_build.argtypes = [ct.c_uint32, _P32, _P32, ct.c_int]
+        L.orc_pull_plan_run.restype = ct.c_int
+        L.orc_pull_plan_run.argtypes = [ct.c_void_p, ct.c_double, ct.c_int, ct.c_double, _PF64,
+                                        ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]
+        L.orc_pull_plan_arcs.restype = ct.c_uint64
+        L.orc_pull_plan_arcs.argtypes = [ct.c_void_p]
+        L.orc_pull_plan_free.restype = None
+        L.orc_pull_plan_free.argtypes = [ct.c_void_p]
+        L.orc_cc_unionfind_mt.restype = None
+        L.orc_cc_unionfind_mt.argtypes = [ct.c_uint32, _P32, _P32, _P32, ct.c_int]
+        L.orc_sssp_check_mt.restype = ct.c_uint64
+        L.orc_sssp_check_mt.argtypes = [ct.c_uint32, _P32, _P32, ct.c_void_p, _P32, ct.c_uint32, ct.c_int]
         L.orc_pr_pull_mt.restype = ct.c_int
         L.orc_pr_pull_mt.argtypes = [ct.c_uint32, _P32, _P32, ct.c_double, ct.c_int, ct.c_double, _PF64,
                                      ct.POINTER(ct.c_double), ct.c_int]
@@ -245,3 +258,53 @@ def pr_pull(g, damping=0.85, iters=0, tol=1e-10, nthreads=0):
     else:
         it = lib().orc_pr_pull(g.n, g.rp, g.col, damping, iters, tol, r, ct.byref(res))
     return r, it, res.value
+
+
+class PullPlan:
+    """The threaded FP64 pull PageRank with its transposed graph built once:
+    run() repeats a solve from r0 = 1/n and returns (r, iterations,
+    last residual, seconds of the iteration loop)."""
+
+    def __init__(self, rp, col, nthreads):
+        self.rp = np.ascontiguousarray(rp, dtype=np.uint32)
+        self.col = np.ascontiguousarray(col, dtype=np.uint32)
+        self.n = len(self.rp) - 1
+        self.h = lib().orc_pull_plan_build(self.n, self.rp, self.col, nthreads)
+
+    @property
+    def arcs(self):
+        return int(lib().orc_pull_plan_arcs(self.h))
+
+    def run(self, iters=0, tol=1e-6, damping=0.85):
+        r = np.zeros(self.n, dtype=np.float64)
+        res, sec = ct.c_double(0.0), ct.c_double(0.0)
+        it = lib().orc_pull_plan_run(self.h, damping, iters, tol, r, ct.byref(res), ct.byref(sec))
+        return r, it, res.value, sec.value
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().orc_pull_plan_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.free()
+
+
+def cc_unionfind_mt(rp, col, nthreads):
+    rp = np.ascontiguousarray(rp, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    out = np.zeros(len(rp) - 1, dtype=np.uint32)
+    lib().orc_cc_unionfind_mt(len(rp) - 1, rp, col, out, nthreads)
+    return out
+
+
+def sssp_violations(rp, col, w, dist, src, nthreads):
+    """Violated Bellman conditions of dist from src (0 = exact distances)."""
+    rp = np.ascontiguousarray(rp, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    dist = np.ascontiguousarray(dist, dtype=np.uint32)
+    wp = None
+    if w is not None:
+        w = np.ascontiguousarray(w, dtype=np.uint32)
+        wp = w.ctypes.data
+    return int(lib().orc_sssp_check_mt(len(rp) - 1, rp, col, wp, dist, src, nthreads))

@@ -231,3 +231,58 @@ def test_hash_known_answers():
     # published first output of SplitMix64 with seed 0
     assert mix(0) == 0xE220A8397B1DCDAF
     assert O.lib().orc_hash(0, mix(0)) == mix(0)  # mix(mix(0) ^ mix(0)) = mix(0)
+
+
+# ---- the full-size checks of bench.py, pinned here ----
+
+def test_pull_plan_equals_threaded_pull():
+    """The reusable pull plan is the same recurrence, summed in the same
+    order, as orc_pr_pull_mt (itself pinned by networkx above), and its
+    iteration loop is timed."""
+    g = O.rmat(12, 16, seed=7)
+    P = O.PullPlan(g.rp, g.col, 4)
+    assert P.arcs == g.m
+    r, it, res, sec = P.run(iters=0, tol=1e-8)
+    r2, it2, res2 = O.pr_pull(g, iters=0, tol=1e-8, nthreads=3)
+    assert it == it2 and np.array_equal(r, r2) and res == res2
+    assert sec > 0
+    r3, it3, _, _ = P.run(iters=5)
+    assert it3 == 5 and np.array_equal(r3, O.pr_pull(g, iters=5, nthreads=2)[0])
+    P.free()
+
+
+@pytest.mark.parametrize("scale,sym", [(10, True), (13, True), (13, False)])
+def test_parallel_unionfind_equals_sequential(scale, sym):
+    g = O.rmat(scale, 8, seed=scale, symmetrize=sym)
+    ref = O.cc_unionfind(g)
+    for nt in (1, 4, 8):
+        assert np.array_equal(O.cc_unionfind_mt(g.rp, g.col, nt), ref)
+
+
+def test_parallel_unionfind_min_gid_by_hand():
+    # components {0, 3}, {1, 2, 4}, {5}
+    g = O.from_edges(6, [(3, 0), (4, 2), (2, 1), (1, 4)])
+    assert O.cc_unionfind_mt(g.rp, g.col, 2).tolist() == [0, 1, 1, 0, 1, 5]
+
+
+def test_sssp_check_accepts_exact_and_rejects_planted_errors():
+    g = O.rmat(12, 8, seed=3, symmetrize=True).with_weights(2)
+    src = int(np.argmax(g.out_degree()))
+    d = O.dijkstra(g, src)
+    assert O.sssp_violations(g.rp, g.col, g.w, d, src, 4) == 0
+    hop = O.bfs(g, src)
+    assert O.sssp_violations(g.rp, g.col, None, hop, src, 4) == 0
+    reached = np.flatnonzero((d != 0xFFFFFFFF) & (np.arange(g.n) != src))
+    v = int(reached[len(reached) // 2])
+    for delta in (-1, 1):  # too short (no tight arc) and too long (an arc relaxes it)
+        bad = d.copy()
+        bad[v] = int(bad[v]) + delta
+        assert O.sssp_violations(g.rp, g.col, g.w, bad, src, 4) > 0, delta
+    bad = d.copy()
+    bad[v] = 0xFFFFFFFF  # reachable vertex reported unreached
+    assert O.sssp_violations(g.rp, g.col, g.w, bad, src, 4) > 0
+    bad = d.copy()
+    bad[src] = 1
+    assert O.sssp_violations(g.rp, g.col, g.w, bad, src, 4) > 0
+    # the wrong weights (hop distances against weighted arcs) are refused
+    assert O.sssp_violations(g.rp, g.col, g.w, hop, src, 4) > 0
