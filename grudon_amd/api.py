@@ -10,7 +10,9 @@ import weakref
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libgrudon_hip.so")
+# GRUDON_HIP_LIB selects another build of the same library (the sanitizer
+# build of grudon_amd/build.py --sanitize runs the CPU tests against one)
+LIB_PATH = os.environ.get("GRUDON_HIP_LIB") or os.path.join(PKG, "libgrudon_hip.so")
 
 PR, CC, SSSP, PR_PULL = 0, 1, 2, 3
 OPT_PUSH_SPARSE, OPT_PULL_SPARSE, OPT_PUSH_DENSE, OPT_PULL_DENSE, OPT_SHIP_EDGES = 0, 1, 2, 3, 4
@@ -242,6 +244,7 @@ class Graph:
     def __init__(self, ctx, h):
         self.ctx = ctx
         self.h = h
+        self._states = []  # weak references: a graph frees its states first
         ctx._track(self)
         info = np.zeros(16, dtype=np.uint64)
         _check(lib().grd_graph_info(self.h, _ptr(info)), "grd_graph_info")
@@ -308,6 +311,11 @@ class Graph:
 
     def free(self):
         if self.h:
+            for ref in self._states:
+                st = ref()
+                if st is not None:
+                    st.free()
+            self._states = []
             lib().grd_graph_free(self.h)
             self.h = None
 
@@ -328,6 +336,7 @@ class State:
         h = ct.c_void_p()
         _check(lib().grd_state_create_typed(graph.h, algo, value_bytes, ct.byref(h)), "grd_state_create")
         self.h = h
+        graph._states.append(weakref.ref(self))
         graph.ctx._track(self)
 
     def _vdtype(self):

@@ -2,6 +2,7 @@
 
     python -m grudon_amd.build            # everything
     python -m grudon_amd.build --no-oracle
+    python -m grudon_amd.build --sanitize # host code under ASan + UBSan, then the CPU tests
 
 Outputs (in-tree, git-ignored, shipped with the working tree):
     grudon_amd/libgrudon_hip.so      HIP kernels + C-ABI (include/grudon_hip.h)
@@ -107,6 +108,63 @@ def build_oracle():
     return out
 
 
+CLANG = "/opt/rocm/llvm/bin/clang"
+CLANGXX = "/opt/rocm/llvm/bin/clang++"
+SAN = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-fno-sanitize-recover=undefined", "-g", "-O1"]
+
+
+def asan_runtime():
+    out = subprocess.run([CLANG, "-print-file-name=libclang_rt.asan-x86_64.so"], stdout=subprocess.PIPE,
+                         text=True).stdout.strip()
+    if not os.path.exists(out):
+        import glob
+        hits = glob.glob("/opt/rocm/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so")
+        out = hits[0] if hits else ""
+    return out
+
+
+def build_sanitized(jobs=8):
+    """Host code under AddressSanitizer + UndefinedBehaviorSanitizer (CPU
+    only): the library's host translation units (C-ABI, graph readers, host
+    layout build) compiled by clang++ with the sanitizers and linked with the
+    unchanged device objects, and the oracle.  Outputs in _build/san/."""
+    sdir = os.path.join(BUILD, "san")
+    os.makedirs(sdir, exist_ok=True)
+    hip_objs = [os.path.join(BUILD, f + ".o") for f in sorted(os.listdir(CSRC)) if f.endswith(".hip")]
+    if not all(os.path.exists(o) for o in hip_objs):
+        build_library(jobs)
+    cpp_srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".cpp"))
+    inc = ["-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", "-std=c++17", "-fPIC"]
+
+    def cc(src):
+        obj = os.path.join(sdir, os.path.basename(src) + ".o")
+        _run([CLANGXX] + SAN + inc + ["-c", src, "-o", obj])
+        return obj
+
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(cc, cpp_srcs))
+    lib = os.path.join(sdir, "libgrudon_hip.so")
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-shared-libasan", "-Xarch_host",
+          "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined", "-o", lib] + objs + hip_objs + ["-lpthread"])
+    orc = os.path.join(sdir, "liboracle.so")
+    _run([CLANG] + SAN + ["-std=c11", "-fPIC", "-shared", "-shared-libasan", "-pthread",
+                          os.path.join(ROOT, "oracle", "grudon_oracle.c"), "-o", orc, "-lm"])
+    return lib, orc
+
+
+def run_sanitized_cpu_tests():
+    """pytest -m 'not gpu' against the sanitizer builds, the ASan runtime
+    preloaded into the interpreter; returns pytest's exit status."""
+    lib, orc = build_sanitized()
+    env = dict(os.environ)
+    env.update({"LD_PRELOAD": asan_runtime(), "GRUDON_HIP_LIB": lib, "GRUDON_ORACLE_LIB": orc,
+                "ASAN_OPTIONS": "detect_leaks=0:abort_on_error=1",
+                "UBSAN_OPTIONS": "print_stacktrace=1:halt_on_error=1"})
+    p = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-m", "not gpu", "-p", "no:cacheprovider",
+                        os.path.join(ROOT, "tests")], env=env, cwd=ROOT)
+    return p.returncode
+
+
 def build_all(oracle=True):
     outs = [build_library(), build_probe()]
     h = build_host()
@@ -118,5 +176,7 @@ def build_all(oracle=True):
 
 
 if __name__ == "__main__":
+    if "--sanitize" in sys.argv:
+        sys.exit(run_sanitized_cpu_tests())
     for o in build_all(oracle="--no-oracle" not in sys.argv):
         print("built", o)

@@ -55,6 +55,8 @@ struct grd_batch {
   grd::BatchBlock *blocks = nullptr;
   grd::BatchLong *longs = nullptr;
   grd::BatchTile *tiles = nullptr;
+  uint32_t *gtile = nullptr;  // [count+1] first vertex tile of each graph
+  double *tred = nullptr;     // [2 * ntile] (residual, dangling mass) of each vertex tile
   uint32_t nblk = 0, nlong = 0, ntile = 0;
   // per-graph state
   grd::DevMem st_mem;
@@ -62,7 +64,7 @@ struct grd_batch {
   float2 *partial = nullptr;
   uint32_t *lab[2] = {nullptr, nullptr};
   uint32_t *active = nullptr, *iters = nullptr, *changed = nullptr, *alive = nullptr;
-  double *residual = nullptr, *dcur = nullptr, *dnext = nullptr;
+  double *residual = nullptr, *dcur = nullptr;
   uint32_t *h_alive = nullptr;  // pinned
   int last_algo = -1;
   std::vector<uint32_t> last_iters;
@@ -155,12 +157,13 @@ __global__ void __launch_bounds__(kBlock) k_bpr_fixup(const BatchLong *__restric
 }
 
 // vertex tiles lie within one graph: residual and dangling mass are reduced
-// per tile and added to the graph's FP64 accumulators
+// per tile into the tile's own FP64 slot (no atomics), summed per graph in
+// tile order by k_bpr_finish, so convergence does not depend on scheduling
 __global__ void __launch_bounds__(kBlock) k_bpr_update(const BatchTile *__restrict__ tiles, uint32_t ntile,
                                                        const uint64_t *__restrict__ vb, const uint32_t *__restrict__ deg,
                                                        const float *__restrict__ acc, float *__restrict__ r,
                                                        float *__restrict__ contrib, const uint32_t *__restrict__ active,
-                                                       const double *__restrict__ dcur, double *residual, double *dnext,
+                                                       const double *__restrict__ dcur, double *__restrict__ tred,
                                                        float damping)
 {
   for (uint32_t t = blockIdx.x; t < ntile; t += gridDim.x) {
@@ -180,23 +183,28 @@ __global__ void __launch_bounds__(kBlock) k_bpr_update(const BatchTile *__restri
     res = block_sum(res);
     dang = block_sum(dang);
     if (threadIdx.x == 0) {
-      atomicAdd(residual + T.graph, res);
-      if (dang != 0.0) atomicAdd(dnext + T.graph, dang);
+      tred[2 * t] = res;
+      tred[2 * t + 1] = dang;
     }
   }
 }
 
-__global__ void k_bpr_finish(uint32_t count, uint32_t *active, uint32_t *iters, double *residual, double *dcur,
-                             double *dnext, double tolerance, uint32_t max_iter, uint32_t *alive)
+__global__ void k_bpr_finish(uint32_t count, uint32_t *active, uint32_t *iters, const uint32_t *__restrict__ gtile,
+                             const double *__restrict__ tred, double *residual, double *dcur, double tolerance,
+                             uint32_t max_iter, uint32_t *alive)
 {
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < count; g += gridDim.x * blockDim.x) {
     if (!active[g]) continue;
+    double res = 0.0, dang = 0.0;
+    for (uint32_t t = gtile[g]; t < gtile[g + 1]; t++) {
+      res += tred[2 * t];
+      dang += tred[2 * t + 1];
+    }
     uint32_t it = ++iters[g];
-    if (residual[g] < tolerance || it >= max_iter) active[g] = 0;
+    residual[g] = res;
+    if (res < tolerance || it >= max_iter) active[g] = 0;
     else atomicAdd(alive, 1u);
-    dcur[g] = dnext[g];
-    dnext[g] = 0.0;
-    residual[g] = 0.0;
+    dcur[g] = dang;
   }
 }
 
@@ -332,8 +340,10 @@ static grd_status batch_build_blocks(grd_batch *b)
   std::vector<BatchBlock> blks;
   std::vector<BatchLong> longs;
   std::vector<BatchTile> tiles;
+  std::vector<uint32_t> gtile(b->count + 1, 0);
   const uint32_t T = b->tile;
   for (uint32_t g = 0; g < b->count; g++) {
+    gtile[g] = (uint32_t)tiles.size();
     const uint64_t v0 = b->vbase[g], v1 = b->vbase[g + 1];
     uint64_t r = v0;
     while (r < v1) {
@@ -356,6 +366,7 @@ static grd_status batch_build_blocks(grd_batch *b)
     for (uint64_t v = v0; v < v1; v += 4096)
       tiles.push_back(BatchTile{(uint32_t)v, (uint32_t)std::min<uint64_t>(v1, v + 4096), g, 0});
   }
+  gtile[b->count] = (uint32_t)tiles.size();
   b->blk_mem.release();
   b->nblk = (uint32_t)blks.size();
   b->nlong = (uint32_t)longs.size();
@@ -365,6 +376,9 @@ static grd_status batch_build_blocks(grd_batch *b)
   GRD_TRY(b->blk_mem.alloc_n(&b->longs, longs.size()));
   GRD_TRY(b->blk_mem.alloc_n(&b->tiles, std::max<size_t>(1, tiles.size())));
   GRD_TRY(b->blk_mem.alloc(reinterpret_cast<void **>(&b->partial), (uint64_t)std::max<uint32_t>(1, b->nblk) * 8));
+  GRD_TRY(b->blk_mem.alloc_n(&b->gtile, gtile.size()));
+  GRD_TRY(b->blk_mem.alloc_n(&b->tred, 2ull * std::max<size_t>(1, tiles.size())));
+  GRD_HIP(hipMemcpy(b->gtile, gtile.data(), gtile.size() * 4, hipMemcpyHostToDevice));
   GRD_HIP(hipMemcpy(b->blocks, blks.data(), blks.size() * sizeof(BatchBlock), hipMemcpyHostToDevice));
   GRD_HIP(hipMemcpy(b->longs, longs.data(), longs.size() * sizeof(BatchLong), hipMemcpyHostToDevice));
   GRD_HIP(hipMemcpy(b->tiles, tiles.data(), tiles.size() * sizeof(BatchTile), hipMemcpyHostToDevice));
@@ -427,10 +441,10 @@ static grd_status capture(grd_batch *b, int algo, double damping, double tol, ui
       hipLaunchKernelGGL(k_bpr_fixup, dim3(std::max<uint32_t>(1, (b->nlong + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                          st, b->longs, b->nlong, b->partial, b->acc, b->active);
       hipLaunchKernelGGL(k_bpr_update, dim3(std::min<uint32_t>(b->ntile, persistent_grid(b))), dim3(kBlock), 0, st,
-                         b->tiles, b->ntile, b->vb, b->deg, b->acc, b->r, b->contrib, b->active, b->dcur, b->residual,
-                         b->dnext, (float)damping);
+                         b->tiles, b->ntile, b->vb, b->deg, b->acc, b->r, b->contrib, b->active, b->dcur, b->tred,
+                         (float)damping);
       hipLaunchKernelGGL(k_bpr_finish, dim3((b->count + 255) / 256), dim3(256), 0, st, b->count, b->active, b->iters,
-                         b->residual, b->dcur, b->dnext, tol, max_iter, b->alive);
+                         b->gtile, b->tred, b->residual, b->dcur, tol, max_iter, b->alive);
     } else {
       launch_bcc_pull(b, st, (int)(i & 1));
       hipLaunchKernelGGL(k_bcc_finish, dim3((b->count + 255) / 256), dim3(256), 0, st, b->count, b->active, b->iters,
@@ -455,7 +469,6 @@ static grd_status batch_run(grd_batch *b, int algo, double damping, double tol, 
   hipLaunchKernelGGL(k_bfill_u32, dim3((cnt + 255) / 256), dim3(256), 0, st, b->iters, cnt, 0u);
   hipLaunchKernelGGL(k_bfill_u32, dim3((cnt + 255) / 256), dim3(256), 0, st, b->changed, cnt, 0u);
   hipLaunchKernelGGL(k_bfill_f64, dim3((cnt + 255) / 256), dim3(256), 0, st, b->residual, cnt, 0.0);
-  hipLaunchKernelGGL(k_bfill_f64, dim3((cnt + 255) / 256), dim3(256), 0, st, b->dnext, cnt, 0.0);
   const uint32_t ig = (uint32_t)std::min<uint64_t>(4096, b->N / kBlock + 1);
   if (algo == GRD_PR_PULL) {
     hipLaunchKernelGGL(k_bpr_init, dim3(ig), dim3(kBlock), 0, st, b->N, b->vb, cnt, b->changed + cnt, b->deg, b->r,
@@ -585,8 +598,7 @@ grd_status grd_batch_create(grd_ctx *ctx, uint32_t count, const uint32_t *n, con
         (st = sm.alloc_n(&b->lab[0], b->N)) != GRD_OK || (st = sm.alloc_n(&b->lab[1], b->N)) != GRD_OK ||
         (st = sm.alloc_n(&b->active, count)) != GRD_OK || (st = sm.alloc_n(&b->iters, count)) != GRD_OK ||
         (st = sm.alloc_n(&b->changed, count + b->N)) != GRD_OK || (st = sm.alloc_n(&b->alive, 1)) != GRD_OK ||
-        (st = sm.alloc_n(&b->residual, count)) != GRD_OK || (st = sm.alloc_n(&b->dcur, count)) != GRD_OK ||
-        (st = sm.alloc_n(&b->dnext, count)) != GRD_OK) {
+        (st = sm.alloc_n(&b->residual, count)) != GRD_OK || (st = sm.alloc_n(&b->dcur, count)) != GRD_OK) {
       delete b;
       return st;
     }

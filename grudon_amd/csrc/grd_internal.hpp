@@ -177,10 +177,9 @@ struct grd_graph {
   uint32_t tile_arcs = 0;
   uint64_t n_pairs = 0;
   uint16_t *tcol = nullptr;        // [m] source offset within the pair's tier window
-  uint32_t *tptr = nullptr;        // [pairs+1] arcs of each pair, pairs in tier order
-  uint32_t *trow = nullptr;        // [pairs] proxy row of the pair | 1 << 31 when the row has one pair
+  uint2 *trec = nullptr;           // [pairs+1] pairs in tier order: x = first arc, y = destination
+                                   // (row | 1 << 31 for a row's only pair, else its (row, tier)-order index)
   uint32_t *trptr = nullptr;       // [P+1] pairs of each proxy row, in (row, tier) order
-  uint32_t *tinv = nullptr;        // [pairs] tier-order index of the pairs in (row, tier) order
   grd::PullBlock *tblk = nullptr;  // row blocks over pairs (r0 = first pair); nrows 0 = segment of a long pair
   uint32_t n_tblk = 0;
   grd::PullLong *tlong = nullptr;  // long pairs (row = pair index)

@@ -14,11 +14,10 @@
 //   tcol[e]      = the source's 16-bit offset within its tier
 //   tile         = a run of row blocks of one (partition, tier) group
 //
-// A 512-thread workgroup takes a tile, stages the tier's window of mprop
-// (<= 128 KB) into LDS with coalesced loads, then its two 256-thread halves
-// walk the tile's row blocks like the plain pull (8 coalesced column loads
-// and 8 LDS gathers per thread, a group of lanes per row, FP32 Kahan per lane
-// merged by TwoSum).  Each pair yields one compensated partial; a row with a
+// A 1024-thread workgroup takes a tile and stages the tier's window of mprop
+// (<= 128 KB) into LDS with coalesced 16-byte loads; then each wave walks its
+// own row blocks (4 coalesced column loads and 4 LDS gathers per lane, a
+// group of lanes per pair, FP32 Kahan per lane merged by TwoSum).  Each pair yields one compensated partial; a row with a
 // single pair writes its value straight into the reduce buffer, the others
 // are merged per row in tier order (k_prp_tier_merge), so the result does not
 // depend on scheduling.  A pair longer than one block is cut into segments
@@ -39,7 +38,10 @@ namespace grd {
 
 namespace {
 
-constexpr int kTierBlock = 512;  // threads per tile workgroup: two halves of kBlock
+constexpr int kTierBlock = 1024;                  // threads per tile workgroup
+constexpr int kTierWaves = kTierBlock / kWave;    // waves, each walking its own row blocks
+constexpr uint32_t kTNnz = 256;                   // arcs per row block (4 per lane)
+constexpr uint32_t kTRows = 256;                  // pairs per row block
 constexpr uint32_t kSole = 0x80000000u;
 
 inline uint32_t tgrid(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(65535, (n + kBlock - 1) / kBlock)); }
@@ -108,10 +110,13 @@ __global__ void k_t_rptr(uint32_t P, const uint32_t *__restrict__ crp, const uin
   TIER_LOOP(r, (uint64_t)P + 1) trptr[r] = pid[crp[r]];
 }
 
-// per pair in tier order: length, row record, and the inverse permutation
+// per pair in tier order: length, destination and the inverse permutation.
+// The destination of a row's only pair is the row (its value goes straight
+// to the reduce buffer); a pair of a row with several is written to its
+// (row, tier)-order index, so that each row's partials are contiguous.
 __global__ void k_t_order(uint64_t np, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ prow,
                           const uint32_t *__restrict__ pe0, const uint32_t *__restrict__ crp,
-                          const uint32_t *__restrict__ trptr, uint32_t *__restrict__ tlen, uint32_t *__restrict__ trow,
+                          const uint32_t *__restrict__ trptr, uint32_t *__restrict__ tlen, uint2 *__restrict__ trec,
                           uint32_t *__restrict__ tinv)
 {
   TIER_LOOP(i, np)
@@ -120,9 +125,14 @@ __global__ void k_t_order(uint64_t np, const uint32_t *__restrict__ perm, const 
     uint32_t end = crp[r + 1];
     if (p + 1 < np && pe0[p + 1] < end) end = pe0[p + 1];
     tlen[i] = end - pe0[p];
-    trow[i] = r | ((trptr[r + 1] - trptr[r] == 1) ? kSole : 0u);
+    trec[i].y = (trptr[r + 1] - trptr[r] == 1) ? (r | kSole) : p;
     tinv[p] = (uint32_t)i;
   }
+}
+
+__global__ void k_t_starts(uint64_t np, const uint32_t *__restrict__ tptr, uint2 *__restrict__ trec)
+{
+  TIER_LOOP(i, np + 1) trec[i].x = tptr[i];
 }
 
 // every arc to its place in tier order, as an offset within its tier
@@ -162,92 +172,139 @@ grd_status t_upload(DevMem &mem, hipStream_t st, T **dst, const std::vector<T> &
 
 // ---- the tiled pull ----
 
+// the loads of one row block, issued one block ahead of its use
+struct WaveBlock {
+  PullBlock d;
+  uint32_t G, nr;
+  uint32_t c[kTNnz / kWave];
+  uint32_t ps[kTRows / kWave], pe[kTRows / kWave], pd[kTRows / kWave];
+};
+
+__device__ __forceinline__ void wave_block_load(WaveBlock &x, const PullBlock &d, uint32_t lane,
+                                                const uint16_t *__restrict__ tcol, const uint2 *__restrict__ trec)
+{
+  x.d = d;
+  x.G = 64;
+  x.nr = 0;
+  if (d.nrows == 0) return;  // empty slot, or a segment of a long pair (read where it is used)
+  while (x.G > 1 && x.G * d.nrows > (uint32_t)kWave) x.G >>= 1;
+  const uint32_t ngroups = kWave / x.G;
+  x.nr = (d.nrows + ngroups - 1) / ngroups;
+  const uint32_t ne = d.e1 - d.e0;
+#pragma unroll
+  for (int k = 0; k < (int)(kTNnz / kWave); k++) {
+    const uint32_t i = lane + k * kWave;
+    x.c[k] = i < ne ? tcol[d.e0 + i] : 0u;
+  }
+  const uint32_t *starts = reinterpret_cast<const uint32_t *>(trec);
+#pragma unroll
+  for (int k = 0; k < (int)(kTRows / kWave); k++) {
+    const uint32_t ql = k * ngroups + lane / x.G;
+    const bool ok = (uint32_t)k < x.nr && ql < d.nrows;
+    const uint2 r = ok ? trec[d.r0 + ql] : make_uint2(0u, 0u);
+    x.ps[k] = r.x;
+    x.pd[k] = r.y;
+    x.pe[k] = ok ? starts[2 * (d.r0 + ql + 1)] : 0u;
+  }
+}
+
+__device__ __forceinline__ void wave_fence()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One 1024-thread workgroup per tile: the tier window is staged with 16-byte
+// loads, all issued before the first LDS store, behind one barrier.  Then
+// each of the 16 waves walks its own share of the tile's row blocks (at most
+// kTNnz arcs and kTRows pairs each) with no workgroup barrier, the loads of
+// its next block in flight while it gathers the current block's arcs from
+// the window into its own slice of s_v and groups of G lanes sum the pairs.
 __global__ void __launch_bounds__(kTierBlock) k_prp_pull_tiled(const TierTile *__restrict__ tiles,
                                                                const PullBlock *__restrict__ blocks,
-                                                               const uint32_t *__restrict__ tptr,
-                                                               const uint32_t *__restrict__ trow,
+                                                               const uint2 *__restrict__ trec,
                                                                const uint16_t *__restrict__ tcol,
                                                                const float *__restrict__ mprop,
                                                                float *__restrict__ out, float2 *__restrict__ part,
                                                                float2 *__restrict__ seg)
 {
-  __shared__ float s_win[1u << kTierBitsMax];
-  __shared__ float s_v[2][kPullNnz];
-  __shared__ float s_ws[2][kWavesPerBlock], s_wc[2][kWavesPerBlock];
+  __shared__ float4 s_win4[(1u << kTierBitsMax) / 4];
+  __shared__ float s_v[kTierWaves][kTNnz];
   const TierTile T = tiles[blockIdx.x];
-  for (uint32_t i = threadIdx.x; i < T.wlen; i += kTierBlock) s_win[i] = mprop[T.wbase + i];
+  {
+    // wlen is a multiple of 16 and wbase of 64: whole, aligned float4s
+    const float4 *__restrict__ src = reinterpret_cast<const float4 *>(mprop + T.wbase);
+    const uint32_t n4 = T.wlen >> 2;
+    constexpr int kStage = (1u << kTierBitsMax) / 4 / kTierBlock;
+    float4 x[kStage];
+#pragma unroll
+    for (int k = 0; k < kStage; k++) {
+      const uint32_t i = threadIdx.x + k * kTierBlock;
+      if (i < n4) x[k] = src[i];
+    }
+#pragma unroll
+    for (int k = 0; k < kStage; k++) {
+      const uint32_t i = threadIdx.x + k * kTierBlock;
+      if (i < n4) s_win4[i] = x[k];
+    }
+  }
   __syncthreads();
-  const uint32_t half = threadIdx.x / kBlock, lt = threadIdx.x % kBlock;
-  const uint32_t rounds = (T.nb + 1) / 2;
-  for (uint32_t rd = 0; rd < rounds; rd++) {
-    const uint32_t bi = 2 * rd + half;
-    const bool valid = bi < T.nb;
-    const PullBlock d = valid ? blocks[T.b0 + bi] : PullBlock{0, 0, 0, 0};
-    const uint32_t ne = d.e1 - d.e0;
-    if (valid && d.nrows == 0) {
+  const float *s_win = reinterpret_cast<const float *>(s_win4);
+  const uint32_t w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  float *sv = s_v[w];
+  const PullBlock none{0, 0, 0, 0};
+  uint32_t bi = w;
+  WaveBlock cur;
+  wave_block_load(cur, bi < T.nb ? blocks[T.b0 + bi] : none, lane, tcol, trec);
+  PullBlock dnext = bi + kTierWaves < T.nb ? blocks[T.b0 + bi + kTierWaves] : none;
+  for (; bi < T.nb; bi += kTierWaves) {
+    const uint32_t bn = bi + kTierWaves;
+    const PullBlock dafter = bn + kTierWaves < T.nb ? blocks[T.b0 + bn + kTierWaves] : none;
+    WaveBlock nxt;
+    wave_block_load(nxt, dnext, lane, tcol, trec);
+    const PullBlock &d = cur.d;
+    if (d.nrows == 0) {
       // one segment of a long pair: a compensated sum of its arcs
       Kahan k;
-      for (uint32_t e = d.e0 + lt; e < d.e1; e += kBlock) k.add(s_win[tcol[e]]);
+      for (uint32_t e = d.e0 + lane; e < d.e1; e += kWave) k.add(s_win[tcol[e]]);
       kahan_wave_merge(k, kWave);
-      if (lane_id() == 0) {
-        s_ws[half][lt / kWave] = k.s;
-        s_wc[half][lt / kWave] = k.c;
-      }
-    } else if (valid) {
-      constexpr int kPer = kPullNnz / kBlock;
-      uint32_t c[kPer];
+      if (lane == 0) seg[T.b0 + bi] = make_float2(k.s, k.c);
+    } else {
+      const uint32_t ne = d.e1 - d.e0;
 #pragma unroll
-      for (int k = 0; k < kPer; k++) {
-        const uint32_t i = lt + k * kBlock;
-        c[k] = i < ne ? tcol[d.e0 + i] : 0u;
+      for (int k = 0; k < (int)(kTNnz / kWave); k++) {
+        const uint32_t i = lane + k * kWave;
+        if (i < ne) sv[i] = s_win[cur.c[k]];
       }
+      wave_fence();
+      const uint32_t G = cur.G, sub = lane % G, ngroups = kWave / G;
 #pragma unroll
-      for (int k = 0; k < kPer; k++) {
-        const uint32_t i = lt + k * kBlock;
-        if (i < ne) s_v[half][i] = s_win[c[k]];
-      }
-    }
-    __syncthreads();
-    if (valid && d.nrows == 0) {
-      if (lt == 0) {
-        Kahan t;
-        t.s = s_ws[half][0];
-        t.c = s_wc[half][0];
-        for (int j = 1; j < kWavesPerBlock; j++) t.merge(s_ws[half][j], s_wc[half][j]);
-        seg[T.b0 + bi] = make_float2(t.s, t.c);
-      }
-    } else if (valid) {
-      // the pairs of the block, a group of G lanes each; every group runs
-      // the same number of rounds so that the group shuffles stay in step
-      int G = 64;
-      while (G > 1 && (uint32_t)G * d.nrows > (uint32_t)kBlock) G >>= 1;
-      const uint32_t sub = lt % G, ngroups = kBlock / G;
-      const uint32_t nr = (d.nrows + ngroups - 1) / ngroups;
-      for (uint32_t k = 0; k < nr; k++) {
-        const uint32_t ql = k * ngroups + lt / G;
-        const uint32_t q = d.r0 + ql;
+      for (int k = 0; k < (int)(kTRows / kWave); k++) {
+        if ((uint32_t)k >= cur.nr) break;  // uniform over the wave
+        const uint32_t ql = k * ngroups + lane / G;
         Kahan acc;
-        if (ql < d.nrows) {
-          const uint32_t b = tptr[q] - d.e0, e = tptr[q + 1] - d.e0;
-          for (uint32_t x = b + sub; x < e; x += G) acc.add(s_v[half][x]);
-        }
-        kahan_wave_merge(acc, G);
+        if (ql < d.nrows)
+          for (uint32_t x = cur.ps[k] - d.e0 + sub; x < cur.pe[k] - d.e0; x += G) acc.add(sv[x]);
+        kahan_wave_merge(acc, (int)G);
         if (sub == 0 && ql < d.nrows) {
-          const uint32_t rw = trow[q];
-          if (rw & kSole) out[rw & ~kSole] = acc.value();
-          else part[q] = make_float2(acc.s, acc.c);
+          const uint32_t dst = cur.pd[k];
+          if (dst & kSole) out[dst & ~kSole] = acc.value();
+          else part[dst] = make_float2(acc.s, acc.c);
         }
       }
+      // every lane has read the slice before the next block writes it
+      wave_fence();
     }
-    __syncthreads();
+    cur = nxt;
+    dnext = dafter;
   }
 }
 
 // long pairs: their segments merged in order
 __global__ void __launch_bounds__(kBlock) k_prp_tier_fixup(const PullLong *__restrict__ longs, uint32_t nlong,
-                                                           const float2 *__restrict__ seg,
-                                                           const uint32_t *__restrict__ trow, float *__restrict__ out,
-                                                           float2 *__restrict__ part)
+                                                           const float2 *__restrict__ seg, const uint2 *__restrict__ trec,
+                                                           float *__restrict__ out, float2 *__restrict__ part)
 {
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nlong; i += gridDim.x * kBlock) {
     const PullLong L = longs[i];
@@ -256,15 +313,14 @@ __global__ void __launch_bounds__(kBlock) k_prp_tier_fixup(const PullLong *__res
       const float2 p = seg[L.first + j];
       t.merge(p.x, p.y);
     }
-    const uint32_t rw = trow[L.row];
-    if (rw & kSole) out[rw & ~kSole] = t.value();
-    else part[L.row] = make_float2(t.s, t.c);
+    const uint32_t dst = trec[L.row].y;
+    if (dst & kSole) out[dst & ~kSole] = t.value();
+    else part[dst] = make_float2(t.s, t.c);
   }
 }
 
-// rows with no pair (0) or several (their partials in tier order)
+// rows with no pair (0) or several: their partials, contiguous in tier order
 __global__ void __launch_bounds__(kBlock) k_prp_tier_merge(uint32_t P, const uint32_t *__restrict__ trptr,
-                                                           const uint32_t *__restrict__ tinv,
                                                            const float2 *__restrict__ part, float *__restrict__ out)
 {
   for (uint32_t r = blockIdx.x * kBlock + threadIdx.x; r < P; r += gridDim.x * kBlock) {
@@ -272,7 +328,7 @@ __global__ void __launch_bounds__(kBlock) k_prp_tier_merge(uint32_t P, const uin
     if (b - a == 1) continue;
     Kahan t;
     for (uint32_t x = a; x < b; x++) {
-      const float2 p = part[tinv[x]];
+      const float2 p = part[x];
       t.merge(p.x, p.y);
     }
     out[r] = t.value();
@@ -354,23 +410,24 @@ grd_status build_tiers(grd_graph *g)
     pkey = kb.current();
     pval = vb.current();
   }
-  uint32_t *tlen = nullptr;
+  uint32_t *tlen = nullptr, *tinv = nullptr, *d_tptr = nullptr;
   GRD_TRY(tmp.alloc_n(&tlen, np + 1));
-  GRD_TRY(gm.alloc_n(&g->trow, std::max<uint64_t>(np, 1)));
-  GRD_TRY(gm.alloc_n(&g->tinv, std::max<uint64_t>(np, 1)));
-  GRD_TRY(gm.alloc_n(&g->tptr, np + 1));
+  GRD_TRY(tmp.alloc_n(&tinv, std::max<uint64_t>(np, 1)));
+  GRD_TRY(tmp.alloc_n(&d_tptr, np + 1));
+  GRD_TRY(gm.alloc_n(&g->trec, np + 1));
   GRD_TRY(gm.alloc_n(&g->tcol, std::max<uint64_t>(m, 1)));
   hipLaunchKernelGGL(k_t_fill, dim3(1), dim3(kBlock), 0, st, tlen + np, 1, 0u);
   hipLaunchKernelGGL(k_t_order, dim3(tgrid(np)), dim3(kBlock), 0, st, np, pval, prow, pe0, g->crp, g->trptr, tlen,
-                     g->trow, g->tinv);
+                     g->trec, tinv);
   GRD_HIP(hipGetLastError());
-  GRD_TRY(t_scan(tmp, st, tlen, g->tptr, np + 1));
-  hipLaunchKernelGGL(k_t_copy, dim3(tgrid(m)), dim3(kBlock), 0, st, m, flag, pid, pe0, prow, g->tinv, g->tptr, g->ccol,
+  GRD_TRY(t_scan(tmp, st, tlen, d_tptr, np + 1));
+  hipLaunchKernelGGL(k_t_starts, dim3(tgrid(np + 1)), dim3(kBlock), 0, st, np, d_tptr, g->trec);
+  hipLaunchKernelGGL(k_t_copy, dim3(tgrid(m)), dim3(kBlock), 0, st, m, flag, pid, pe0, prow, tinv, d_tptr, g->ccol,
                      d_pb, M, tb, g->tcol);
   GRD_HIP(hipGetLastError());
   // host: row blocks within each (partition, tier) group, then tiles
   std::vector<uint32_t> tptr(np + 1), key(np);
-  GRD_HIP(hipMemcpyAsync(tptr.data(), g->tptr, (np + 1) * 4, hipMemcpyDeviceToHost, st));
+  GRD_HIP(hipMemcpyAsync(tptr.data(), d_tptr, (np + 1) * 4, hipMemcpyDeviceToHost, st));
   if (np) GRD_HIP(hipMemcpyAsync(key.data(), pkey, np * 4, hipMemcpyDeviceToHost, st));
   GRD_HIP(hipStreamSynchronize(st));
   std::vector<PullBlock> blks;
@@ -395,10 +452,10 @@ grd_status build_tiers(grd_graph *g)
       const uint32_t len = tptr[q + 1] - tptr[q];
       uint64_t add_arcs = 0;
       const size_t before = blks.size();
-      if (len > kPullNnz) {
+      if (len > kTNnz) {
         PullLong L{(uint32_t)q, (uint32_t)blks.size(), 0, 0};
-        for (uint32_t e = tptr[q]; e < tptr[q + 1]; e += kPullNnz) {
-          blks.push_back(PullBlock{(uint32_t)q, 0, e, std::min(tptr[q + 1], e + kPullNnz)});
+        for (uint32_t e = tptr[q]; e < tptr[q + 1]; e += kTNnz) {
+          blks.push_back(PullBlock{(uint32_t)q, 0, e, std::min(tptr[q + 1], e + kTNnz)});
           L.nseg++;
         }
         longs.push_back(L);
@@ -407,8 +464,7 @@ grd_status build_tiers(grd_graph *g)
       } else {
         const uint64_t q0 = q;
         const uint32_t e0 = tptr[q];
-        while (q < g_end && q - q0 < kPullMaxRows && tptr[q + 1] - tptr[q] <= kPullNnz &&
-               tptr[q + 1] - e0 <= kPullNnz)
+        while (q < g_end && q - q0 < kTRows && tptr[q + 1] - tptr[q] <= kTNnz && tptr[q + 1] - e0 <= kTNnz)
           q++;
         blks.push_back(PullBlock{(uint32_t)q0, (uint32_t)(q - q0), e0, tptr[q]});
         add_arcs = tptr[q] - e0;
@@ -440,13 +496,12 @@ grd_status prp_pull_tiled(grd_state *s)
   hipStream_t st = g->ctx->stream;
   float *out = (float *)s->valR;
   if (g->n_ttile)
-    hipLaunchKernelGGL(k_prp_pull_tiled, dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile, g->tblk, g->tptr,
-                       g->trow, g->tcol, (const float *)s->mprop, out, s->tpart, s->tseg);
+    hipLaunchKernelGGL(k_prp_pull_tiled, dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile, g->tblk, g->trec,
+                       g->tcol, (const float *)s->mprop, out, s->tpart, s->tseg);
   if (g->n_tlong)
     hipLaunchKernelGGL(k_prp_tier_fixup, dim3(tgrid(g->n_tlong)), dim3(kBlock), 0, st, g->tlong, g->n_tlong, s->tseg,
-                       g->trow, out, s->tpart);
-  hipLaunchKernelGGL(k_prp_tier_merge, dim3(tgrid(g->P)), dim3(kBlock), 0, st, g->P, g->trptr, g->tinv, s->tpart,
-                     out);
+                       g->trec, out, s->tpart);
+  hipLaunchKernelGGL(k_prp_tier_merge, dim3(tgrid(g->P)), dim3(kBlock), 0, st, g->P, g->trptr, s->tpart, out);
   GRD_HIP(hipGetLastError());
   return GRD_OK;
 }

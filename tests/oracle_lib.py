@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+LIB_PATH = os.environ.get("GRUDON_ORACLE_LIB") or os.path.join(ROOT, "oracle", "liboracle.so")
 
 _P32 = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _P64 = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")

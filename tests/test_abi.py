@@ -10,7 +10,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "grudon_hip.h")
-LIB = os.path.join(ROOT, "grudon_amd", "libgrudon_hip.so")
+LIB = os.environ.get("GRUDON_HIP_LIB") or os.path.join(ROOT, "grudon_amd", "libgrudon_hip.so")
 CLI = os.path.join(ROOT, "grudon_amd", "bin", "grudon")
 
 

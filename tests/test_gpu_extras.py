@@ -216,3 +216,20 @@ def test_learned_run_decisions_match_host_features(ctx):
                 want = api.OPT_SHIP_EDGES if want & 1 == 0 else want
                 assert r.option == want, (algo, r.iteration)
         S.free()
+
+
+def test_batch_pagerank_repeats_bit_exact(ctx, small_batch_graphs):
+    """Residual and dangling mass are reduced per vertex tile and summed in
+    tile order: two runs give the same iterations and bit-identical ranks."""
+    _say("batch pr repeat")
+    B = api.Batch(ctx, [(g.rp, g.col) for g in small_batch_graphs])
+    try:
+        B.tune(1024, 16, 4)
+        it1, _ = B.pagerank(tolerance=1e-7, max_iter=300)
+        r1 = [B.result(i, api.PR_PULL).view(np.uint32) for i in range(B.count)]
+        it2, _ = B.pagerank(tolerance=1e-7, max_iter=300)
+        r2 = [B.result(i, api.PR_PULL).view(np.uint32) for i in range(B.count)]
+        assert np.array_equal(it1, it2)
+        assert all(np.array_equal(a, b) for a, b in zip(r1, r2))
+    finally:
+        B.free()
