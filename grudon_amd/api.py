@@ -29,7 +29,8 @@ EXPORTS = [
     "grd_state_create", "grd_state_create_typed", "grd_state_free", "grd_init", "grd_broadcast", "grd_traverse", "grd_reduce",
     "grd_update", "grd_aggregate_one", "grd_run", "grd_select_option", "grd_state_stats", "grd_download",
     "grd_features", "grd_snapshot", "grd_pair_exchange",
-    "grd_restore", "grd_selector_load", "grd_selector_train", "grd_selector_infer", "grd_selector_grad",
+    "grd_restore", "grd_selector_load", "grd_selector_train", "grd_selector_infer", "grd_selector_infer_noalloc",
+    "grd_selector_grad",
     "grd_batch_create", "grd_batch_free", "grd_batch_tune", "grd_batch_autotune", "grd_batch_pagerank",
     "grd_batch_cc", "grd_batch_download",
 ]
@@ -111,6 +112,7 @@ def lib():
             "grd_selector_train": [vp, vp, vp, vp, ct.c_uint64, ct.c_uint32, ct.c_uint32, ct.c_float, ct.c_uint64,
                                    vp, ct.POINTER(ct.c_float)],
             "grd_selector_infer": [vp, vp, ct.c_uint64, vp],
+            "grd_selector_infer_noalloc": [vp, vp, ct.c_uint64, vp],
             "grd_selector_grad": [vp, vp, vp, vp, vp, ct.c_uint64, ct.POINTER(ct.c_float), vp],
             "grd_batch_create": [vp, ct.c_uint32, vp, vp, vp, vp, pp], "grd_batch_free": [vp],
             "grd_batch_tune": [vp, ct.c_uint32, ct.c_uint32, ct.c_uint32], "grd_batch_autotune": [vp, vp],

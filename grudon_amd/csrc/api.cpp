@@ -256,6 +256,15 @@ grd_status grd_create(int device, grd_ctx **out)
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
     c->num_cus = cus;
+  // the allocation-free selector entry point's scratch on this device
+  grd_status sst = mlp_scratch_init(device);
+  if (sst != GRD_OK) {
+    for (auto &ev : c->ev)
+      if (ev) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return sst;
+  }
   *out = c;
   return GRD_OK;
 }

@@ -311,6 +311,9 @@ grd_status write_gr(const std::string &path, uint32_t n, const uint32_t *rp, con
 int heuristic_option(const grd_state *s);
 void make_features(const grd_state *s, float *f);
 // mlp.hip
+// device address of the selector's scratch array on `device` (current),
+// taken once per device; called by grd_create
+grd_status mlp_scratch_init(int device);
 grd_status mlp_select(grd_ctx *ctx, const float *feat16_host, int *option, float *margin);
 // Device-side selection for the next superstep from the counters of the one
 // just run: computes the 16 features from the counters on the device, runs

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "grd_device.hpp"
@@ -25,6 +26,31 @@ namespace grd {
 constexpr int kIn = 16, kH = 64, kOut = 4;
 constexpr int oW1 = 0, oB1 = oW1 + kH * kIn, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + kOut * kH;
 constexpr int kParams = oB3 + kOut;  // 5508
+
+// ---- staging of the allocation-free inference (grd_selector_infer_noalloc) ----
+//
+// Features and logits of up to kInferMax samples, in a file-scope device
+// array: every GPU the library runs on holds its own copy (320 KB) until the
+// process ends.  In host code `scratch` names a host-side stand-in, not the
+// device copy: its device address is taken once per device by
+// mlp_scratch_init (from grd_create, with that device current) and only that
+// address is used.  Calls on one GPU share the array: calls that overlap on
+// one GPU race.
+constexpr uint32_t kInferMax = 4096;
+static __device__ __align__(256) float scratch[kInferMax * (kIn + kOut)];
+constexpr int kMaxDevices = 64;
+static float *g_scratch_ptr[kMaxDevices] = {};
+
+grd_status mlp_scratch_init(int device)
+{
+  if (device < 0 || device >= kMaxDevices) return fail(GRD_ERR_ARG, "selector scratch: device id beyond 63");
+  if (g_scratch_ptr[device]) return GRD_OK;
+  float *ptr = nullptr;
+  GRD_HIP(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) return fail(GRD_ERR_HIP, "selector scratch: no device address");
+  g_scratch_ptr[device] = ptr;
+  return GRD_OK;
+}
 
 __host__ __device__ __forceinline__ float bf16_to_f32(uint16_t h)
 {
@@ -461,6 +487,28 @@ grd_status grd_selector_infer(grd_ctx *ctx, const float *feat, uint64_t n, float
   GRD_TRY(mem.alloc_n(&df, n * kIn));
   GRD_TRY(mem.alloc_n(&dl, n * kOut));
   GRD_HIP(hipMemcpy(df, feat, n * kIn * 4, hipMemcpyHostToDevice));
+  uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(1024, (n + 3) / 4));
+  hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->sel_w, df, (uint32_t)n, dl);
+  GRD_HIP(hipGetLastError());
+  GRD_HIP(hipMemcpyAsync(logits, dl, n * kOut * 4, hipMemcpyDeviceToHost, ctx->stream));
+  GRD_HIP(hipStreamSynchronize(ctx->stream));
+  return GRD_OK;
+}
+
+grd_status grd_selector_infer_noalloc(grd_ctx *ctx, const float *feat, uint64_t n, float *logits)
+{
+  if (!ctx || !feat || !logits || n == 0) return fail(GRD_ERR_ARG, "grd_selector_infer_noalloc: bad argument");
+  if (n > kInferMax)
+    return fail(GRD_ERR_ARG, "grd_selector_infer_noalloc: at most " + std::to_string(kInferMax) + " samples per call");
+  if (!ctx->sel_loaded) return fail(GRD_ERR_STATE, "grd_selector_infer_noalloc: no weights loaded");
+  if (ctx->device < 0 || ctx->device >= kMaxDevices || !g_scratch_ptr[ctx->device])
+    return fail(GRD_ERR_STATE, "grd_selector_infer_noalloc: scratch not set up for this device");
+  GRD_HIP(hipSetDevice(ctx->device));
+  // staging in the device's scratch copy: features, then logits; every
+  // element read is written first in this call
+  float *df = g_scratch_ptr[ctx->device];
+  float *dl = df + (size_t)kInferMax * kIn;
+  GRD_HIP(hipMemcpyAsync(df, feat, n * kIn * 4, hipMemcpyHostToDevice, ctx->stream));
   uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(1024, (n + 3) / 4));
   hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->sel_w, df, (uint32_t)n, dl);
   GRD_HIP(hipGetLastError());

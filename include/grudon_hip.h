@@ -294,6 +294,14 @@ grd_status grd_selector_train(grd_ctx *ctx, const float *feat, const uint32_t *l
                               uint16_t *out_bf16, float *final_loss);
 /* Forward pass on the GPU: logits[n*4]. */
 grd_status grd_selector_infer(grd_ctx *ctx, const float *feat, uint64_t n, float *logits);
+/* No counterpart in the reference; for callers that cannot allocate device
+ * memory (tight serving loops, code that forbids hipMalloc): the same
+ * forward pass as grd_selector_infer, bit for bit, with no allocation.  The
+ * features and logits are staged in a fixed device array of the library
+ * (320 KB per GPU, held until the process ends), so n is at most 4096
+ * (GRD_ERR_ARG beyond).  Calls that overlap on one GPU share that array
+ * and race; the caller still passes the host inputs and outputs. */
+grd_status grd_selector_infer_noalloc(grd_ctx *ctx, const float *feat, uint64_t n, float *logits);
 /* One forward+backward on the GPU from FP32 weights w[5508] (no BF16
  * rounding); sample_weight[n] may be NULL (all 1).  Writes the summed
  * weighted cross-entropy and its gradient (for the CPU gradient check). */
