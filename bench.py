@@ -568,13 +568,15 @@ def run_batch(api, ctx, args):
     return line(args, gteps, "GTEPS", secs * 1e3 / args.steps, "f32", config, None, cpu, extra)
 
 
-def _timed_leg(api, S, args, init_kw, arcs, sources=None):
+def _timed_leg(api, S, args, init_kw, arcs, sources=None, per_iteration=False):
     """Timed solves of one leg: (GTEPS, ms per step, iterations per step,
-    roofline of its dominant traversal kernel, last run stats)."""
+    kernel stats of its dominant traversal kernel, last run stats, seconds).
+    GTEPS = arcs x iterations / time with per_iteration (PageRank), else
+    arcs x solves / time (one solve per source)."""
     secs, iters, (kms, kl, kb), last = timed_solves(S, args, init_kw, sources=sources)
     nsrc = 1 if sources is None else len(sources)
-    gteps = arcs * nsrc * args.steps / secs / 1e9
-    return gteps, secs * 1e3 / args.steps, iters // max(1, args.steps), (kms, kl, kb), last, secs
+    work = arcs * iters if per_iteration else arcs * nsrc * args.steps
+    return work / secs / 1e9, secs * 1e3 / args.steps, iters // max(1, args.steps), (kms, kl, kb), last, secs
 
 
 def run_headline(api, ctx, args):
@@ -600,7 +602,7 @@ def run_headline(api, ctx, args):
         % (G.n, G.m, G.proxies, G.cross, t_gen, t_part, G.device_bytes / 1e9))
     S = api.State(G, api.PR_PULL)
     gteps, ms_step, iters, (kms, kl, kb), last, secs = _timed_leg(
-        api, S, args, {"max_iter": 1000, "tolerance": 1e-6}, G.m)
+        api, S, args, {"max_iter": 1000, "tolerance": 1e-6}, G.m, per_iteration=True)
     pr_iters = last["iterations"]
     log("PR pull: %d steps, %.2f ms/step, %d iterations, %.1f GTEPS" % (args.steps, ms_step, pr_iters, gteps))
     roof = roofline_of(kms, kl, kb, "k_prp_pull", traffic_from_profiles("k_prp_pull_rmat%d" % scale))
@@ -777,9 +779,10 @@ def main():
                     help="cc/sssp/pr: the heuristic never runs a push superstep as edge shipping")
     ap.add_argument("--workload", default="headline", choices=["headline", "pr_pull", "cc", "sssp", "pr", "batch",
                                                                    "learned", "schedules"])
-    ap.add_argument("--pull", default="tiled", choices=["tiled", "plain"],
-                    help="pull PageRank: LDS-tiled pull (default) or the plain row-block pull")
-    ap.add_argument("--tier-bits", type=int, default=0, help="tiled pull: 2^bits sources per tier (0 = 15)")
+    ap.add_argument("--pull", default="plain", choices=["plain", "tiled", "segmented"],
+                    help="pull PageRank: the row-block pull (default), LDS tiles or L2 segments")
+    ap.add_argument("--tier-bits", type=int, default=0,
+                    help="tiled / segmented pull: 2^bits sources per tier (0 = 15 / 19)")
     ap.add_argument("--tile-arcs", type=int, default=0, help="tiled pull: arcs per tile (0 = 65536)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 / exactness check")
@@ -806,7 +809,7 @@ def main():
     from grudon_amd import api
 
     ctx = api.Context(0)
-    ctx.set_pull_tiling(args.pull == "tiled", args.tier_bits, args.tile_arcs)
+    ctx.set_pull_tiling({"plain": 0, "tiled": 1, "segmented": 2}[args.pull], args.tier_bits, args.tile_arcs)
     run = {"headline": run_headline, "pr_pull": run_pagerank, "pr": run_pagerank, "cc": run_cc, "sssp": run_sssp, "batch": run_batch,
            "learned": run_learned, "schedules": run_schedules}[args.workload]
     out = run(api, ctx, args)

@@ -171,10 +171,11 @@ class Context:
     def set_host_threads(self, n):
         _check(lib().grd_set_host_threads(self.h, n), "grd_set_host_threads")
 
-    def set_pull_tiling(self, enable=True, tier_bits=0, tile_arcs=0):
-        """Pull PageRank states created afterwards: tiled pull (tier of
-        2^tier_bits sources, tiles of about tile_arcs arcs) or the plain pull."""
-        _check(lib().grd_set_pull_tiling(self.h, int(enable), tier_bits, tile_arcs), "grd_set_pull_tiling")
+    def set_pull_tiling(self, mode=1, tier_bits=0, tile_arcs=0):
+        """How pull PageRank states created afterwards pull: 0 (the default)
+        the row-block pull, 1 LDS tiles (tiers of 2^tier_bits sources, tiles
+        of about tile_arcs arcs), 2 L2 segments (tiers of 2^tier_bits)."""
+        _check(lib().grd_set_pull_tiling(self.h, int(mode), tier_bits, tile_arcs), "grd_set_pull_tiling")
 
     def set_layout(self, on_device):
         """Build partitioned layouts with device kernels (True) or host threads."""
@@ -254,6 +255,12 @@ class Graph:
         self.slots, self.proxies, self.cross = int(info[4]), int(info[5]), int(info[6])
         self.weighted = bool(info[7])
         self.device_bytes = int(info[8])
+
+    def info(self):
+        """The 16 words of grd_graph_info (the tiled pull's sizes in 11-14)."""
+        out = np.zeros(16, dtype=np.uint64)
+        _check(lib().grd_graph_info(self.h, _ptr(out)), "grd_graph_info")
+        return [int(x) for x in out]
 
     @classmethod
     def from_csr(cls, ctx, rp, col, w=None, C=1, M=1, mirror=None):

@@ -284,12 +284,14 @@ grd_status grd_destroy(grd_ctx *c)
   return GRD_OK;
 }
 
-grd_status grd_set_pull_tiling(grd_ctx *c, int enable, uint32_t tier_bits, uint32_t tile_arcs)
+grd_status grd_set_pull_tiling(grd_ctx *c, int mode, uint32_t tier_bits, uint32_t tile_arcs)
 {
   if (!c) return fail(GRD_ERR_ARG, "grd_set_pull_tiling: NULL context");
-  if (tier_bits && (tier_bits < 4 || tier_bits > kTierBitsMax))
-    return fail(GRD_ERR_ARG, "grd_set_pull_tiling: tier bits must be 0 (default) or 4..15");
-  c->pull_tiled = enable != 0;
+  if (mode < 0 || mode > 2) return fail(GRD_ERR_ARG, "grd_set_pull_tiling: mode is 0, 1 or 2");
+  const uint32_t hi = mode == 2 ? kSegBitsMax : kTierBitsMax;
+  if (tier_bits && (tier_bits < 4 || tier_bits > hi))
+    return fail(GRD_ERR_ARG, "grd_set_pull_tiling: tier bits must be 0 (default) or 4.." + std::to_string(hi));
+  c->pull_mode = mode;
   c->tier_bits = tier_bits;
   c->tile_arcs = tile_arcs;
   return GRD_OK;
@@ -466,6 +468,10 @@ grd_status grd_graph_info(const grd_graph *g, uint64_t *info)
   info[8] = g->mem.bytes;
   info[9] = g->n_pblk;
   info[10] = g->n_plong;
+  info[11] = g->n_pairs;  // tiled pull (0 until a pull PageRank state tiled the graph)
+  info[12] = g->n_ttile;
+  info[13] = g->n_tblk;
+  info[14] = g->n_tlong;
   return GRD_OK;
 }
 
@@ -528,7 +534,7 @@ grd_status grd_state_create_typed(grd_graph *g, int algo, uint32_t value_bytes, 
   s->vbytes = value_bytes;
   s->wide = value_bytes == 8;
   grd_status st = GRD_OK;
-  if (algo == GRD_PR_PULL && g->ctx->pull_tiled) {
+  if (algo == GRD_PR_PULL && g->ctx->pull_mode != 0) {
     try {
       st = build_tiers(g);
     } catch (const std::bad_alloc &) {

@@ -110,6 +110,8 @@ constexpr uint32_t kShipTile = 1024;     // edge shipping: wire positions per wo
 constexpr int kMaxParts = 64;            // largest C and M
 constexpr uint32_t kTierBitsMax = 15;    // tiled pull: largest tier, 2^15 sources (128 KB of LDS)
 constexpr uint32_t kTierTileArcs = 65536;  // tiled pull: default arcs per tile
+constexpr uint32_t kSegBitsMax = 24;       // segmented pull: largest segment, 2^24 sources
+constexpr uint32_t kSegBits = 19;          // segmented pull: default segment, 2^19 sources (2 MB of values)
 
 }  // namespace grd
 
@@ -119,8 +121,8 @@ struct grd_ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int host_threads = 0;
   int num_cus = 256;
-  bool pull_tiled = true;   // pull PageRank states use the tiled pull (tiers.hip)
-  uint32_t tier_bits = 0;   // its tunables for graphs tiled later (0 = default)
+  int pull_mode = 0;        // pull PageRank: 0 row-block pull, 1 LDS-tiled, 2 L2-segmented (tiers.hip)
+  uint32_t tier_bits = 0;   // tunables for graphs regrouped later (0 = the mode's default)
   uint32_t tile_arcs = 0;
   bool layout_on_device = true;  // build the partitioned layout with device kernels (else on the host)
   // learned selector weights (BF16, device) and threshold
@@ -173,10 +175,13 @@ struct grd_graph {
   // state: the CSC regrouped by (memory partition, source tier, row), a
   // (row, tier) "pair" per row and tier it has arcs in
   bool tiers_built = false;
+  int tier_mode = 0;               // 1 LDS-tiled (16-bit offsets, tiles), 2 L2-segmented (32-bit sources)
   uint32_t tier_bits = 0;          // a tier is 2^tier_bits consecutive proxies of a partition
   uint32_t tile_arcs = 0;
   uint64_t n_pairs = 0;
-  uint16_t *tcol = nullptr;        // [m] source offset within the pair's tier window
+  uint16_t *tcol = nullptr;        // [m] mode 1: source offset within the pair's tier window
+  uint32_t *tcol32 = nullptr;      // [m] mode 2: source proxy
+  uint32_t *tblk_hot = nullptr;    // mode 2: first proxy of each row block's tier (its LDS window)
   uint2 *trec = nullptr;           // [pairs+1] pairs in tier order: x = first arc, y = destination
                                    // (row | 1 << 31 for a row's only pair, else its (row, tier)-order index)
   uint32_t *trptr = nullptr;       // [P+1] pairs of each proxy row, in (row, tier) order
@@ -186,6 +191,8 @@ struct grd_graph {
   uint32_t n_tlong = 0;
   grd::TierTile *ttile = nullptr;
   uint32_t n_ttile = 0;
+  uint32_t *twide = nullptr;       // rows with many pairs, merged by a wave each
+  uint32_t n_twide = 0;
   grd::DevMem tmem;
 };
 

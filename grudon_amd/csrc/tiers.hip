@@ -42,6 +42,7 @@ constexpr int kTierBlock = 1024;                  // threads per tile workgroup
 constexpr int kTierWaves = kTierBlock / kWave;    // waves, each walking its own row blocks
 constexpr uint32_t kTNnz = 256;                   // arcs per row block (4 per lane)
 constexpr uint32_t kTRows = 256;                  // pairs per row block
+constexpr uint32_t kWideMerge = 32;               // rows with more pairs are merged by a wave
 constexpr uint32_t kSole = 0x80000000u;
 
 inline uint32_t tgrid(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(65535, (n + kBlock - 1) / kBlock)); }
@@ -135,19 +136,24 @@ __global__ void k_t_starts(uint64_t np, const uint32_t *__restrict__ tptr, uint2
   TIER_LOOP(i, np + 1) trec[i].x = tptr[i];
 }
 
-// every arc to its place in tier order, as an offset within its tier
+// every arc to its place in tier order: as a 16-bit offset within its tier
+// (tcol, mode 1) or as its source proxy (tcol32, mode 2)
 __global__ void k_t_copy(uint64_t m, const uint32_t *__restrict__ flag, const uint32_t *__restrict__ pid,
                          const uint32_t *__restrict__ pe0, const uint32_t *__restrict__ prow,
                          const uint32_t *__restrict__ tinv, const uint32_t *__restrict__ tptr,
                          const uint32_t *__restrict__ ccol, const uint32_t *__restrict__ pb, uint32_t M, uint32_t tb,
-                         uint16_t *__restrict__ tcol)
+                         uint16_t *__restrict__ tcol, uint32_t *__restrict__ tcol32)
 {
   TIER_LOOP(e, m)
   {
     const uint32_t p = pid[e] + flag[e] - 1;  // the pair holding arc e
-    const uint32_t base = pb[part_of(pb, M, prow[p])];
-    const uint32_t off = ccol[e] - base;
-    tcol[tptr[tinv[p]] + ((uint32_t)e - pe0[p])] = (uint16_t)(off & ((1u << tb) - 1u));
+    const uint32_t at = tptr[tinv[p]] + ((uint32_t)e - pe0[p]);
+    if (tcol32) {
+      tcol32[at] = ccol[e];
+    } else {
+      const uint32_t base = pb[part_of(pb, M, prow[p])];
+      tcol[at] = (uint16_t)((ccol[e] - base) & ((1u << tb) - 1u));
+    }
   }
 }
 
@@ -172,39 +178,36 @@ grd_status t_upload(DevMem &mem, hipStream_t st, T **dst, const std::vector<T> &
 
 // ---- the tiled pull ----
 
-// the loads of one row block, issued one block ahead of its use
-struct WaveBlock {
-  PullBlock d;
-  uint32_t G, nr;
-  uint32_t c[kTNnz / kWave];
-  uint32_t ps[kTRows / kWave], pe[kTRows / kWave], pd[kTRows / kWave];
-};
+// the loads of one row block, issued one block ahead of its use; plain
+// register arrays, fully unrolled, so that nothing goes to scratch
+constexpr int kTPer = kTNnz / kWave, kTRnd = kTRows / kWave;
 
-__device__ __forceinline__ void wave_block_load(WaveBlock &x, const PullBlock &d, uint32_t lane,
-                                                const uint16_t *__restrict__ tcol, const uint2 *__restrict__ trec)
+__device__ __forceinline__ void wave_block_load(const PullBlock &d, uint32_t lane, const uint16_t *__restrict__ tcol,
+                                                const uint2 *__restrict__ trec, uint32_t &G, uint32_t &nr,
+                                                uint32_t (&c)[kTPer], uint32_t (&ps)[kTRnd], uint32_t (&pe)[kTRnd],
+                                                uint32_t (&pd)[kTRnd])
 {
-  x.d = d;
-  x.G = 64;
-  x.nr = 0;
-  if (d.nrows == 0) return;  // empty slot, or a segment of a long pair (read where it is used)
-  while (x.G > 1 && x.G * d.nrows > (uint32_t)kWave) x.G >>= 1;
-  const uint32_t ngroups = kWave / x.G;
-  x.nr = (d.nrows + ngroups - 1) / ngroups;
-  const uint32_t ne = d.e1 - d.e0;
-#pragma unroll
-  for (int k = 0; k < (int)(kTNnz / kWave); k++) {
-    const uint32_t i = lane + k * kWave;
-    x.c[k] = i < ne ? tcol[d.e0 + i] : 0u;
+  G = 64;
+  nr = 0;
+  if (d.nrows != 0) {  // else an empty slot, or a segment of a long pair (read where it is used)
+    while (G > 1 && G * d.nrows > (uint32_t)kWave) G >>= 1;
+    nr = (d.nrows + kWave / G - 1) / (kWave / G);
   }
+  const uint32_t ngroups = kWave / G, ne = d.e1 - d.e0;
   const uint32_t *starts = reinterpret_cast<const uint32_t *>(trec);
 #pragma unroll
-  for (int k = 0; k < (int)(kTRows / kWave); k++) {
-    const uint32_t ql = k * ngroups + lane / x.G;
-    const bool ok = (uint32_t)k < x.nr && ql < d.nrows;
+  for (int k = 0; k < kTPer; k++) {
+    const uint32_t i = lane + k * kWave;
+    c[k] = (d.nrows != 0 && i < ne) ? tcol[d.e0 + i] : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < kTRnd; k++) {
+    const uint32_t ql = k * ngroups + lane / G;
+    const bool ok = (uint32_t)k < nr && ql < d.nrows;
     const uint2 r = ok ? trec[d.r0 + ql] : make_uint2(0u, 0u);
-    x.ps[k] = r.x;
-    x.pd[k] = r.y;
-    x.pe[k] = ok ? starts[2 * (d.r0 + ql + 1)] : 0u;
+    ps[k] = r.x;
+    pd[k] = r.y;
+    pe[k] = ok ? starts[2 * (d.r0 + ql + 1)] : 0u;
   }
 }
 
@@ -241,7 +244,7 @@ __global__ void __launch_bounds__(kTierBlock) k_prp_pull_tiled(const TierTile *_
 #pragma unroll
     for (int k = 0; k < kStage; k++) {
       const uint32_t i = threadIdx.x + k * kTierBlock;
-      if (i < n4) x[k] = src[i];
+      x[k] = src[i < n4 ? i : 0];  // unconditional loads keep x in registers
     }
 #pragma unroll
     for (int k = 0; k < kStage; k++) {
@@ -255,15 +258,15 @@ __global__ void __launch_bounds__(kTierBlock) k_prp_pull_tiled(const TierTile *_
   float *sv = s_v[w];
   const PullBlock none{0, 0, 0, 0};
   uint32_t bi = w;
-  WaveBlock cur;
-  wave_block_load(cur, bi < T.nb ? blocks[T.b0 + bi] : none, lane, tcol, trec);
+  PullBlock d = bi < T.nb ? blocks[T.b0 + bi] : none;
+  uint32_t G, nr, c[kTPer], ps[kTRnd], pe[kTRnd], pd[kTRnd];
+  wave_block_load(d, lane, tcol, trec, G, nr, c, ps, pe, pd);
   PullBlock dnext = bi + kTierWaves < T.nb ? blocks[T.b0 + bi + kTierWaves] : none;
   for (; bi < T.nb; bi += kTierWaves) {
     const uint32_t bn = bi + kTierWaves;
     const PullBlock dafter = bn + kTierWaves < T.nb ? blocks[T.b0 + bn + kTierWaves] : none;
-    WaveBlock nxt;
-    wave_block_load(nxt, dnext, lane, tcol, trec);
-    const PullBlock &d = cur.d;
+    uint32_t G2, nr2, c2[kTPer], ps2[kTRnd], pe2[kTRnd], pd2[kTRnd];
+    wave_block_load(dnext, lane, tcol, trec, G2, nr2, c2, ps2, pe2, pd2);
     if (d.nrows == 0) {
       // one segment of a long pair: a compensated sum of its arcs
       Kahan k;
@@ -273,65 +276,198 @@ __global__ void __launch_bounds__(kTierBlock) k_prp_pull_tiled(const TierTile *_
     } else {
       const uint32_t ne = d.e1 - d.e0;
 #pragma unroll
-      for (int k = 0; k < (int)(kTNnz / kWave); k++) {
+      for (int k = 0; k < kTPer; k++) {
         const uint32_t i = lane + k * kWave;
-        if (i < ne) sv[i] = s_win[cur.c[k]];
+        if (i < ne) sv[i] = s_win[c[k]];
       }
       wave_fence();
-      const uint32_t G = cur.G, sub = lane % G, ngroups = kWave / G;
+      const uint32_t sub = lane % G, ngroups = kWave / G;
 #pragma unroll
-      for (int k = 0; k < (int)(kTRows / kWave); k++) {
-        if ((uint32_t)k >= cur.nr) break;  // uniform over the wave
-        const uint32_t ql = k * ngroups + lane / G;
-        Kahan acc;
-        if (ql < d.nrows)
-          for (uint32_t x = cur.ps[k] - d.e0 + sub; x < cur.pe[k] - d.e0; x += G) acc.add(sv[x]);
-        kahan_wave_merge(acc, (int)G);
-        if (sub == 0 && ql < d.nrows) {
-          const uint32_t dst = cur.pd[k];
-          if (dst & kSole) out[dst & ~kSole] = acc.value();
-          else part[dst] = make_float2(acc.s, acc.c);
+      for (int k = 0; k < kTRnd; k++) {
+        if ((uint32_t)k < nr) {  // uniform over the wave
+          const uint32_t ql = k * ngroups + lane / G;
+          Kahan acc;
+          if (ql < d.nrows)
+            for (uint32_t x = ps[k] - d.e0 + sub; x < pe[k] - d.e0; x += G) acc.add(sv[x]);
+          kahan_wave_merge(acc, (int)G);
+          if (sub == 0 && ql < d.nrows) {
+            if (pd[k] & kSole) out[pd[k] & ~kSole] = acc.value();
+            else part[pd[k]] = make_float2(acc.s, acc.c);
+          }
         }
       }
       // every lane has read the slice before the next block writes it
       wave_fence();
     }
-    cur = nxt;
+    d = dnext;
     dnext = dafter;
+    G = G2;
+    nr = nr2;
+#pragma unroll
+    for (int k = 0; k < kTPer; k++) c[k] = c2[k];
+#pragma unroll
+    for (int k = 0; k < kTRnd; k++) {
+      ps[k] = ps2[k];
+      pe[k] = pe2[k];
+      pd[k] = pd2[k];
+    }
   }
 }
 
-// long pairs: their segments merged in order
+// ---- the segmented pull (mode 2) ----
+//
+// Tiers of 2^19 sources by default: a segment's values (2 MB) stay in L2
+// while its row blocks run, the blocks being ordered by segment.  The
+// blocks are the row-block pull's (at most kPullNnz arcs and kPullMaxRows
+// pairs, 8 column loads then 8 gathers per thread, a group of lanes per
+// pair); the hottest kPullHot sources of the block's segment are staged in
+// LDS.
+__global__ void __launch_bounds__(kBlock) k_prp_pull_seg(const PullBlock *__restrict__ blocks,
+                                                         const uint32_t *__restrict__ hot_base,
+                                                         const uint2 *__restrict__ trec,
+                                                         const uint32_t *__restrict__ tcol32,
+                                                         const float *__restrict__ mprop, uint32_t P,
+                                                         float *__restrict__ out, float2 *__restrict__ part,
+                                                         float2 *__restrict__ seg)
+{
+  __shared__ float s_v[kPullNnz];
+  __shared__ float s_hot[kPullHot];
+  __shared__ float s_ws[kWavesPerBlock], s_wc[kWavesPerBlock];
+  const uint32_t bid = blockIdx.x;
+  const PullBlock d = blocks[bid];
+  if (d.nrows == 0) {  // one segment of a long pair
+    Kahan k;
+    for (uint32_t e = d.e0 + threadIdx.x; e < d.e1; e += kBlock) k.add(mprop[tcol32[e]]);
+    kahan_wave_merge(k, kWave);
+    if (lane_id() == 0) {
+      s_ws[wave_in_block()] = k.s;
+      s_wc[wave_in_block()] = k.c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      Kahan t;
+      t.s = s_ws[0];
+      t.c = s_wc[0];
+      for (int j = 1; j < kWavesPerBlock; j++) t.merge(s_ws[j], s_wc[j]);
+      seg[bid] = make_float2(t.s, t.c);
+    }
+    return;
+  }
+  const uint32_t ne = d.e1 - d.e0;
+  {
+    constexpr int kPer = kPullNnz / kBlock;
+    uint32_t src[kPer];
+    float v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = threadIdx.x + k * kBlock;
+      src[k] = i < ne ? tcol32[d.e0 + i] : 0u;
+    }
+    const uint32_t hb = hot_base[bid];
+    const uint32_t hn = min(kPullHot, P - hb);
+    for (uint32_t i = threadIdx.x; i < hn; i += kBlock) s_hot[i] = mprop[hb + i];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = threadIdx.x + k * kBlock;
+      const uint32_t h = src[k] - hb;
+      v[k] = i < ne ? (h < hn ? s_hot[h] : mprop[src[k]]) : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = threadIdx.x + k * kBlock;
+      if (i < ne) s_v[i] = v[k];
+    }
+  }
+  __syncthreads();
+  int G = 64;
+  while (G > 1 && (uint32_t)G * d.nrows > (uint32_t)kBlock) G >>= 1;
+  const uint32_t sub = threadIdx.x % G, ngroups = kBlock / G;
+  const uint32_t rounds = (d.nrows + ngroups - 1) / ngroups;
+  const uint32_t *starts = reinterpret_cast<const uint32_t *>(trec);
+  for (uint32_t k = 0; k < rounds; k++) {
+    const uint32_t ql = k * ngroups + threadIdx.x / G;
+    const uint32_t q = d.r0 + ql;
+    Kahan acc;
+    uint32_t dst = 0;
+    if (ql < d.nrows) {
+      const uint2 r = trec[q];
+      dst = r.y;
+      const uint32_t b = r.x - d.e0, e = starts[2 * (q + 1)] - d.e0;
+      for (uint32_t x = b + sub; x < e; x += G) acc.add(s_v[x]);
+    }
+    kahan_wave_merge(acc, G);
+    if (sub == 0 && ql < d.nrows) {
+      if (dst & kSole) out[dst & ~kSole] = acc.value();
+      else part[dst] = make_float2(acc.s, acc.c);
+    }
+  }
+}
+
+// long pairs: a wave each, lane j taking segments j, j + 64, ... and the
+// lanes merged by a fixed butterfly, so the order is fixed
 __global__ void __launch_bounds__(kBlock) k_prp_tier_fixup(const PullLong *__restrict__ longs, uint32_t nlong,
                                                            const float2 *__restrict__ seg, const uint2 *__restrict__ trec,
                                                            float *__restrict__ out, float2 *__restrict__ part)
 {
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nlong; i += gridDim.x * kBlock) {
+  const uint32_t lane = lane_id();
+  for (uint32_t i = blockIdx.x * kWavesPerBlock + wave_in_block(); i < nlong; i += gridDim.x * kWavesPerBlock) {
     const PullLong L = longs[i];
     Kahan t;
-    for (uint32_t j = 0; j < L.nseg; j++) {
+    for (uint32_t j = lane; j < L.nseg; j += kWave) {
       const float2 p = seg[L.first + j];
       t.merge(p.x, p.y);
     }
-    const uint32_t dst = trec[L.row].y;
-    if (dst & kSole) out[dst & ~kSole] = t.value();
-    else part[dst] = make_float2(t.s, t.c);
+    kahan_wave_merge(t, kWave);
+    if (lane == 0) {
+      const uint32_t dst = trec[L.row].y;
+      if (dst & kSole) out[dst & ~kSole] = t.value();
+      else part[dst] = make_float2(t.s, t.c);
+    }
   }
 }
 
-// rows with no pair (0) or several: their partials, contiguous in tier order
+// rows with no pair (0) or 2..kWideMerge pairs: a thread each, over the
+// row's partials (contiguous, in tier order)
 __global__ void __launch_bounds__(kBlock) k_prp_tier_merge(uint32_t P, const uint32_t *__restrict__ trptr,
                                                            const float2 *__restrict__ part, float *__restrict__ out)
 {
   for (uint32_t r = blockIdx.x * kBlock + threadIdx.x; r < P; r += gridDim.x * kBlock) {
     const uint32_t a = trptr[r], b = trptr[r + 1];
-    if (b - a == 1) continue;
+    if (b - a == 1 || b - a > kWideMerge) continue;
     Kahan t;
-    for (uint32_t x = a; x < b; x++) {
+    uint32_t x = a;
+    for (; x + 4 <= b; x += 4) {
+      const float2 p0 = part[x], p1 = part[x + 1], p2 = part[x + 2], p3 = part[x + 3];
+      t.merge(p0.x, p0.y);
+      t.merge(p1.x, p1.y);
+      t.merge(p2.x, p2.y);
+      t.merge(p3.x, p3.y);
+    }
+    for (; x < b; x++) {
       const float2 p = part[x];
       t.merge(p.x, p.y);
     }
     out[r] = t.value();
+  }
+}
+
+// rows with more than kWideMerge pairs (the hubs): a wave each, as the
+// long pairs
+__global__ void __launch_bounds__(kBlock) k_prp_tier_merge_wide(const uint32_t *__restrict__ rows, uint32_t nrows,
+                                                                const uint32_t *__restrict__ trptr,
+                                                                const float2 *__restrict__ part, float *__restrict__ out)
+{
+  const uint32_t lane = lane_id();
+  for (uint32_t i = blockIdx.x * kWavesPerBlock + wave_in_block(); i < nrows; i += gridDim.x * kWavesPerBlock) {
+    const uint32_t r = rows[i], a = trptr[r], b = trptr[r + 1];
+    Kahan t;
+    for (uint32_t x = a + lane; x < b; x += kWave) {
+      const float2 p = part[x];
+      t.merge(p.x, p.y);
+    }
+    kahan_wave_merge(t, kWave);
+    if (lane == 0) out[r] = t.value();
   }
 }
 
@@ -346,9 +482,13 @@ grd_status build_tiers(grd_graph *g)
   hipStream_t st = ctx->stream;
   const uint32_t P = g->P, M = (uint32_t)g->M, C = (uint32_t)g->C;
   const uint64_t m = g->m;
-  const uint32_t tb = ctx->tier_bits ? ctx->tier_bits : kTierBitsMax;
+  const int mode = ctx->pull_mode;
+  if (mode != 1 && mode != 2) return fail(GRD_ERR_ARG, "tiled pull: mode 1 (LDS tiles) or 2 (L2 segments)");
+  const uint32_t tb = ctx->tier_bits ? ctx->tier_bits : (mode == 1 ? kTierBitsMax : kSegBits);
   const uint32_t tile_arcs = ctx->tile_arcs ? ctx->tile_arcs : kTierTileArcs;
-  if (tb < 4 || tb > kTierBitsMax) return fail(GRD_ERR_ARG, "tiled pull: tier bits must be 4..15");
+  if (tb < 4 || tb > (mode == 1 ? kTierBitsMax : kSegBitsMax)) return fail(GRD_ERR_ARG, "tiled pull: tier bits out of range");
+  // row blocks: a wave's (mode 1) or a workgroup's (mode 2)
+  const uint32_t blk_nnz = mode == 1 ? kTNnz : kPullNnz, blk_rows = mode == 1 ? kTRows : kPullMaxRows;
   if (P >= kSole) return fail(GRD_ERR_ARG, "tiled pull: more than 2^31 proxies");
   // the partitions' proxy ranges
   std::vector<uint32_t> pb(M + 1);
@@ -415,7 +555,8 @@ grd_status build_tiers(grd_graph *g)
   GRD_TRY(tmp.alloc_n(&tinv, std::max<uint64_t>(np, 1)));
   GRD_TRY(tmp.alloc_n(&d_tptr, np + 1));
   GRD_TRY(gm.alloc_n(&g->trec, np + 1));
-  GRD_TRY(gm.alloc_n(&g->tcol, std::max<uint64_t>(m, 1)));
+  if (mode == 1) GRD_TRY(gm.alloc_n(&g->tcol, std::max<uint64_t>(m, 1)));
+  else GRD_TRY(gm.alloc_n(&g->tcol32, std::max<uint64_t>(m, 1)));
   hipLaunchKernelGGL(k_t_fill, dim3(1), dim3(kBlock), 0, st, tlen + np, 1, 0u);
   hipLaunchKernelGGL(k_t_order, dim3(tgrid(np)), dim3(kBlock), 0, st, np, pval, prow, pe0, g->crp, g->trptr, tlen,
                      g->trec, tinv);
@@ -423,7 +564,7 @@ grd_status build_tiers(grd_graph *g)
   GRD_TRY(t_scan(tmp, st, tlen, d_tptr, np + 1));
   hipLaunchKernelGGL(k_t_starts, dim3(tgrid(np + 1)), dim3(kBlock), 0, st, np, d_tptr, g->trec);
   hipLaunchKernelGGL(k_t_copy, dim3(tgrid(m)), dim3(kBlock), 0, st, m, flag, pid, pe0, prow, tinv, d_tptr, g->ccol,
-                     d_pb, M, tb, g->tcol);
+                     d_pb, M, tb, g->tcol, g->tcol32);
   GRD_HIP(hipGetLastError());
   // host: row blocks within each (partition, tier) group, then tiles
   std::vector<uint32_t> tptr(np + 1), key(np);
@@ -433,6 +574,7 @@ grd_status build_tiers(grd_graph *g)
   std::vector<PullBlock> blks;
   std::vector<PullLong> longs;
   std::vector<TierTile> tiles;
+  std::vector<uint32_t> hot;  // mode 2: the window base of each block's tier
   uint64_t q = 0;
   while (q < np) {
     uint64_t g_end = q;
@@ -452,10 +594,10 @@ grd_status build_tiers(grd_graph *g)
       const uint32_t len = tptr[q + 1] - tptr[q];
       uint64_t add_arcs = 0;
       const size_t before = blks.size();
-      if (len > kTNnz) {
+      if (len > blk_nnz) {
         PullLong L{(uint32_t)q, (uint32_t)blks.size(), 0, 0};
-        for (uint32_t e = tptr[q]; e < tptr[q + 1]; e += kTNnz) {
-          blks.push_back(PullBlock{(uint32_t)q, 0, e, std::min(tptr[q + 1], e + kTNnz)});
+        for (uint32_t e = tptr[q]; e < tptr[q + 1]; e += blk_nnz) {
+          blks.push_back(PullBlock{(uint32_t)q, 0, e, std::min(tptr[q + 1], e + blk_nnz)});
           L.nseg++;
         }
         longs.push_back(L);
@@ -464,26 +606,37 @@ grd_status build_tiers(grd_graph *g)
       } else {
         const uint64_t q0 = q;
         const uint32_t e0 = tptr[q];
-        while (q < g_end && q - q0 < kTRows && tptr[q + 1] - tptr[q] <= kTNnz && tptr[q + 1] - e0 <= kTNnz)
+        while (q < g_end && q - q0 < blk_rows && tptr[q + 1] - tptr[q] <= blk_nnz && tptr[q + 1] - e0 <= blk_nnz)
           q++;
         blks.push_back(PullBlock{(uint32_t)q0, (uint32_t)(q - q0), e0, tptr[q]});
         add_arcs = tptr[q] - e0;
       }
       cur_arcs += add_arcs;
+      while (hot.size() < blks.size()) hot.push_back(wbase);
       (void)before;
-      if (cur_arcs >= tile_arcs) close_tile();
+      if (mode == 1 && cur_arcs >= tile_arcs) close_tile();
     }
-    close_tile();
+    if (mode == 1) close_tile();
   }
   if (blks.size() >= 0x7FFFFFFFull || tiles.size() >= 0x7FFFFFFFull) return fail(GRD_ERR_ARG, "tiled pull: too many blocks");
+  // rows whose partials a wave merges
+  std::vector<uint32_t> trptr((size_t)P + 1), wide;
+  GRD_HIP(hipMemcpyAsync(trptr.data(), g->trptr, ((size_t)P + 1) * 4, hipMemcpyDeviceToHost, st));
+  GRD_HIP(hipStreamSynchronize(st));
+  for (uint32_t r = 0; r < P; r++)
+    if (trptr[r + 1] - trptr[r] > kWideMerge) wide.push_back(r);
+  g->n_twide = (uint32_t)wide.size();
+  GRD_TRY(t_upload(gm, st, &g->twide, wide));
   g->n_tblk = (uint32_t)blks.size();
   g->n_tlong = (uint32_t)longs.size();
   g->n_ttile = (uint32_t)tiles.size();
   GRD_TRY(t_upload(gm, st, &g->tblk, blks));
   GRD_TRY(t_upload(gm, st, &g->tlong, longs));
   GRD_TRY(t_upload(gm, st, &g->ttile, tiles));
+  if (mode == 2) GRD_TRY(t_upload(gm, st, &g->tblk_hot, hot));
   GRD_HIP(hipStreamSynchronize(st));  // the host vectors and temporaries go out of scope
   g->n_pairs = np;
+  g->tier_mode = mode;
   g->tier_bits = tb;
   g->tile_arcs = tile_arcs;
   g->tiers_built = true;
@@ -495,13 +648,22 @@ grd_status prp_pull_tiled(grd_state *s)
   grd_graph *g = s->g;
   hipStream_t st = g->ctx->stream;
   float *out = (float *)s->valR;
-  if (g->n_ttile)
+  if (g->tier_mode == 1 && g->n_ttile)
     hipLaunchKernelGGL(k_prp_pull_tiled, dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile, g->tblk, g->trec,
                        g->tcol, (const float *)s->mprop, out, s->tpart, s->tseg);
+  if (g->tier_mode == 2 && g->n_tblk)
+    hipLaunchKernelGGL(k_prp_pull_seg, dim3(g->n_tblk), dim3(kBlock), 0, st, g->tblk, g->tblk_hot, g->trec, g->tcol32,
+                       (const float *)s->mprop, g->P, out, s->tpart, s->tseg);
+  auto wave_grid = [](uint64_t n) {
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(65535, (n + kWavesPerBlock - 1) / kWavesPerBlock));
+  };
   if (g->n_tlong)
-    hipLaunchKernelGGL(k_prp_tier_fixup, dim3(tgrid(g->n_tlong)), dim3(kBlock), 0, st, g->tlong, g->n_tlong, s->tseg,
-                       g->trec, out, s->tpart);
+    hipLaunchKernelGGL(k_prp_tier_fixup, dim3(wave_grid(g->n_tlong)), dim3(kBlock), 0, st, g->tlong, g->n_tlong,
+                       s->tseg, g->trec, out, s->tpart);
   hipLaunchKernelGGL(k_prp_tier_merge, dim3(tgrid(g->P)), dim3(kBlock), 0, st, g->P, g->trptr, s->tpart, out);
+  if (g->n_twide)
+    hipLaunchKernelGGL(k_prp_tier_merge_wide, dim3(wave_grid(g->n_twide)), dim3(kBlock), 0, st, g->twide, g->n_twide,
+                       g->trptr, s->tpart, out);
   GRD_HIP(hipGetLastError());
   return GRD_OK;
 }

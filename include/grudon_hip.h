@@ -136,15 +136,20 @@ grd_status grd_set_host_threads(grd_ctx *ctx, int nthreads);
 grd_status grd_set_layout(grd_ctx *ctx, int on_device);
 
 /* No counterpart in the reference: how pull-PageRank states (GRD_PR_PULL)
- * created afterwards run their pull.  enable != 0 (the default): the tiled
- * pull, which regroups a graph's CSC by (memory partition, source tier, row)
- * when its first pull-PageRank state is created and serves every gather from
- * a tier window staged in LDS; a tier is 2^tier_bits consecutive proxies
- * (4..15, 0 = 15) and a tile about tile_arcs arcs (0 = 65536).  A graph
- * keeps the tiling it was first given.  enable == 0: the plain row-block
- * pull over the CSC.  Both compute the same sums within FP32 rounding (Kahan
- * partials merged in a fixed order). */
-grd_status grd_set_pull_tiling(grd_ctx *ctx, int enable, uint32_t tier_bits, uint32_t tile_arcs);
+ * created afterwards run their pull.
+ *   mode 0 (the default): the row-block pull over the CSC;
+ *   mode 1: LDS tiles -- the CSC regrouped by (memory partition, source
+ *     tier, row), a tier being 2^tier_bits consecutive proxies (4..15, 0 =
+ *     15), every gather served from the tier's window staged in LDS by
+ *     tiles of about tile_arcs arcs (0 = 65536);
+ *   mode 2: L2 segments -- the same regrouping with tiers of 2^tier_bits
+ *     proxies (4..24, 0 = 19: 2 MB of values, resident in L2 while their row
+ *     blocks run) and the row-block pull's kernel over them.
+ * Modes 1 and 2 regroup a graph when its first pull-PageRank state is
+ * created, and a graph keeps the regrouping it was first given; per row, the
+ * tiers' Kahan partials are merged in tier order.  All modes compute the
+ * same sums within FP32 rounding. */
+grd_status grd_set_pull_tiling(grd_ctx *ctx, int mode, uint32_t tier_bits, uint32_t tile_arcs);
 
 /* ---- graphs ---- */
 
@@ -190,7 +195,10 @@ grd_status grd_graph_read(const char *path, uint32_t *n, uint64_t *m, uint32_t *
 grd_status grd_graph_write_gr(const char *path, uint32_t n, const uint32_t *row_ptr, const uint32_t *col,
                               const uint32_t *weights);
 /* info[0] n, [1] m, [2] C, [3] M, [4] compute slots, [5] memory proxies,
- * [6] cross mirrors, [7] weighted, [8] device bytes held by the graph. */
+ * [6] cross mirrors, [7] weighted, [8] device bytes held by the graph,
+ * [9] pull row blocks, [10] long rows; the tiled pull's [11] (row, tier)
+ * pairs, [12] tiles, [13] row blocks, [14] long pairs (0 until the first
+ * pull-PageRank state of the graph). */
 grd_status grd_graph_info(const grd_graph *g, uint64_t *info16);
 /* Translation table of pair (c, m) as gids in wire-index order: the
  * reference sTranslationTable/rTranslationTable content

@@ -21,8 +21,8 @@ def _say(*a):
     print(*a, flush=True)
 
 
-def _solve(ctx, g, C, M, tiled, tier_bits=0, tile_arcs=0, tol=1e-6):
-    ctx.set_pull_tiling(tiled, tier_bits, tile_arcs)
+def _solve(ctx, g, C, M, mode, tier_bits=0, tile_arcs=0, tol=1e-6):
+    ctx.set_pull_tiling(mode, tier_bits, tile_arcs)
     try:
         G = api.Graph.from_csr(ctx, g.rp, g.col, C=C, M=M)
         S = api.State(G, api.PR_PULL)
@@ -32,18 +32,19 @@ def _solve(ctx, g, C, M, tiled, tier_bits=0, tile_arcs=0, tol=1e-6):
         S.free()
         G.free()
     finally:
-        ctx.set_pull_tiling(True, 0, 0)
+        ctx.set_pull_tiling(0, 0, 0)
     return r, st
 
 
-@pytest.mark.parametrize("scale,C,M,tier_bits,tile_arcs", [
-    (12, 1, 1, 4, 0), (12, 1, 4, 6, 4096), (13, 2, 3, 5, 2048), (14, 1, 2, 8, 0), (14, 3, 2, 10, 8192),
-    (16, 1, 1, 15, 0), (16, 1, 4, 12, 0),
+@pytest.mark.parametrize("mode,scale,C,M,tier_bits,tile_arcs", [
+    (1, 12, 1, 1, 4, 0), (1, 12, 1, 4, 6, 4096), (1, 13, 2, 3, 5, 2048), (1, 14, 1, 2, 8, 0),
+    (1, 14, 3, 2, 10, 8192), (1, 16, 1, 1, 15, 0), (1, 16, 1, 4, 12, 0),
+    (2, 12, 1, 1, 4, 0), (2, 13, 2, 3, 6, 0), (2, 14, 1, 4, 8, 0), (2, 16, 1, 1, 10, 0), (2, 16, 1, 4, 0, 0),
 ])
-def test_tiled_pull_fp64_bound(ctx, scale, C, M, tier_bits, tile_arcs):
-    _say("tiled pull", scale, C, M, tier_bits, tile_arcs)
+def test_tiled_pull_fp64_bound(ctx, mode, scale, C, M, tier_bits, tile_arcs):
+    _say("tiled pull", mode, scale, C, M, tier_bits, tile_arcs)
     g = O.rmat(scale, 16, seed=scale + 7)
-    r, st = _solve(ctx, g, C, M, True, tier_bits, tile_arcs)
+    r, st = _solve(ctx, g, C, M, mode, tier_bits, tile_arcs)
     ref, _, _ = O.pr_pull(g, iters=st["iterations"], nthreads=4)
     l1 = float(np.abs(r.astype(np.float64) - ref).sum())
     _, it_conv, _ = O.pr_pull(g, iters=0, tol=1e-6, nthreads=4)
@@ -55,12 +56,13 @@ def test_tiled_pull_fp64_bound(ctx, scale, C, M, tier_bits, tile_arcs):
 def test_tiled_equals_plain_pull_and_repeats_bit_exact(ctx):
     _say("tiled vs plain")
     g = O.rmat(15, 16, seed=3)
-    rt, st_t = _solve(ctx, g, 1, 4, True, 9, 4096)
-    rt2, _ = _solve(ctx, g, 1, 4, True, 9, 4096)
-    rp, st_p = _solve(ctx, g, 1, 4, False)
-    assert np.array_equal(rt.view(np.uint32), rt2.view(np.uint32))  # deterministic
-    assert st_t["iterations"] == st_p["iterations"]
-    assert float(np.abs(rt.astype(np.float64) - rp.astype(np.float64)).sum()) <= 1e-6
+    rp, st_p = _solve(ctx, g, 1, 4, 0)
+    for mode, tb in ((1, 9), (2, 11)):
+        rt, st_t = _solve(ctx, g, 1, 4, mode, tb, 4096)
+        rt2, _ = _solve(ctx, g, 1, 4, mode, tb, 4096)
+        assert np.array_equal(rt.view(np.uint32), rt2.view(np.uint32)), mode  # deterministic
+        assert st_t["iterations"] == st_p["iterations"], mode
+        assert float(np.abs(rt.astype(np.float64) - rp.astype(np.float64)).sum()) <= 1e-6, mode
 
 
 def test_tiled_pull_er_and_sink_heavy(ctx):
@@ -69,14 +71,20 @@ def test_tiled_pull_er_and_sink_heavy(ctx):
     _say("tiled er / sinks")
     for g in (O.er(20000, 12.0, seed=5), O.from_edges(9, [(0, 1), (1, 2), (2, 0), (3, 4), (5, 4), (6, 4)])):
         for M in (1, 3):
-            r, st = _solve(ctx, g, 1, M, True, 4, 1024)
-            ref, _, _ = O.pr_pull(g, iters=st["iterations"])
-            assert float(np.abs(r.astype(np.float64) - ref).sum()) <= 1e-6, (g.n, M)
+            for mode in (1, 2):
+                r, st = _solve(ctx, g, 1, M, mode, 4, 1024)
+                ref, _, _ = O.pr_pull(g, iters=st["iterations"])
+                assert float(np.abs(r.astype(np.float64) - ref).sum()) <= 1e-6, (g.n, M, mode)
 
 
 def test_pull_tiling_argument_checks(ctx):
     with pytest.raises(api.GrdError):
-        ctx.set_pull_tiling(True, 3, 0)
+        ctx.set_pull_tiling(1, 3, 0)
     with pytest.raises(api.GrdError):
-        ctx.set_pull_tiling(True, 16, 0)
-    ctx.set_pull_tiling(True, 0, 0)
+        ctx.set_pull_tiling(1, 16, 0)
+    with pytest.raises(api.GrdError):
+        ctx.set_pull_tiling(2, 25, 0)
+    with pytest.raises(api.GrdError):
+        ctx.set_pull_tiling(3, 0, 0)
+    ctx.set_pull_tiling(2, 24, 0)
+    ctx.set_pull_tiling(0, 0, 0)
