@@ -41,6 +41,10 @@ Other workloads (one JSON line each, same fields), e.g. for profiling:
   schedules  the reference's offloaded push, its NDPEngine vote, edge
            shipping every superstep and the heuristic: CC, SSSP hop and
            weighted; value = geometric mean speedup of edge shipping
+  selector  every option timed at every superstep of held-out CC / SSSP
+           runs: the hindsight-best schedule against the heuristic and the
+           learned selector; value = heuristic / best (the ceiling of any
+           per-superstep selector's gain over the heuristic)
 
     python bench.py [--steps K] [--warmup W] [--workload NAME] [--scale S]
                     [--parts M] [--no-cpu] [--no-check]
@@ -711,6 +715,79 @@ def run_learned(api, ctx, args):
                 cpu, extra)
 
 
+def run_selector(api, ctx, args):
+    """What a per-superstep schedule selector can gain over the heuristic.
+    Held-out seeded graphs (RMAT-22/24 EF16 seed 77, Erdos-Renyi n 2^22
+    degree 16 seed 78), C=1 M=4: CC (symmetrized), SSSP hop count and
+    weighted delta-stepping from the highest-degree vertex.  Their supersteps
+    are deterministic, so every fixed option visits the same states: with
+    each option timed at every superstep, the sum over supersteps of the
+    fastest option's time is the best any per-superstep selector could reach
+    (hindsight best).  heuristic / best bounds what the learned selector can
+    gain over the heuristic; learned / heuristic is what it does gain.  All
+    times are per-superstep device times of the same runs (time_iters)."""
+    import numpy as np
+
+    w = np.fromfile(os.path.join(ROOT, "grudon_amd", "selector_bf16.bin"), dtype="<u2")
+    ctx.selector_load(w, 0.0)
+    options = [api.OPT_PUSH_SPARSE, api.OPT_PULL_SPARSE, api.OPT_PUSH_DENSE, api.OPT_PULL_DENSE, api.OPT_SHIP_EDGES]
+    graphs = [("rmat22", 22), ("rmat24", 24), ("er22", 0)]
+    rows = []
+    for gname, scale in graphs:
+        for sym, algo, aname, kw in [(True, api.CC, "cc", {}), (False, api.SSSP, "sssp_hop", {}),
+                                     (False, api.SSSP, "sssp_weighted", {"weighted": True, "delta": args.delta})]:
+            if scale:
+                G = api.Graph.rmat(ctx, scale, 16, seed=77, symmetrize=sym, wseed=2 if kw.get("weighted") else 0,
+                                   C=1, M=4)
+            else:
+                G = api.Graph.er(ctx, 1 << 22, 16.0, seed=78, wseed=2 if kw.get("weighted") else 0, C=1, M=4)
+            if algo == api.SSSP:
+                kw = dict(kw, source=int(np.argmax(G.out_degree())))
+            S = api.State(G, algo)
+
+            def traced(**ikw):
+                S.init(max_iter=100000, time_iters=True, **kw, **ikw)
+                S.run(trace_cap=1 << 16)  # warm-up
+                S.init(max_iter=100000, time_iters=True, **kw, **ikw)
+                st, tr = S.run(trace_cap=1 << 16)
+                return st, np.array([r.ms for r in tr]), S.result()
+
+            per_opt, ref = [], None
+            for o in options:
+                st, t, res = traced(sched=api.SCHED_FIXED, fixed_option=o)
+                ref = res if ref is None else ref
+                assert np.array_equal(res, ref), "a fixed option changed the result"
+                per_opt.append(t)
+            n = min(len(t) for t in per_opt)
+            best = float(np.min(np.stack([t[:n] for t in per_opt]), axis=0).sum())
+            st_h, t_h, res_h = traced(sched=api.SCHED_HEURISTIC)
+            st_l, t_l, res_l = traced(sched=api.SCHED_LEARNED)
+            assert np.array_equal(res_h, ref) and np.array_equal(res_l, ref)
+            row = {"graph": gname, "algo": aname, "supersteps": int(n), "arcs": G.m,
+                   "ms": {"best": round(best, 3), "heuristic": round(float(t_h.sum()), 3),
+                          "learned": round(float(t_l.sum()), 3),
+                          "fixed": [round(float(t.sum()), 3) for t in per_opt]},
+                   "heuristic_over_best": round(float(t_h.sum()) / best, 3),
+                   "learned_vs_heuristic": round(float(t_h.sum()) / float(t_l.sum()), 3)}
+            log("selector %s" % json.dumps(row))
+            rows.append(row)
+            S.free()
+            G.free()
+    geo = lambda xs: math.exp(sum(math.log(x) for x in xs) / len(xs))
+    ceiling = geo([r["heuristic_over_best"] for r in rows])
+    gain = geo([r["learned_vs_heuristic"] for r in rows])
+    config = {"workload": "per-superstep schedule selection: CC, SSSP hop and weighted delta-stepping (delta %d) on "
+                          "held-out RMAT-22/24 EF16 seed 77 and Erdos-Renyi 2^22 degree 16 seed 78, 1 compute + 4 "
+                          "memory partitions" % args.delta,
+              "value_definition": "geometric mean of heuristic time / hindsight-best time: the largest gain any "
+                                  "per-superstep selector could have over the heuristic"}
+    cpu = {"value": None, "unit": "x", "cores": 0, "kind": "port",
+           "sample": "none: the value is a GPU-to-GPU schedule ratio"}
+    ms = sum(r["ms"]["learned"] for r in rows)
+    return line(args, ceiling, "x (heuristic / hindsight best, geometric mean)", ms, "u32", config, None, cpu,
+                {"selector": rows, "learned_vs_heuristic_geomean": round(gain, 4)})
+
+
 def run_schedules(api, ctx, args):
     """The reference's two supersteps and its offload vote on RMAT-`scale`
     (C compute + M memory partitions): plain (the offloaded push every
@@ -778,7 +855,7 @@ def main():
     ap.add_argument("--offload-only", action="store_true",
                     help="cc/sssp/pr: the heuristic never runs a push superstep as edge shipping")
     ap.add_argument("--workload", default="headline", choices=["headline", "pr_pull", "cc", "sssp", "pr", "batch",
-                                                                   "learned", "schedules"])
+                                                                   "learned", "schedules", "selector"])
     ap.add_argument("--pull", default="plain", choices=["plain", "tiled", "segmented"],
                     help="pull PageRank: the row-block pull (default), LDS tiles or L2 segments")
     ap.add_argument("--tier-bits", type=int, default=0,
@@ -792,9 +869,18 @@ def main():
                          "its learned-vs-plain leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a cold device copy of MIB MiB (PMC counter calibration)")
+    ap.add_argument("--calibrate-stream", type=int, default=0, metavar="MIB",
+                    help="only run the probe library's streaming read and streaming write of MIB MiB, "
+                         "one launch each (PMC counter calibration with kernels of known bytes)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.calibrate_stream:
+        from grudon_amd import api as _api
+        nb = args.calibrate_stream << 20
+        log("calibration stream: read %d MiB: %.3f ms" % (args.calibrate_stream, _api.probe_stream(0, nb, 1)))
+        log("calibration stream: write %d MiB: %.3f ms" % (args.calibrate_stream, _api.probe_stream(0, nb, 1, True)))
         return
     if not args.scale:
         args.scale = 26 if args.workload in ("headline", "learned") else 24
@@ -811,7 +897,7 @@ def main():
     ctx = api.Context(0)
     ctx.set_pull_tiling({"plain": 0, "tiled": 1, "segmented": 2}[args.pull], args.tier_bits, args.tile_arcs)
     run = {"headline": run_headline, "pr_pull": run_pagerank, "pr": run_pagerank, "cc": run_cc, "sssp": run_sssp, "batch": run_batch,
-           "learned": run_learned, "schedules": run_schedules}[args.workload]
+           "learned": run_learned, "schedules": run_schedules, "selector": run_selector}[args.workload]
     out = run(api, ctx, args)
     print(json.dumps(out), flush=True)
 

@@ -506,7 +506,7 @@ def probe_lib():
         L.grdm_probe_gather.argtypes = [ct.c_int, ct.c_uint64, ct.c_uint64, ct.POINTER(ct.c_double),
                                         ct.POINTER(ct.c_double)]
         L.grdm_probe_stream.restype = ct.c_int
-        L.grdm_probe_stream.argtypes = [ct.c_int, ct.c_uint64, ct.c_int, ct.POINTER(ct.c_double)]
+        L.grdm_probe_stream.argtypes = [ct.c_int, ct.c_uint64, ct.c_int, ct.c_int, ct.POINTER(ct.c_double)]
         _probe = L
     return _probe
 
@@ -520,10 +520,10 @@ def probe_gather(device, table_entries, n_gathers=1 << 28):
     return ms.value, rate.value
 
 
-def probe_stream(device, nbytes, launches=3):
-    """Streaming read of nbytes, launches times: mean ms per launch."""
+def probe_stream(device, nbytes, launches=3, write=False):
+    """Streaming read (or write) of nbytes, launches times: mean ms per launch."""
     ms = ct.c_double()
     L = probe_lib()
-    if L.grdm_probe_stream(device, nbytes, launches, ct.byref(ms)) != 0:
+    if L.grdm_probe_stream(device, nbytes, launches, int(write), ct.byref(ms)) != 0:
         raise GrdError("grdm_probe_stream: " + L.grdm_last_error().decode())
     return ms.value

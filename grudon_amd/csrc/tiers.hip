@@ -15,9 +15,10 @@
 //   tile         = a run of row blocks of one (partition, tier) group
 //
 // A 1024-thread workgroup takes a tile and stages the tier's window of mprop
-// (<= 128 KB) into LDS with coalesced 16-byte loads; then each wave walks its
-// own row blocks (4 coalesced column loads and 4 LDS gathers per lane, a
-// group of lanes per pair, FP32 Kahan per lane merged by TwoSum).  Each pair yields one compensated partial; a row with a
+// (<= 128 KB) into LDS with coalesced 16-byte loads; then its four quarters
+// walk the tile's row blocks four at a time (8 coalesced column loads and 8
+// LDS gathers per thread, a group of lanes per pair, FP32 Kahan per lane
+// merged by TwoSum).  Each pair yields one compensated partial; a row with a
 // single pair writes its value straight into the reduce buffer, the others
 // are merged per row in tier order (k_prp_tier_merge), so the result does not
 // depend on scheduling.  A pair longer than one block is cut into segments
@@ -39,9 +40,9 @@ namespace grd {
 namespace {
 
 constexpr int kTierBlock = 1024;                  // threads per tile workgroup
-constexpr int kTierWaves = kTierBlock / kWave;    // waves, each walking its own row blocks
-constexpr uint32_t kTNnz = 256;                   // arcs per row block (4 per lane)
-constexpr uint32_t kTRows = 256;                  // pairs per row block
+constexpr int kTierQuarters = kTierBlock / kBlock;  // row blocks per round, one per 256 threads
+constexpr uint32_t kTNnz = kPullNnz;              // arcs per row block (8 per thread)
+constexpr uint32_t kTRows = kPullMaxRows;         // pairs per row block (at most 4 per thread)
 constexpr uint32_t kWideMerge = 32;               // rows with more pairs are merged by a wave
 constexpr uint32_t kSole = 0x80000000u;
 
@@ -178,52 +179,13 @@ grd_status t_upload(DevMem &mem, hipStream_t st, T **dst, const std::vector<T> &
 
 // ---- the tiled pull ----
 
-// the loads of one row block, issued one block ahead of its use; plain
-// register arrays, fully unrolled, so that nothing goes to scratch
-constexpr int kTPer = kTNnz / kWave, kTRnd = kTRows / kWave;
-
-__device__ __forceinline__ void wave_block_load(const PullBlock &d, uint32_t lane, const uint16_t *__restrict__ tcol,
-                                                const uint2 *__restrict__ trec, uint32_t &G, uint32_t &nr,
-                                                uint32_t (&c)[kTPer], uint32_t (&ps)[kTRnd], uint32_t (&pe)[kTRnd],
-                                                uint32_t (&pd)[kTRnd])
-{
-  G = 64;
-  nr = 0;
-  if (d.nrows != 0) {  // else an empty slot, or a segment of a long pair (read where it is used)
-    while (G > 1 && G * d.nrows > (uint32_t)kWave) G >>= 1;
-    nr = (d.nrows + kWave / G - 1) / (kWave / G);
-  }
-  const uint32_t ngroups = kWave / G, ne = d.e1 - d.e0;
-  const uint32_t *starts = reinterpret_cast<const uint32_t *>(trec);
-#pragma unroll
-  for (int k = 0; k < kTPer; k++) {
-    const uint32_t i = lane + k * kWave;
-    c[k] = (d.nrows != 0 && i < ne) ? tcol[d.e0 + i] : 0u;
-  }
-#pragma unroll
-  for (int k = 0; k < kTRnd; k++) {
-    const uint32_t ql = k * ngroups + lane / G;
-    const bool ok = (uint32_t)k < nr && ql < d.nrows;
-    const uint2 r = ok ? trec[d.r0 + ql] : make_uint2(0u, 0u);
-    ps[k] = r.x;
-    pd[k] = r.y;
-    pe[k] = ok ? starts[2 * (d.r0 + ql + 1)] : 0u;
-  }
-}
-
-__device__ __forceinline__ void wave_fence()
-{
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// One 1024-thread workgroup per tile: the tier window is staged with 16-byte
-// loads, all issued before the first LDS store, behind one barrier.  Then
-// each of the 16 waves walks its own share of the tile's row blocks (at most
-// kTNnz arcs and kTRows pairs each) with no workgroup barrier, the loads of
-// its next block in flight while it gathers the current block's arcs from
-// the window into its own slice of s_v and groups of G lanes sum the pairs.
+// One 1024-thread workgroup per tile.  The tier window is staged with 16-byte
+// loads, all issued before the first LDS store.  Then the four 256-thread
+// quarters take four row blocks per round (at most kTNnz arcs and kTRows
+// pairs each): every thread issues its 8 column loads and the records of the
+// up to 4 pairs it will sum, gathers its arcs' values from the window into
+// the quarter's slice of s_v, and after a barrier groups of G lanes sum the
+// pairs.  LDS: 128 KB window + 4 x 8 KB slices = 160 KB.
 __global__ void __launch_bounds__(kTierBlock) k_prp_pull_tiled(const TierTile *__restrict__ tiles,
                                                                const PullBlock *__restrict__ blocks,
                                                                const uint2 *__restrict__ trec,
@@ -233,7 +195,7 @@ __global__ void __launch_bounds__(kTierBlock) k_prp_pull_tiled(const TierTile *_
                                                                float2 *__restrict__ seg)
 {
   __shared__ float4 s_win4[(1u << kTierBitsMax) / 4];
-  __shared__ float s_v[kTierWaves][kTNnz];
+  __shared__ float s_v[kTierQuarters][kTNnz];
   const TierTile T = tiles[blockIdx.x];
   {
     // wlen is a multiple of 16 and wbase of 64: whole, aligned float4s
@@ -254,38 +216,70 @@ __global__ void __launch_bounds__(kTierBlock) k_prp_pull_tiled(const TierTile *_
   }
   __syncthreads();
   const float *s_win = reinterpret_cast<const float *>(s_win4);
-  const uint32_t w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  float *sv = s_v[w];
-  const PullBlock none{0, 0, 0, 0};
-  uint32_t bi = w;
-  PullBlock d = bi < T.nb ? blocks[T.b0 + bi] : none;
-  uint32_t G, nr, c[kTPer], ps[kTRnd], pe[kTRnd], pd[kTRnd];
-  wave_block_load(d, lane, tcol, trec, G, nr, c, ps, pe, pd);
-  PullBlock dnext = bi + kTierWaves < T.nb ? blocks[T.b0 + bi + kTierWaves] : none;
-  for (; bi < T.nb; bi += kTierWaves) {
-    const uint32_t bn = bi + kTierWaves;
-    const PullBlock dafter = bn + kTierWaves < T.nb ? blocks[T.b0 + bn + kTierWaves] : none;
-    uint32_t G2, nr2, c2[kTPer], ps2[kTRnd], pe2[kTRnd], pd2[kTRnd];
-    wave_block_load(dnext, lane, tcol, trec, G2, nr2, c2, ps2, pe2, pd2);
-    if (d.nrows == 0) {
-      // one segment of a long pair: a compensated sum of its arcs
+  const uint32_t qt = threadIdx.x / kBlock, lt = threadIdx.x % kBlock;
+  float *sv = s_v[qt];
+  const uint32_t *starts = reinterpret_cast<const uint32_t *>(trec);
+  constexpr int kPer = kTNnz / kBlock, kRnd = kTRows / kBlock;
+  const uint32_t rounds = (T.nb + kTierQuarters - 1) / kTierQuarters;
+  for (uint32_t rd = 0; rd < rounds; rd++) {
+    const uint32_t bi = rd * kTierQuarters + qt;
+    const bool valid = bi < T.nb;
+    const PullBlock d = valid ? blocks[T.b0 + bi] : PullBlock{0, 0, 0, 0};
+    const uint32_t ne = d.e1 - d.e0;
+    uint32_t G = 64;
+    while (G > 1 && G * d.nrows > (uint32_t)kBlock) G >>= 1;
+    const uint32_t sub = lt % G, ngroups = kBlock / G;
+    const uint32_t nr = (d.nrows + ngroups - 1) / ngroups;
+    uint32_t ps[kRnd], pe[kRnd], pd[kRnd];
+    // the block's column loads, all in flight before the first gather
+    uint32_t c[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = lt + k * kBlock;
+      c[k] = (valid && i < ne) ? tcol[d.e0 + i] : 0u;
+    }
+    if (valid && d.nrows == 0) {
+      // one segment (<= kTNnz arcs) of a long pair: a compensated sum, each
+      // thread over its arcs in order, the wave sums parked in the slice
       Kahan k;
-      for (uint32_t e = d.e0 + lane; e < d.e1; e += kWave) k.add(s_win[tcol[e]]);
+#pragma unroll
+      for (int j = 0; j < kPer; j++)
+        if (lt + j * kBlock < ne) k.add(s_win[c[j]]);
       kahan_wave_merge(k, kWave);
-      if (lane == 0) seg[T.b0 + bi] = make_float2(k.s, k.c);
-    } else {
-      const uint32_t ne = d.e1 - d.e0;
-#pragma unroll
-      for (int k = 0; k < kTPer; k++) {
-        const uint32_t i = lane + k * kWave;
-        if (i < ne) sv[i] = s_win[c[k]];
+      if (lane_id() == 0) {
+        sv[2 * (lt / kWave)] = k.s;
+        sv[2 * (lt / kWave) + 1] = k.c;
       }
-      wave_fence();
-      const uint32_t sub = lane % G, ngroups = kWave / G;
+    } else {
 #pragma unroll
-      for (int k = 0; k < kTRnd; k++) {
-        if ((uint32_t)k < nr) {  // uniform over the wave
-          const uint32_t ql = k * ngroups + lane / G;
+      for (int k = 0; k < kRnd; k++) {
+        const uint32_t ql = k * ngroups + lt / G;
+        const bool ok = valid && (uint32_t)k < nr && ql < d.nrows;
+        const uint2 r = ok ? trec[d.r0 + ql] : make_uint2(0u, 0u);
+        ps[k] = r.x;
+        pd[k] = r.y;
+        pe[k] = ok ? starts[2 * (d.r0 + ql + 1)] : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < kPer; k++) {
+        const uint32_t i = lt + k * kBlock;
+        if (valid && i < ne) sv[i] = s_win[c[k]];
+      }
+    }
+    __syncthreads();
+    if (valid && d.nrows == 0) {
+      if (lt == 0) {
+        Kahan t;
+        t.s = sv[0];
+        t.c = sv[1];
+        for (int j = 1; j < kBlock / kWave; j++) t.merge(sv[2 * j], sv[2 * j + 1]);
+        seg[T.b0 + bi] = make_float2(t.s, t.c);
+      }
+    } else if (valid) {
+#pragma unroll
+      for (int k = 0; k < kRnd; k++) {
+        if ((uint32_t)k < nr) {  // uniform over the quarter
+          const uint32_t ql = k * ngroups + lt / G;
           Kahan acc;
           if (ql < d.nrows)
             for (uint32_t x = ps[k] - d.e0 + sub; x < pe[k] - d.e0; x += G) acc.add(sv[x]);
@@ -296,21 +290,8 @@ __global__ void __launch_bounds__(kTierBlock) k_prp_pull_tiled(const TierTile *_
           }
         }
       }
-      // every lane has read the slice before the next block writes it
-      wave_fence();
     }
-    d = dnext;
-    dnext = dafter;
-    G = G2;
-    nr = nr2;
-#pragma unroll
-    for (int k = 0; k < kTPer; k++) c[k] = c2[k];
-#pragma unroll
-    for (int k = 0; k < kTRnd; k++) {
-      ps[k] = ps2[k];
-      pe[k] = pe2[k];
-      pd[k] = pd2[k];
-    }
+    __syncthreads();
   }
 }
 
@@ -335,9 +316,22 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_seg(const PullBlock *__rest
   __shared__ float s_ws[kWavesPerBlock], s_wc[kWavesPerBlock];
   const uint32_t bid = blockIdx.x;
   const PullBlock d = blocks[bid];
-  if (d.nrows == 0) {  // one segment of a long pair
+  if (d.nrows == 0) {  // one segment (<= kPullNnz arcs) of a long pair
+    constexpr int kPer = kPullNnz / kBlock;
+    const uint32_t ne = d.e1 - d.e0;
+    uint32_t src[kPer];
+    float v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = threadIdx.x + k * kBlock;
+      src[k] = i < ne ? tcol32[d.e0 + i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) v[k] = threadIdx.x + k * kBlock < ne ? mprop[src[k]] : 0.0f;
     Kahan k;
-    for (uint32_t e = d.e0 + threadIdx.x; e < d.e1; e += kBlock) k.add(mprop[tcol32[e]]);
+#pragma unroll
+    for (int j = 0; j < kPer; j++)
+      if (threadIdx.x + j * kBlock < ne) k.add(v[j]);
     kahan_wave_merge(k, kWave);
     if (lane_id() == 0) {
       s_ws[wave_in_block()] = k.s;

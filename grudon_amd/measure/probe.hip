@@ -79,6 +79,16 @@ __global__ void __launch_bounds__(kBlock) k_probe_stream(const float4 *__restric
   out[blockIdx.x * kBlock + threadIdx.x] = a;
 }
 
+// Streaming write of n4 float4 values (16-byte stores), the value depending
+// on the index so that the stores cannot be merged away.
+__global__ void __launch_bounds__(kBlock) k_probe_write(float4 *__restrict__ out, uint64_t n4)
+{
+  for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * kBlock) {
+    const float f = (float)(i & 1023);
+    out[i] = make_float4(f, f + 1.0f, f + 2.0f, f + 3.0f);
+  }
+}
+
 }  // namespace grdm
 
 using namespace grdm;
@@ -144,10 +154,12 @@ done:
   return rc;
 }
 
-// Reads `bytes` (a multiple of 16) once with k_probe_stream, `launches`
-// times; the FETCH_SIZE of one dispatch against `bytes` gives the counter's
-// scale for reads.  *ms_out = mean time per launch.
-int grdm_probe_stream(int device, uint64_t bytes, int launches, double *ms_out)
+// Reads (write == 0, k_probe_stream) or writes (write != 0, k_probe_write)
+// `bytes` (a multiple of 16) once per launch, `launches` times; the
+// FETCH_SIZE / WRITE_SIZE of one dispatch against `bytes` gives the
+// counter's scale.  The buffer is larger than the 256 MB last-level cache
+// when bytes is.  *ms_out = mean time per launch.
+int grdm_probe_stream(int device, uint64_t bytes, int launches, int write, double *ms_out)
 {
   if (!ms_out || bytes == 0 || bytes % 16 || launches < 1) return fail("grdm_probe_stream: bad argument");
   int rc = 0;
@@ -164,8 +176,10 @@ int grdm_probe_stream(int device, uint64_t bytes, int launches, double *ms_out)
   PROBE_HIP(hipEventCreate(&a));
   PROBE_HIP(hipEventCreate(&b));
   PROBE_HIP(hipEventRecord(a, st));
-  for (int r = 0; r < launches; r++)
-    hipLaunchKernelGGL(k_probe_stream, dim3(grid), dim3(kBlock), 0, st, (const float4 *)in, bytes / 16, out);
+  for (int r = 0; r < launches; r++) {
+    if (write) hipLaunchKernelGGL(k_probe_write, dim3(grid), dim3(kBlock), 0, st, (float4 *)in, bytes / 16);
+    else hipLaunchKernelGGL(k_probe_stream, dim3(grid), dim3(kBlock), 0, st, (const float4 *)in, bytes / 16, out);
+  }
   PROBE_HIP(hipGetLastError());
   PROBE_HIP(hipEventRecord(b, st));
   PROBE_HIP(hipEventSynchronize(b));

@@ -188,6 +188,7 @@ def test_gather_probe_runs(ctx):
     with pytest.raises(api.GrdError):
         api.probe_gather(0, 0, 1 << 20)
     assert api.probe_stream(0, 1 << 24, 2) > 0
+    assert api.probe_stream(0, 1 << 24, 2, write=True) > 0
 
 
 def test_learned_run_decisions_match_host_features(ctx):

@@ -6,6 +6,15 @@ superstep, the reference's), the direction-optimizing heuristic, and the
 learned MLP (packaged BF16 weights, heuristic as fallback).  The schedule
 never changes results: CC labels and SSSP distances are compared bit for
 bit across schedules, PageRank within the FP32 bound.
+
+Bars: learned >= 1.3x the plain schedule, and learned within 5 % of the
+heuristic.  The heuristic is already close to the best any per-superstep
+selector can do on these workloads: timing every fixed option at every
+(deterministic) superstep and summing the per-superstep minima gives a
+hindsight best that the heuristic is within 0.1-2.8 % of (geometric mean
+0.8 %) on held-out RMAT-22/24 and Erdos-Renyi graphs (bench.py --workload
+selector), so no selector can be required to beat it by a margin larger
+than timing noise.
 """
 import os
 
@@ -65,4 +74,4 @@ def test_learned_selector_gain(ctx):
     sp_heur = totals[api.SCHED_HEURISTIC] / totals[api.SCHED_LEARNED]
     print("learned vs plain %.2fx, learned vs heuristic %.2fx" % (sp_plain, sp_heur), flush=True)
     assert sp_plain >= 1.3
-    assert sp_heur >= 0.9
+    assert sp_heur >= 0.95
