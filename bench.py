@@ -856,7 +856,7 @@ def main():
                     help="cc/sssp/pr: the heuristic never runs a push superstep as edge shipping")
     ap.add_argument("--workload", default="headline", choices=["headline", "pr_pull", "cc", "sssp", "pr", "batch",
                                                                    "learned", "schedules", "selector"])
-    ap.add_argument("--pull", default="plain", choices=["plain", "tiled", "segmented"],
+    ap.add_argument("--pull", default="plain", choices=["plain", "tiled", "segmented", "hotcold"],
                     help="pull PageRank: the row-block pull (default), LDS tiles or L2 segments")
     ap.add_argument("--tier-bits", type=int, default=0,
                     help="tiled / segmented pull: 2^bits sources per tier (0 = 15 / 19)")
@@ -895,7 +895,8 @@ def main():
     from grudon_amd import api
 
     ctx = api.Context(0)
-    ctx.set_pull_tiling({"plain": 0, "tiled": 1, "segmented": 2}[args.pull], args.tier_bits, args.tile_arcs)
+    ctx.set_pull_tiling({"plain": 0, "tiled": 1, "segmented": 2, "hotcold": 3}[args.pull], args.tier_bits,
+                        args.tile_arcs)
     run = {"headline": run_headline, "pr_pull": run_pagerank, "pr": run_pagerank, "cc": run_cc, "sssp": run_sssp, "batch": run_batch,
            "learned": run_learned, "schedules": run_schedules, "selector": run_selector}[args.workload]
     out = run(api, ctx, args)

@@ -287,8 +287,9 @@ grd_status grd_destroy(grd_ctx *c)
 grd_status grd_set_pull_tiling(grd_ctx *c, int mode, uint32_t tier_bits, uint32_t tile_arcs)
 {
   if (!c) return fail(GRD_ERR_ARG, "grd_set_pull_tiling: NULL context");
-  if (mode < 0 || mode > 2) return fail(GRD_ERR_ARG, "grd_set_pull_tiling: mode is 0, 1 or 2");
+  if (mode < 0 || mode > 3) return fail(GRD_ERR_ARG, "grd_set_pull_tiling: mode is 0, 1, 2 or 3");
   const uint32_t hi = mode == 2 ? kSegBitsMax : kTierBitsMax;
+  if (mode == 3 && tile_arcs) return fail(GRD_ERR_ARG, "grd_set_pull_tiling: mode 3 takes no tile size (pass 0)");
   if (tier_bits && (tier_bits < 4 || tier_bits > hi))
     return fail(GRD_ERR_ARG, "grd_set_pull_tiling: tier bits must be 0 (default) or 4.." + std::to_string(hi));
   c->pull_mode = mode;

@@ -637,6 +637,7 @@ grd_status state_alloc(grd_state *s)
   if (s->tiled) {
     GRD_TRY(m.alloc_n(&s->tpart, std::max<uint64_t>(g->n_pairs, 1)));
     GRD_TRY(m.alloc_n(&s->tseg, std::max<uint32_t>(g->n_tblk, 1)));
+    if (g->tier_mode == 3) GRD_TRY(m.alloc_n(&s->cseg, std::max<uint32_t>(g->n_cblk, 1)));
   }
   GRD_TRY(m.alloc_n(&s->cnt, 1));
   GRD_HIP(hipHostMalloc((void **)&s->hcnt, sizeof(Counters), hipHostMallocDefault));

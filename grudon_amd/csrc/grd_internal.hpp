@@ -112,6 +112,7 @@ constexpr uint32_t kTierBitsMax = 15;    // tiled pull: largest tier, 2^15 sourc
 constexpr uint32_t kTierTileArcs = 65536;  // tiled pull: default arcs per tile
 constexpr uint32_t kSegBitsMax = 24;       // segmented pull: largest segment, 2^24 sources
 constexpr uint32_t kSegBits = 19;          // segmented pull: default segment, 2^19 sources (2 MB of values)
+constexpr uint32_t kHotWin = 32768;        // hot / cold pull: hottest sources of a partition in LDS (128 KB)
 
 }  // namespace grd
 
@@ -193,6 +194,21 @@ struct grd_graph {
   uint32_t n_ttile = 0;
   uint32_t *twide = nullptr;       // rows with many pairs, merged by a wave each
   uint32_t n_twide = 0;
+  // mode 3, the hot / cold split: a row's arcs from its partition's kHotWin
+  // hottest sources (a prefix of the row) are summed from an LDS window into
+  // a per-row partial; its other arcs are pulled by the row-block kernel,
+  // which merges that partial.  Hot arcs reuse tcol (16-bit window offsets),
+  // tblk / tlong (their wave blocks and long rows) and ttile (each hot
+  // workgroup's range of blocks, all of one partition); n_pairs = P (one
+  // partial per row)
+  uint32_t *hrp = nullptr;          // [P+1] hot arcs of each row
+  uint32_t *corp = nullptr;         // [P+1] cold arcs of each row
+  uint32_t *cocol = nullptr;        // [cold arcs] source proxies
+  grd::PullBlock *cblk = nullptr;   // row blocks over the cold arcs
+  uint32_t *cblk_hot = nullptr;     // per cold block: its LDS window base (P: none)
+  uint32_t n_cblk = 0;
+  grd::PullLong *clong = nullptr;
+  uint32_t n_clong = 0;
   grd::DevMem tmem;
 };
 
@@ -237,6 +253,7 @@ struct grd_state {
   bool tiled = false;       // PR_PULL: the tiled pull runs (else the plain row-block pull)
   float2 *tpart = nullptr;  // tiled pull: Kahan partial per pair, then per tier-block segment
   float2 *tseg = nullptr;
+  float2 *cseg = nullptr;   // hot / cold pull: partial per cold row block (long-row segments)
   double *red = nullptr;    // PR pull update: (residual, dangling) per workgroup
   // edge shipping wire (grown on demand): record lengths / offsets, the first
   // record of every tile, the records (gids) and their edge weights
@@ -299,6 +316,13 @@ void take_frontier_parts(grd_state *s);
 grd_status build_ship_tables(grd_graph *g);
 grd_status op_frontier_parts(grd_state *s);  // per-partition frontier counts into the counters (C > 1)
 grd_status op_ship(grd_state *s);            // edge shipping: memory send + compute apply
+// pull.hip: the row-block PageRank pull over any CSC cut into PullBlocks, and its
+// long-row fix-up; `pre` (or null) holds a Kahan partial per row merged first
+grd_status prp_pull_rows(hipStream_t st, const PullBlock *blocks, uint32_t nblk, const uint32_t *hot_base,
+                         const uint32_t *rptr, const uint32_t *col, const float *mprop, uint32_t P, float *out,
+                         float2 *partial, const float2 *pre);
+grd_status prp_pull_rows_fixup(hipStream_t st, const PullLong *longs, uint32_t nlong, const float2 *partial,
+                               float *out, const float2 *pre);
 // tiers.hip
 grd_status build_tiers(grd_graph *g);
 grd_status prp_pull_tiled(grd_state *s);  // pull + long-pair fix-up + merge into valR

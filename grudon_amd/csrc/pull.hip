@@ -41,16 +41,40 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_rb(const PullBlock *__restr
                                                         const uint32_t *__restrict__ crp,
                                                         const uint32_t *__restrict__ ccol,
                                                         const float *__restrict__ mprop, uint32_t P,
-                                                        float *__restrict__ out, float2 *__restrict__ partial)
+                                                        float *__restrict__ out, float2 *__restrict__ partial,
+                                                        const float2 *__restrict__ pre)
 {
   __shared__ float s_v[kPullNnz];
   __shared__ float s_hot[kPullHot];
   __shared__ float s_ws[kWavesPerBlock], s_wc[kWavesPerBlock];
   const uint32_t bid = blockIdx.x;
   const PullBlock d = blocks[bid];
-  if (d.nrows == 0) {  // segment of a long row
+  if (d.nrows == 0) {  // segment (<= kPullNnz arcs) of a long row
+    // as below: all column loads, then all gathers (hot sources from LDS),
+    // then the compensated sum of each thread's arcs in order
+    constexpr int kPer = kPullNnz / kBlock;
+    const uint32_t ne = d.e1 - d.e0;
+    uint32_t src[kPer];
+    float v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = threadIdx.x + k * kBlock;
+      src[k] = i < ne ? ccol[d.e0 + i] : 0u;
+    }
+    const uint32_t hb = hot_base[bid];
+    const uint32_t hn = min(kPullHot, P - hb);
+    for (uint32_t i = threadIdx.x; i < hn; i += kBlock) s_hot[i] = mprop[hb + i];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = threadIdx.x + k * kBlock;
+      const uint32_t h = src[k] - hb;
+      v[k] = i < ne ? (h < hn ? s_hot[h] : mprop[src[k]]) : 0.0f;
+    }
     Kahan k;
-    for (uint32_t e = d.e0 + threadIdx.x; e < d.e1; e += kBlock) k.add(mprop[ccol[e]]);
+#pragma unroll
+    for (int j = 0; j < kPer; j++)
+      if (threadIdx.x + j * kBlock < ne) k.add(v[j]);
     kahan_wave_merge(k, kWave);
     if (lane_id() == 0) {
       s_ws[wave_in_block()] = k.s;
@@ -112,16 +136,31 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_rb(const PullBlock *__restr
       for (uint32_t x = b + sub; x < e; x += G) acc.add(s_v[x]);
     }
     kahan_wave_merge(acc, G);
-    if (sub == 0 && rl < d.nrows) out[r] = acc.value();
+    if (sub == 0 && rl < d.nrows) {
+      if (pre) {  // the row's partial from the hot pass comes first
+        Kahan t;
+        t.s = pre[r].x;
+        t.c = pre[r].y;
+        t.merge(acc.s, acc.c);
+        out[r] = t.value();
+      } else {
+        out[r] = acc.value();
+      }
+    }
   }
 }
 
 __global__ void __launch_bounds__(kBlock) k_prp_pull_fixup(const PullLong *__restrict__ longs, uint32_t nlong,
-                                                           const float2 *__restrict__ partial, float *__restrict__ out)
+                                                           const float2 *__restrict__ partial, float *__restrict__ out,
+                                                           const float2 *__restrict__ pre)
 {
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nlong; i += gridDim.x * kBlock) {
     PullLong L = longs[i];
     Kahan t;
+    if (pre) {
+      t.s = pre[L.row].x;
+      t.c = pre[L.row].y;
+    }
     for (uint32_t j = 0; j < L.nseg; j++) {
       float2 p = partial[L.first + j];
       t.merge(p.x, p.y);
@@ -135,14 +174,6 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_fixup(const PullLong *__res
 // The broadcast receive of a pull superstep writes every own proxy: active
 // ones with their value, inactive ones with the combine identity (Op::edge
 // keeps the identity), so one gather per arc gives the arc's contribution.
-
-template <class Op>
-__device__ __forceinline__ typename Op::V pull_val(uint32_t e, const uint32_t *__restrict__ ccol,
-                                                   const uint32_t *__restrict__ cw,
-                                                   const typename Op::V *__restrict__ mprop)
-{
-  return Op::edge(mprop[ccol[e]], cw ? cw[e] : 1u);
-}
 
 // apply the pulled value to the proxy's running update; true = dirty
 template <class Op>
@@ -180,10 +211,29 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
   __shared__ unsigned long long s_dirty[kDirtyWords];
   const uint32_t bid = blockIdx.x;
   const PullBlock d = blocks[bid];
-  if (d.nrows == 0) {
+  if (d.nrows == 0) {  // segment (<= kPullNnz arcs) of a long row
+    // column ids (and weights), then the gathers (hot sources from LDS),
+    // then each thread combines its arcs in order
+    constexpr int kPer = kPullNnz / kBlock;
+    const uint32_t ne = d.e1 - d.e0;
+    uint32_t src[kPer], wt[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = threadIdx.x + k * kBlock;
+      src[k] = i < ne ? ccol[d.e0 + i] : 0u;
+      wt[k] = (cw && i < ne) ? cw[d.e0 + i] : 1u;
+    }
+    const uint32_t hb = hot_base[bid];
+    const uint32_t hn = min(kPullHot, P - hb);
+    for (uint32_t i = threadIdx.x; i < hn; i += kBlock) s_hot[i] = mprop[hb + i];
+    __syncthreads();
     V acc = op_identity<Op>();
-    for (uint32_t e = d.e0 + threadIdx.x; e < d.e1; e += kBlock)
-      acc = op_combine<Op>(acc, pull_val<Op>(e, ccol, cw, mprop));
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = threadIdx.x + k * kBlock;
+      const uint32_t h = src[k] - hb;
+      if (i < ne) acc = op_combine<Op>(acc, Op::edge(h < hn ? s_hot[h] : mprop[src[k]], wt[k]));
+    }
     if constexpr (Op::kMin) acc = block_min(acc);
     else acc = block_sum(acc);
     if (threadIdx.x == 0) partial[bid] = acc;
@@ -263,18 +313,34 @@ grd_status prp_pull_launch(grd_state *s)
 {
   grd_graph *g = s->g;
   hipStream_t st = g->ctx->stream;
-  hipLaunchKernelGGL(k_prp_pull_rb, dim3(g->n_pblk), dim3(kBlock), 0, st, g->pblk, g->pblk_hot, g->crp, g->ccol,
-                     (const float *)s->mprop, g->P, (float *)s->valR, (float2 *)s->partial);
-  GRD_HIP(hipGetLastError());
-  return GRD_OK;
+  return prp_pull_rows(st, g->pblk, g->n_pblk, g->pblk_hot, g->crp, g->ccol, (const float *)s->mprop, g->P,
+                       (float *)s->valR, (float2 *)s->partial, nullptr);
 }
 
 grd_status prp_pull_fixup(grd_state *s)
 {
   grd_graph *g = s->g;
-  if (!g->n_plong) return GRD_OK;
-  hipLaunchKernelGGL(k_prp_pull_fixup, dim3((g->n_plong + kBlock - 1) / kBlock), dim3(kBlock), 0, g->ctx->stream,
-                     g->plong, g->n_plong, (const float2 *)s->partial, (float *)s->valR);
+  return prp_pull_rows_fixup(g->ctx->stream, g->plong, g->n_plong, (const float2 *)s->partial, (float *)s->valR,
+                             nullptr);
+}
+
+grd_status prp_pull_rows(hipStream_t st, const PullBlock *blocks, uint32_t nblk, const uint32_t *hot_base,
+                         const uint32_t *rptr, const uint32_t *col, const float *mprop, uint32_t P, float *out,
+                         float2 *partial, const float2 *pre)
+{
+  if (nblk)
+    hipLaunchKernelGGL(k_prp_pull_rb, dim3(nblk), dim3(kBlock), 0, st, blocks, hot_base, rptr, col, mprop, P, out,
+                       partial, pre);
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+
+grd_status prp_pull_rows_fixup(hipStream_t st, const PullLong *longs, uint32_t nlong, const float2 *partial,
+                               float *out, const float2 *pre)
+{
+  if (!nlong) return GRD_OK;
+  hipLaunchKernelGGL(k_prp_pull_fixup, dim3((nlong + kBlock - 1) / kBlock), dim3(kBlock), 0, st, longs, nlong,
+                     partial, out, pre);
   GRD_HIP(hipGetLastError());
   return GRD_OK;
 }

@@ -465,19 +465,379 @@ __global__ void __launch_bounds__(kBlock) k_prp_tier_merge_wide(const uint32_t *
   }
 }
 
+
+// ---- mode 3: the hot / cold split ----
+//
+// A row's arcs from its memory partition's 2^b hottest sources (the first
+// proxies of the partition, in out-degree order; b <= 15) are a prefix of
+// the row, since rows keep their sources ascending.  1024-thread workgroups,
+// each with one partition's window of those sources in LDS, sum the
+// prefixes into one Kahan partial per row, written in row order.  The rest
+// of each row goes to the row-block kernel over a CSC of the remaining arcs,
+// which merges the row's partial first.
+
+// hot and cold arc counts of every row: the first source at or beyond the
+// partition's window end, by binary search over the sorted row
+__global__ void k_h_split(uint32_t P, const uint32_t *__restrict__ crp, const uint32_t *__restrict__ ccol,
+                          const uint32_t *__restrict__ pb, uint32_t M, uint32_t win, uint32_t *__restrict__ hlen,
+                          uint32_t *__restrict__ clen)
+{
+  TIER_LOOP(r, P)
+  {
+    const uint32_t mp = part_of(pb, M, (uint32_t)r);
+    const uint32_t lim = pb[mp] + min(win, pb[mp + 1] - pb[mp]);
+    const uint32_t b = crp[r], e = crp[r + 1];
+    uint32_t lo = b, hi = e;
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (ccol[mid] < lim) lo = mid + 1;
+      else hi = mid;
+    }
+    hlen[r] = lo - b;
+    clen[r] = e - lo;
+  }
+}
+
+// a wave per row: its hot arcs as 16-bit window offsets, its cold arcs as
+// source proxies
+__global__ void k_h_copy(uint32_t P, const uint32_t *__restrict__ crp, const uint32_t *__restrict__ ccol,
+                         const uint32_t *__restrict__ hrp, const uint32_t *__restrict__ corp,
+                         const uint32_t *__restrict__ pb, uint32_t M, uint16_t *__restrict__ hcol,
+                         uint32_t *__restrict__ cocol)
+{
+  const uint32_t lane = lane_id();
+  const uint64_t stride = (uint64_t)gridDim.x * kWavesPerBlock;
+  for (uint64_t r = (uint64_t)blockIdx.x * kWavesPerBlock + wave_in_block(); r < P; r += stride) {
+    const uint32_t base = pb[part_of(pb, M, (uint32_t)r)];
+    const uint32_t e0 = crp[r], n = crp[r + 1] - e0;
+    const uint32_t h0 = hrp[r], nh = hrp[r + 1] - h0, c0 = corp[r];
+    for (uint32_t i = lane; i < n; i += kWave) {
+      const uint32_t v = ccol[e0 + i];
+      if (i < nh) hcol[h0 + i] = (uint16_t)(v - base);
+      else cocol[c0 + (i - nh)] = v;
+    }
+  }
+}
+
+// Wave blocks of the hot prefixes: at most kWNnz arcs and kWRows rows each
+// (a row with more hot arcs is cut into segments of kWNnz arcs).  Workgroup
+// w owns the blocks [T.b0, T.b0 + T.nb) of one partition (T = ranges[w]):
+// it stages that partition's window once, then its 16 waves take the blocks
+// one at a time from a counter in LDS and work on their own -- column loads
+// and row bounds, gathers from the window into the wave's LDS slice, a group
+// of G lanes per row -- with no workgroup barrier, so that the waves hide
+// each other's latency.  LDS: 128 KB window + 16 slices of 504 values + the
+// block counter.
+constexpr uint32_t kWNnz = 504;   // arcs per wave block (at most 8 per lane; 8 slots short of 512 leave
+                                  // LDS room for the block counter)
+constexpr uint32_t kWRows = 256;  // rows per wave block (at most 4 per lane)
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+__global__ void __launch_bounds__(kTierBlock) k_prp_pull_hot(const TierTile *__restrict__ ranges,
+                                                             const PullBlock *__restrict__ blocks,
+                                                             const uint32_t *__restrict__ hrp,
+                                                             const uint16_t *__restrict__ hcol,
+                                                             const float *__restrict__ mprop,
+                                                             float2 *__restrict__ hpart, float2 *__restrict__ hseg)
+{
+  constexpr uint32_t kWaves = kTierBlock / kWave;
+  constexpr int kPer = (kWNnz + kWave - 1) / kWave, kRows = kWRows / kWave, kStage = kHotWin / 4 / kTierBlock;
+  __shared__ float4 s_win4[kHotWin / 4];
+  __shared__ float s_v[kWaves][kWNnz];
+  __shared__ uint32_t s_next;  // the range's next block not yet taken by a wave
+  const TierTile T = ranges[blockIdx.x];
+  if (threadIdx.x == 0) s_next = T.b0 + kWaves;
+  {
+    // wlen is a multiple of 4 and wbase of 64: whole, aligned float4s
+    const float4 *__restrict__ src = reinterpret_cast<const float4 *>(mprop + T.wbase);
+    const uint32_t n4 = T.wlen >> 2;
+    float4 x[kStage];
+#pragma unroll
+    for (int k = 0; k < kStage; k++) {
+      const uint32_t i = threadIdx.x + k * kTierBlock;
+      x[k] = src[i < n4 ? i : 0];  // all loads in flight, then the stores
+    }
+    // every slot is written (those past the window with a value no arc
+    // reads), so that no store waits on a branch
+#pragma unroll
+    for (int k = 0; k < kStage; k++) s_win4[threadIdx.x + k * kTierBlock] = x[k];
+  }
+  __syncthreads();
+  const float *s_win = reinterpret_cast<const float *>(s_win4);
+  // the wave index made uniform, so that block descriptors are scalar loads
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = lane_id();
+  float *sv = s_v[w];
+  const uint32_t bend = T.b0 + T.nb;
+  // the global loads of a block: its columns and the lanes' row bounds
+  auto issue = [&](const PullBlock &d, uint32_t (&c)[kPer], uint32_t (&rb)[kRows], uint32_t (&re)[kRows]) {
+    const uint32_t ne = d.e1 - d.e0;
+    uint32_t G = 64;
+    while (G > 1 && G * d.nrows > (uint32_t)kWave) G >>= 1;
+    const uint32_t ngroups = kWave / G;
+    const uint32_t nr = (d.nrows + ngroups - 1) / ngroups;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = lane + k * kWave;
+      c[k] = hcol[d.e0 + (i < ne ? i : 0)];
+    }
+#pragma unroll
+    for (int k = 0; k < kRows; k++) {
+      const uint32_t rl = k * ngroups + lane / G;
+      const uint32_t r = d.r0 + ((uint32_t)k < nr && rl < d.nrows ? rl : 0);
+      rb[k] = hrp[r];
+      re[k] = hrp[r + 1];
+    }
+  };
+  // a wave takes its next block from the range's counter when it starts the
+  // current one (blocks are independent, so the order does not change any
+  // result), and that block's loads are in flight while the current block
+  // is gathered, summed and stored
+  auto take = [&]() {
+    uint32_t x = 0;
+    if (lane == 0) x = atomicAdd(&s_next, 1u);
+    return __builtin_amdgcn_readfirstlane(x);
+  };
+  uint32_t b = T.b0 + w;
+  PullBlock d = b < bend ? blocks[b] : PullBlock{0, 0, 0, 0};
+  uint32_t c[kPer], rb[kRows], re[kRows];
+  issue(d, c, rb, re);
+  while (b < bend) {
+    const uint32_t bn = take();
+    const PullBlock dn = bn < bend ? blocks[bn] : PullBlock{0, 0, 0, 0};
+    const uint32_t ne = d.e1 - d.e0;
+    float v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) v[k] = lane + k * kWave < ne ? s_win[c[k]] : 0.0f;
+    uint32_t cn[kPer], rbn[kRows], ren[kRows];
+    issue(dn, cn, rbn, ren);
+    if (d.nrows == 0) {
+      // a segment of a long row: each lane over its arcs in order, then the
+      // wave's lanes merged by a fixed butterfly
+      Kahan k;
+#pragma unroll
+      for (int j = 0; j < kPer; j++)
+        if (lane + j * kWave < ne) k.add(v[j]);
+      kahan_wave_merge(k, kWave);
+      if (lane == 0) hseg[b] = make_float2(k.s, k.c);
+    } else {
+#pragma unroll
+      for (int k = 0; k < kPer; k++) {
+        const uint32_t i = lane + k * kWave;
+        if (i < ne) sv[i] = v[k];
+      }
+      wave_lds_sync();
+      uint32_t G = 64;
+      while (G > 1 && G * d.nrows > (uint32_t)kWave) G >>= 1;
+      const uint32_t sub = lane % G, ngroups = kWave / G;
+      const uint32_t nr = (d.nrows + ngroups - 1) / ngroups;
+#pragma unroll
+      for (int k = 0; k < kRows; k++) {
+        if ((uint32_t)k < nr) {  // uniform over the wave
+          const uint32_t rl = k * ngroups + lane / G;
+          Kahan acc;
+          if (rl < d.nrows)
+            for (uint32_t x = rb[k] - d.e0 + sub; x < re[k] - d.e0; x += G) acc.add(sv[x]);
+          kahan_wave_merge(acc, (int)G);
+          if (sub == 0 && rl < d.nrows) hpart[d.r0 + rl] = make_float2(acc.s, acc.c);
+        }
+      }
+      wave_lds_sync();  // the slice is rewritten by the next block
+    }
+    b = bn;
+    d = dn;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) c[k] = cn[k];
+#pragma unroll
+    for (int k = 0; k < kRows; k++) {
+      rb[k] = rbn[k];
+      re[k] = ren[k];
+    }
+  }
+}
+
+// long hot prefixes: a wave each, segments merged in a fixed order
+__global__ void __launch_bounds__(kBlock) k_prp_hot_fixup(const PullLong *__restrict__ longs, uint32_t nlong,
+                                                          const float2 *__restrict__ hseg, float2 *__restrict__ hpart)
+{
+  const uint32_t lane = lane_id();
+  for (uint32_t i = blockIdx.x * kWavesPerBlock + wave_in_block(); i < nlong; i += gridDim.x * kWavesPerBlock) {
+    const PullLong L = longs[i];
+    Kahan t;
+    for (uint32_t j = lane; j < L.nseg; j += kWave) {
+      const float2 p = hseg[L.first + j];
+      t.merge(p.x, p.y);
+    }
+    kahan_wave_merge(t, kWave);
+    if (lane == 0) hpart[L.row] = make_float2(t.s, t.c);
+  }
+}
+
 }  // namespace
+
+
+// mode 3: split every row at its partition's window end on the device; the
+// cut into row blocks, rounds and workgroup ranges runs on the host
+static grd_status build_hotcold(grd_graph *g)
+{
+  grd_ctx *ctx = g->ctx;
+  hipStream_t st = ctx->stream;
+  const uint32_t P = g->P, M = (uint32_t)g->M, C = (uint32_t)g->C;
+  const uint32_t tb = ctx->tier_bits ? ctx->tier_bits : kTierBitsMax;
+  if (tb < 4 || tb > kTierBitsMax) return fail(GRD_ERR_ARG, "hot / cold pull: window bits out of range");
+  const uint32_t win = 1u << tb;  // <= kHotWin
+  std::vector<uint32_t> pb(M + 1);
+  for (uint32_t k = 0; k < M; k++) pb[k] = g->obase[(size_t)k * C];
+  pb[M] = P;
+  DevMem tmp;
+  uint32_t *d_pb = nullptr, *hlen = nullptr, *clen = nullptr;
+  GRD_TRY(t_upload(tmp, st, &d_pb, pb));
+  GRD_TRY(tmp.alloc_n(&hlen, (uint64_t)P + 1));
+  GRD_TRY(tmp.alloc_n(&clen, (uint64_t)P + 1));
+  hipLaunchKernelGGL(k_t_fill, dim3(1), dim3(kBlock), 0, st, hlen + P, 1, 0u);
+  hipLaunchKernelGGL(k_t_fill, dim3(1), dim3(kBlock), 0, st, clen + P, 1, 0u);
+  if (P) hipLaunchKernelGGL(k_h_split, dim3(tgrid(P)), dim3(kBlock), 0, st, P, g->crp, g->ccol, d_pb, M, win, hlen, clen);
+  GRD_HIP(hipGetLastError());
+  DevMem &gm = g->tmem;
+  GRD_TRY(gm.alloc_n(&g->hrp, (uint64_t)P + 1));
+  GRD_TRY(gm.alloc_n(&g->corp, (uint64_t)P + 1));
+  GRD_TRY(t_scan(tmp, st, hlen, g->hrp, (uint64_t)P + 1));
+  GRD_TRY(t_scan(tmp, st, clen, g->corp, (uint64_t)P + 1));
+  std::vector<uint32_t> hrp((size_t)P + 1), corp((size_t)P + 1);
+  GRD_HIP(hipMemcpyAsync(hrp.data(), g->hrp, ((size_t)P + 1) * 4, hipMemcpyDeviceToHost, st));
+  GRD_HIP(hipMemcpyAsync(corp.data(), g->corp, ((size_t)P + 1) * 4, hipMemcpyDeviceToHost, st));
+  GRD_HIP(hipStreamSynchronize(st));
+  // padded: a wave block without hot arcs still reads the column at its start
+  GRD_TRY(gm.alloc_n(&g->tcol, (uint64_t)hrp[P] + kWave));
+  GRD_TRY(gm.alloc_n(&g->cocol, std::max<uint64_t>(corp[P], 1)));
+  if (P) {
+    const uint64_t nb = std::min<uint64_t>(65535, ((uint64_t)P + kWavesPerBlock - 1) / kWavesPerBlock);
+    hipLaunchKernelGGL(k_h_copy, dim3((uint32_t)nb), dim3(kBlock), 0, st, P, g->crp, g->ccol, g->hrp, g->corp, d_pb, M,
+                       g->tcol, g->cocol);
+  }
+  GRD_HIP(hipGetLastError());
+  // hot: wave blocks per partition, then each partition's blocks split into
+  // contiguous workgroup ranges of about equal cost, the number of ranges of
+  // a partition in proportion to its cost
+  std::vector<PullBlock> hblk;
+  std::vector<PullLong> hlong;
+  std::vector<uint32_t> pblk0(M + 1, 0);
+  std::vector<uint64_t> bcost;
+  for (uint32_t mp = 0; mp < M; mp++) {
+    pblk0[mp] = (uint32_t)hblk.size();
+    uint32_t r = pb[mp];
+    while (r < pb[mp + 1]) {
+      const uint32_t len = hrp[r + 1] - hrp[r];
+      if (len > kWNnz) {
+        PullLong L{r, (uint32_t)hblk.size(), 0, 0};
+        for (uint32_t e = hrp[r]; e < hrp[r + 1]; e += kWNnz) {
+          hblk.push_back(PullBlock{r, 0, e, std::min(hrp[r + 1], e + kWNnz)});
+          L.nseg++;
+        }
+        hlong.push_back(L);
+        r++;
+        continue;
+      }
+      const uint32_t r0 = r, e0 = hrp[r];
+      while (r < pb[mp + 1] && r - r0 < kWRows && hrp[r + 1] - hrp[r] <= kWNnz && hrp[r + 1] - e0 <= kWNnz) r++;
+      hblk.push_back(PullBlock{r0, r - r0, e0, hrp[r]});
+    }
+  }
+  pblk0[M] = (uint32_t)hblk.size();
+  bcost.reserve(hblk.size());
+  uint64_t total = 0;
+  for (const PullBlock &d : hblk) {
+    bcost.push_back(64 + (d.e1 - d.e0) + 4ull * d.nrows);  // a block's fixed cost: its loads' latency
+    total += bcost.back();
+  }
+  std::vector<TierTile> ranges;
+  const uint64_t nwg = 8ull * (uint64_t)std::max(1, ctx->num_cus);  // eight per CU, in turn: dynamic balance
+  for (uint32_t mp = 0; mp < M; mp++) {
+    const uint32_t b0 = pblk0[mp], b1 = pblk0[mp + 1];
+    if (b0 == b1) continue;
+    const uint32_t wbase = pb[mp], wlen = std::min<uint32_t>(win, pb[mp + 1] - pb[mp]);
+    uint64_t cm = 0;
+    for (uint32_t i = b0; i < b1; i++) cm += bcost[i];
+    const uint64_t k = std::max<uint64_t>(1, std::min<uint64_t>(b1 - b0, cm * nwg / std::max<uint64_t>(total, 1)));
+    uint64_t acc = 0;
+    uint32_t start = b0, j = 1;
+    for (uint32_t i = b0; i < b1 && j < k; i++) {
+      acc += bcost[i];
+      if (acc * k >= cm * j) {
+        ranges.push_back(TierTile{start, i + 1 - start, wbase, wlen});
+        start = i + 1;
+        j++;
+      }
+    }
+    if (start < b1) ranges.push_back(TierTile{start, b1 - start, wbase, wlen});
+  }
+  // cold: the row-block pull's cut over the remaining arcs, without the
+  // kernel's small LDS window (a window base of P is empty): the next 2048
+  // sources after the hot window draw too few arcs to pay for staging them
+  std::vector<PullBlock> cblk;
+  std::vector<PullLong> clong;
+  std::vector<uint32_t> chot;
+  {
+    uint32_t r = 0;
+    auto window_after = [&](uint32_t) { return P; };
+    while (r < P) {
+      const uint32_t len = corp[r + 1] - corp[r];
+      if (len > kPullNnz) {
+        PullLong L{r, (uint32_t)cblk.size(), 0, 0};
+        for (uint32_t e = corp[r]; e < corp[r + 1]; e += kPullNnz) {
+          cblk.push_back(PullBlock{r, 0, e, std::min(corp[r + 1], e + kPullNnz)});
+          chot.push_back(window_after(r));
+          L.nseg++;
+        }
+        clong.push_back(L);
+        r++;
+        continue;
+      }
+      const uint32_t r0 = r, e0 = corp[r];
+      while (r < P && r - r0 < kPullMaxRows && corp[r + 1] - corp[r] <= kPullNnz && corp[r + 1] - e0 <= kPullNnz) r++;
+      cblk.push_back(PullBlock{r0, r - r0, e0, corp[r]});
+      chot.push_back(window_after(r0));
+    }
+  }
+  if (hblk.size() >= 0x7FFFFFFFull || cblk.size() >= 0x7FFFFFFFull)
+    return fail(GRD_ERR_ARG, "hot / cold pull: too many row blocks");
+  GRD_TRY(t_upload(gm, st, &g->tblk, hblk));
+  GRD_TRY(t_upload(gm, st, &g->tlong, hlong));
+  GRD_TRY(t_upload(gm, st, &g->ttile, ranges));
+  GRD_TRY(t_upload(gm, st, &g->cblk, cblk));
+  GRD_TRY(t_upload(gm, st, &g->cblk_hot, chot));
+  GRD_TRY(t_upload(gm, st, &g->clong, clong));
+  GRD_HIP(hipStreamSynchronize(st));  // the host vectors and temporaries go out of scope
+  g->n_tblk = (uint32_t)hblk.size();
+  g->n_tlong = (uint32_t)hlong.size();
+  g->n_ttile = (uint32_t)ranges.size();
+  g->n_cblk = (uint32_t)cblk.size();
+  g->n_clong = (uint32_t)clong.size();
+  g->n_pairs = P;
+  g->tier_mode = 3;
+  g->tier_bits = tb;
+  g->tile_arcs = 0;
+  g->tiers_built = true;
+  return GRD_OK;
+}
 
 // Regroups the CSC by (partition, tier, row) on the device; the cut into row
 // blocks and tiles, a sequential pass over the pair lengths, runs on the host.
 grd_status build_tiers(grd_graph *g)
 {
   if (g->tiers_built) return GRD_OK;
+  if (g->ctx->pull_mode == 3) return build_hotcold(g);
   grd_ctx *ctx = g->ctx;
   hipStream_t st = ctx->stream;
   const uint32_t P = g->P, M = (uint32_t)g->M, C = (uint32_t)g->C;
   const uint64_t m = g->m;
   const int mode = ctx->pull_mode;
-  if (mode != 1 && mode != 2) return fail(GRD_ERR_ARG, "tiled pull: mode 1 (LDS tiles) or 2 (L2 segments)");
+  if (mode != 1 && mode != 2) return fail(GRD_ERR_ARG, "tiled pull: mode 1 (LDS tiles), 2 (L2 segments) or 3 (hot / cold)");
   const uint32_t tb = ctx->tier_bits ? ctx->tier_bits : (mode == 1 ? kTierBitsMax : kSegBits);
   const uint32_t tile_arcs = ctx->tile_arcs ? ctx->tile_arcs : kTierTileArcs;
   if (tb < 4 || tb > (mode == 1 ? kTierBitsMax : kSegBitsMax)) return fail(GRD_ERR_ARG, "tiled pull: tier bits out of range");
@@ -642,15 +1002,28 @@ grd_status prp_pull_tiled(grd_state *s)
   grd_graph *g = s->g;
   hipStream_t st = g->ctx->stream;
   float *out = (float *)s->valR;
+  auto wave_grid = [](uint64_t n) {
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(65535, (n + kWavesPerBlock - 1) / kWavesPerBlock));
+  };
+  if (g->tier_mode == 3) {
+    // hot prefixes from LDS into per-row partials, then the cold rest
+    if (g->n_ttile)
+      hipLaunchKernelGGL(k_prp_pull_hot, dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile, g->tblk, g->hrp,
+                         g->tcol, (const float *)s->mprop, s->tpart, s->tseg);
+    if (g->n_tlong)
+      hipLaunchKernelGGL(k_prp_hot_fixup, dim3(wave_grid(g->n_tlong)), dim3(kBlock), 0, st, g->tlong, g->n_tlong,
+                         s->tseg, s->tpart);
+    GRD_HIP(hipGetLastError());
+    GRD_TRY(prp_pull_rows(st, g->cblk, g->n_cblk, g->cblk_hot, g->corp, g->cocol, (const float *)s->mprop, g->P, out,
+                          s->cseg, s->tpart));
+    return prp_pull_rows_fixup(st, g->clong, g->n_clong, s->cseg, out, s->tpart);
+  }
   if (g->tier_mode == 1 && g->n_ttile)
     hipLaunchKernelGGL(k_prp_pull_tiled, dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile, g->tblk, g->trec,
                        g->tcol, (const float *)s->mprop, out, s->tpart, s->tseg);
   if (g->tier_mode == 2 && g->n_tblk)
     hipLaunchKernelGGL(k_prp_pull_seg, dim3(g->n_tblk), dim3(kBlock), 0, st, g->tblk, g->tblk_hot, g->trec, g->tcol32,
                        (const float *)s->mprop, g->P, out, s->tpart, s->tseg);
-  auto wave_grid = [](uint64_t n) {
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(65535, (n + kWavesPerBlock - 1) / kWavesPerBlock));
-  };
   if (g->n_tlong)
     hipLaunchKernelGGL(k_prp_tier_fixup, dim3(wave_grid(g->n_tlong)), dim3(kBlock), 0, st, g->tlong, g->n_tlong,
                        s->tseg, g->trec, out, s->tpart);

@@ -144,11 +144,15 @@ grd_status grd_set_layout(grd_ctx *ctx, int on_device);
  *     tiles of about tile_arcs arcs (0 = 65536);
  *   mode 2: L2 segments -- the same regrouping with tiers of 2^tier_bits
  *     proxies (4..24, 0 = 19: 2 MB of values, resident in L2 while their row
- *     blocks run) and the row-block pull's kernel over them.
- * Modes 1 and 2 regroup a graph when its first pull-PageRank state is
- * created, and a graph keeps the regrouping it was first given; per row, the
- * tiers' Kahan partials are merged in tier order.  All modes compute the
- * same sums within FP32 rounding. */
+ *     blocks run) and the row-block pull's kernel over them;
+ *   mode 3: hot / cold split -- each row's arcs from its partition's
+ *     2^tier_bits hottest sources (4..15, 0 = 15) summed from an LDS window
+ *     into a per-row partial, the rest of the row pulled by the row-block
+ *     kernel, which merges that partial first (tile_arcs must be 0).
+ * Modes 1-3 regroup a graph when its first pull-PageRank state is created,
+ * and a graph keeps the regrouping it was first given; per row, the
+ * partials are merged in a fixed order.  All modes compute the same sums
+ * within FP32 rounding. */
 grd_status grd_set_pull_tiling(grd_ctx *ctx, int mode, uint32_t tier_bits, uint32_t tile_arcs);
 
 /* ---- graphs ---- */

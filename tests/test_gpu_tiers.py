@@ -1,4 +1,5 @@
-"""GPU tests of the tiled pull-PageRank (tiers.hip) against the FP64 oracle.
+"""GPU tests of the source-blocked pull-PageRank modes (tiers.hip: LDS tiles,
+L2 segments, the hot / cold split) against the FP64 oracle.
 
 Small tiers and tiles make small graphs exercise what large graphs do: many
 tiers per partition, rows whose arcs span several tiers (partials merged in
@@ -40,6 +41,8 @@ def _solve(ctx, g, C, M, mode, tier_bits=0, tile_arcs=0, tol=1e-6):
     (1, 12, 1, 1, 4, 0), (1, 12, 1, 4, 6, 4096), (1, 13, 2, 3, 5, 2048), (1, 14, 1, 2, 8, 0),
     (1, 14, 3, 2, 10, 8192), (1, 16, 1, 1, 15, 0), (1, 16, 1, 4, 12, 0),
     (2, 12, 1, 1, 4, 0), (2, 13, 2, 3, 6, 0), (2, 14, 1, 4, 8, 0), (2, 16, 1, 1, 10, 0), (2, 16, 1, 4, 0, 0),
+    (3, 12, 1, 1, 4, 0), (3, 13, 2, 3, 6, 0), (3, 14, 1, 4, 8, 0), (3, 16, 1, 1, 10, 0), (3, 16, 1, 4, 0, 0),
+    (3, 17, 3, 2, 12, 0),
 ])
 def test_tiled_pull_fp64_bound(ctx, mode, scale, C, M, tier_bits, tile_arcs):
     _say("tiled pull", mode, scale, C, M, tier_bits, tile_arcs)
@@ -57,9 +60,9 @@ def test_tiled_equals_plain_pull_and_repeats_bit_exact(ctx):
     _say("tiled vs plain")
     g = O.rmat(15, 16, seed=3)
     rp, st_p = _solve(ctx, g, 1, 4, 0)
-    for mode, tb in ((1, 9), (2, 11)):
-        rt, st_t = _solve(ctx, g, 1, 4, mode, tb, 4096)
-        rt2, _ = _solve(ctx, g, 1, 4, mode, tb, 4096)
+    for mode, tb, ta in ((1, 9, 4096), (2, 11, 4096), (3, 8, 0)):
+        rt, st_t = _solve(ctx, g, 1, 4, mode, tb, ta)
+        rt2, _ = _solve(ctx, g, 1, 4, mode, tb, ta)
         assert np.array_equal(rt.view(np.uint32), rt2.view(np.uint32)), mode  # deterministic
         assert st_t["iterations"] == st_p["iterations"], mode
         assert float(np.abs(rt.astype(np.float64) - rp.astype(np.float64)).sum()) <= 1e-6, mode
@@ -71,10 +74,27 @@ def test_tiled_pull_er_and_sink_heavy(ctx):
     _say("tiled er / sinks")
     for g in (O.er(20000, 12.0, seed=5), O.from_edges(9, [(0, 1), (1, 2), (2, 0), (3, 4), (5, 4), (6, 4)])):
         for M in (1, 3):
-            for mode in (1, 2):
-                r, st = _solve(ctx, g, 1, M, mode, 4, 1024)
+            for mode in (1, 2, 3):
+                r, st = _solve(ctx, g, 1, M, mode, 4, 0 if mode == 3 else 1024)
                 ref, _, _ = O.pr_pull(g, iters=st["iterations"])
                 assert float(np.abs(r.astype(np.float64) - ref).sum()) <= 1e-6, (g.n, M, mode)
+
+
+def test_hot_cold_long_hot_and_cold_rows(ctx):
+    """Rows longer than a row block on both sides of the split: every vertex
+    points at 0 and 1, so with a 2^12-source window rows 0 and 1 each have
+    about 4096 hot arcs (two hot segments) and 4900 cold ones (three cold
+    segments); plus a ring so that the other rows are not empty."""
+    _say("hot/cold long rows")
+    n = 9000
+    edges = [(i, 0) for i in range(1, n)] + [(i, 1) for i in range(2, n)] + [(i, (i + 1) % n) for i in range(n)]
+    g = O.from_edges(n, edges)
+    for M in (1, 2):
+        r, st = _solve(ctx, g, 1, M, 3, 12, 0)
+        ref, _, _ = O.pr_pull(g, iters=st["iterations"])
+        assert float(np.abs(r.astype(np.float64) - ref).sum()) <= 1e-6, M
+        rp, _ = _solve(ctx, g, 1, M, 0)
+        assert float(np.abs(r.astype(np.float64) - rp.astype(np.float64)).sum()) <= 1e-6, M
 
 
 def test_pull_tiling_argument_checks(ctx):
@@ -85,6 +105,11 @@ def test_pull_tiling_argument_checks(ctx):
     with pytest.raises(api.GrdError):
         ctx.set_pull_tiling(2, 25, 0)
     with pytest.raises(api.GrdError):
-        ctx.set_pull_tiling(3, 0, 0)
+        ctx.set_pull_tiling(4, 0, 0)
+    with pytest.raises(api.GrdError):
+        ctx.set_pull_tiling(3, 0, 4096)
+    with pytest.raises(api.GrdError):
+        ctx.set_pull_tiling(3, 16, 0)
+    ctx.set_pull_tiling(3, 15, 0)
     ctx.set_pull_tiling(2, 24, 0)
     ctx.set_pull_tiling(0, 0, 0)
