@@ -62,6 +62,11 @@ sys.path.insert(1, os.path.join(ROOT, "tests"))  # the cpu_baseline leg only
 
 METRIC = "GTEPS on 1 MI355X, RMAT PR/CC/SSSP; learned-vs-plain schedule speedup"
 HBM_PEAK_GBS = 8000.0
+# the kernels of one pull step per pull mode, timed together between HIP events
+PULL_KERNELS = {"plain": "k_prp_pull_rb (+ k_prp_pull_fixup)",
+                "hotcold": "k_prp_pull_hot + k_prp_pull_rb (+ fix-ups)",
+                "tiled": "k_prp_pull_tiled + k_prp_tier_merge (+ fix-ups)",
+                "segmented": "k_prp_pull_seg + k_prp_tier_merge (+ fix-ups)"}
 INF = 0xFFFFFFFF
 
 
@@ -360,8 +365,9 @@ def run_pagerank(api, ctx, args):
     gteps = G.m * iters / secs / 1e9
     log("timed %d steps: %.3f s, %d iterations, %.2f ms/step, %.1f GTEPS"
         % (args.steps, secs, iters, secs * 1e3 / args.steps, gteps))
-    kname = "k_prp_pull" if pull else "k_bsp_traverse"
-    roof = roofline_of(kms, kl, kb, kname, traffic_from_profiles("%s_rmat%d" % (kname, args.scale)))
+    kname = PULL_KERNELS[args.pull] if pull else "k_bsp_traverse"
+    traffic = traffic_from_profiles("k_prp_pull_%s_rmat%d" % (args.pull, args.scale)) if pull else None
+    roof = roofline_of(kms, kl, kb, kname, traffic)
     # edges the traversals examined (a pull examines every arc) per second
     extra = {"work_gteps": round(last["timed_edges_examined"] / secs / 1e9, 3)}
     plan = None
@@ -609,7 +615,8 @@ def run_headline(api, ctx, args):
         api, S, args, {"max_iter": 1000, "tolerance": 1e-6}, G.m, per_iteration=True)
     pr_iters = last["iterations"]
     log("PR pull: %d steps, %.2f ms/step, %d iterations, %.1f GTEPS" % (args.steps, ms_step, pr_iters, gteps))
-    roof = roofline_of(kms, kl, kb, "k_prp_pull", traffic_from_profiles("k_prp_pull_rmat%d" % scale))
+    roof = roofline_of(kms, kl, kb, PULL_KERNELS[args.pull],
+                       traffic_from_profiles("k_prp_pull_%s_rmat%d" % (args.pull, scale)))
     extra = {"work_gteps": round(last["timed_edges_examined"] / secs / 1e9, 3)}
     plan = None
     if not args.no_check or not args.no_cpu:
@@ -856,10 +863,11 @@ def main():
                     help="cc/sssp/pr: the heuristic never runs a push superstep as edge shipping")
     ap.add_argument("--workload", default="headline", choices=["headline", "pr_pull", "cc", "sssp", "pr", "batch",
                                                                    "learned", "schedules", "selector"])
-    ap.add_argument("--pull", default="plain", choices=["plain", "tiled", "segmented", "hotcold"],
-                    help="pull PageRank: the row-block pull (default), LDS tiles or L2 segments")
+    ap.add_argument("--pull", default="hotcold", choices=["plain", "tiled", "segmented", "hotcold"],
+                    help="pull PageRank: the row-block pull, LDS tiles, L2 segments or the hot / cold split "
+                         "(default)")
     ap.add_argument("--tier-bits", type=int, default=0,
-                    help="tiled / segmented pull: 2^bits sources per tier (0 = 15 / 19)")
+                    help="tiled / segmented / hot-cold pull: 2^bits sources per tier or window (0 = 15 / 19 / 15)")
     ap.add_argument("--tile-arcs", type=int, default=0, help="tiled pull: arcs per tile (0 = 65536)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 / exactness check")

@@ -171,12 +171,12 @@ class Context:
     def set_host_threads(self, n):
         _check(lib().grd_set_host_threads(self.h, n), "grd_set_host_threads")
 
-    def set_pull_tiling(self, mode=1, tier_bits=0, tile_arcs=0):
-        """How pull PageRank states created afterwards pull: 0 (the default)
-        the row-block pull, 1 LDS tiles (tiers of 2^tier_bits sources, tiles
-        of about tile_arcs arcs), 2 L2 segments (tiers of 2^tier_bits), 3 the
-        hot / cold split (each partition's 2^tier_bits hottest sources from
-        an LDS window, default 2^15; tile_arcs must be 0)."""
+    def set_pull_tiling(self, mode=3, tier_bits=0, tile_arcs=0):
+        """How pull PageRank states created afterwards pull: 0 the row-block
+        pull, 1 LDS tiles (tiers of 2^tier_bits sources, tiles of about
+        tile_arcs arcs), 2 L2 segments (tiers of 2^tier_bits), 3 (the
+        default) the hot / cold split (each partition's 2^tier_bits hottest
+        sources from an LDS window, default 2^15; tile_arcs must be 0)."""
         _check(lib().grd_set_pull_tiling(self.h, int(mode), tier_bits, tile_arcs), "grd_set_pull_tiling")
 
     def set_layout(self, on_device):

@@ -122,7 +122,8 @@ struct grd_ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int host_threads = 0;
   int num_cus = 256;
-  int pull_mode = 0;        // pull PageRank: 0 row-block pull, 1 LDS-tiled, 2 L2-segmented (tiers.hip)
+  int pull_mode = 3;        // pull PageRank: 0 row-block pull, 1 LDS-tiled, 2 L2-segmented, 3 hot / cold
+                            // split (the default; tiers.hip)
   uint32_t tier_bits = 0;   // tunables for graphs regrouped later (0 = the mode's default)
   uint32_t tile_arcs = 0;
   bool layout_on_device = true;  // build the partitioned layout with device kernels (else on the host)

@@ -137,7 +137,7 @@ grd_status grd_set_layout(grd_ctx *ctx, int on_device);
 
 /* No counterpart in the reference: how pull-PageRank states (GRD_PR_PULL)
  * created afterwards run their pull.
- *   mode 0 (the default): the row-block pull over the CSC;
+ *   mode 0: the row-block pull over the CSC;
  *   mode 1: LDS tiles -- the CSC regrouped by (memory partition, source
  *     tier, row), a tier being 2^tier_bits consecutive proxies (4..15, 0 =
  *     15), every gather served from the tier's window staged in LDS by
@@ -148,7 +148,8 @@ grd_status grd_set_layout(grd_ctx *ctx, int on_device);
  *   mode 3: hot / cold split -- each row's arcs from its partition's
  *     2^tier_bits hottest sources (4..15, 0 = 15) summed from an LDS window
  *     into a per-row partial, the rest of the row pulled by the row-block
- *     kernel, which merges that partial first (tile_arcs must be 0).
+ *     kernel, which merges that partial first (tile_arcs must be 0); the
+ *     default, and the fastest measured.
  * Modes 1-3 regroup a graph when its first pull-PageRank state is created,
  * and a graph keeps the regrouping it was first given; per row, the
  * partials are merged in a fixed order.  All modes compute the same sums

@@ -33,7 +33,7 @@ def _solve(ctx, g, C, M, mode, tier_bits=0, tile_arcs=0, tol=1e-6):
         S.free()
         G.free()
     finally:
-        ctx.set_pull_tiling(0, 0, 0)
+        ctx.set_pull_tiling(3, 0, 0)  # the default
     return r, st
 
 
@@ -112,4 +112,4 @@ def test_pull_tiling_argument_checks(ctx):
         ctx.set_pull_tiling(3, 16, 0)
     ctx.set_pull_tiling(3, 15, 0)
     ctx.set_pull_tiling(2, 24, 0)
-    ctx.set_pull_tiling(0, 0, 0)
+    ctx.set_pull_tiling(3, 0, 0)
