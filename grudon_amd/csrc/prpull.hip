@@ -41,9 +41,19 @@ __global__ void __launch_bounds__(kBlock) k_prp_update(uint32_t nw, const uint32
     uint32_t s = w * 64 + lane_id();
     if (slot_gid[s] == 0xFFFFFFFFu) continue;
     uint32_t pw = word_dst[w];
+    // the own partial and up to three cross-mirror partials (every vertex
+    // at M <= 4) loaded together, then summed in the same order as the loop
+    // that takes any further ones
+    const uint32_t x0 = xptr[s], nx = xptr[s + 1] - x0;
+    const uint32_t i0 = nx > 0 ? xlist[x0] : 0u, i1 = nx > 1 ? xlist[x0 + 1] : 0u, i2 = nx > 2 ? xlist[x0 + 2] : 0u;
+    const float own = valR[pw * 64 + lane_id()];
+    const float v0 = nx > 0 ? valR[i0] : 0.0f, v1 = nx > 1 ? valR[i1] : 0.0f, v2 = nx > 2 ? valR[i2] : 0.0f;
     Kahan k;
-    k.add(valR[pw * 64 + lane_id()]);
-    for (uint32_t x = xptr[s]; x < xptr[s + 1]; x++) k.add(valR[xlist[x]]);
+    k.add(own);
+    if (nx > 0) k.add(v0);
+    if (nx > 1) k.add(v1);
+    if (nx > 2) k.add(v2);
+    for (uint32_t x = x0 + 3; x < x0 + nx; x++) k.add(valR[xlist[x]]);
     float rn = base + damping * k.value();
     float ro = r[s];
     res += fabs((double)rn - (double)ro);
