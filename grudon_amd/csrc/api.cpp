@@ -535,13 +535,18 @@ grd_status grd_state_create_typed(grd_graph *g, int algo, uint32_t value_bytes, 
   s->vbytes = value_bytes;
   s->wide = value_bytes == 8;
   grd_status st = GRD_OK;
-  if (algo == GRD_PR_PULL && g->ctx->pull_mode != 0) {
+  // the pull modes: modes 1-2 serve pull PageRank only, the hot / cold split
+  // (mode 3) every pull (the BSP pull of CC, SSSP and reference PageRank too)
+  const int pm = g->ctx->pull_mode;
+  if ((algo == GRD_PR_PULL && pm != 0) || pm == 3) {
     try {
       st = build_tiers(g);
     } catch (const std::bad_alloc &) {
       st = fail(GRD_ERR_OOM, "tiled pull: host allocation failed");
     }
-    s->tiled = st == GRD_OK;
+    // a graph keeps the regrouping it was given first; the BSP pull uses
+    // only the hot / cold one
+    s->tiled = st == GRD_OK && (algo == GRD_PR_PULL || g->tier_mode == 3);
   }
   if (st == GRD_OK) st = state_alloc(s);
   if (st == GRD_OK && (hipEventCreate(&s->kev[0]) != hipSuccess || hipEventCreate(&s->kev[1]) != hipSuccess))

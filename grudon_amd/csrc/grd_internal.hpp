@@ -205,6 +205,8 @@ struct grd_graph {
   uint32_t *hrp = nullptr;          // [P+1] hot arcs of each row
   uint32_t *corp = nullptr;         // [P+1] cold arcs of each row
   uint32_t *cocol = nullptr;        // [cold arcs] source proxies
+  uint32_t *hw = nullptr;           // [hot arcs] arc weights (weighted graphs), as tcol
+  uint32_t *cow = nullptr;          // [cold arcs] arc weights (weighted graphs), as cocol
   grd::PullBlock *cblk = nullptr;   // row blocks over the cold arcs
   uint32_t *cblk_hot = nullptr;     // per cold block: its LDS window base (P: none)
   uint32_t n_cblk = 0;
@@ -326,6 +328,9 @@ grd_status prp_pull_rows_fixup(hipStream_t st, const PullLong *longs, uint32_t n
                                float *out, const float2 *pre);
 // tiers.hip
 grd_status build_tiers(grd_graph *g);
+// does this state pull through the hot / cold split (the BSP pull, CC / SSSP /
+// reference PageRank, over a graph split by mode 3)?
+inline bool bsp_hot_cold(const grd_state *s) { return s->tiled && s->g->tier_mode == 3; }
 grd_status prp_pull_tiled(grd_state *s);  // pull + long-pair fix-up + merge into valR
 // prpull.hip
 grd_status prp_init(grd_state *s);

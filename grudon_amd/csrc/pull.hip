@@ -203,7 +203,8 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
                                                         const uint32_t *__restrict__ cw,
                                                         const typename Op::V *__restrict__ mprop, uint32_t P,
                                                         typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty,
-                                                        typename Op::V *__restrict__ partial)
+                                                        typename Op::V *__restrict__ partial,
+                                                        const typename Op::V *__restrict__ pre)
 {
   using V = typename Op::V;
   __shared__ V s_v[kPullNnz];
@@ -284,6 +285,7 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
       for (uint32_t x = b + sub; x < e; x += G) acc = op_combine<Op>(acc, s_v[x]);
     }
     for (int o = G / 2; o > 0; o >>= 1) acc = op_combine<Op>(acc, __shfl_xor(acc, o, kWave));
+    if (pre && sub == 0 && rl < d.nrows) acc = op_combine<Op>(pre[r], acc);  // the row's hot partial first
     if (sub == 0 && rl < d.nrows && pull_apply<Op>(upd, r, acc))
       atomicOr(&s_dirty[(r >> 6) - w0], 1ull << (r & 63));
   }
@@ -296,11 +298,12 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
 template <class Op>
 __global__ void __launch_bounds__(kBlock) k_bsp_pull_fixup(const PullLong *__restrict__ longs, uint32_t nlong,
                                                            const typename Op::V *__restrict__ partial,
-                                                           typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty)
+                                                           typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty,
+                                                           const typename Op::V *__restrict__ pre)
 {
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nlong; i += gridDim.x * kBlock) {
     PullLong L = longs[i];
-    typename Op::V acc = op_identity<Op>();
+    typename Op::V acc = pre ? pre[L.row] : op_identity<Op>();  // the row's hot partial first
     for (uint32_t j = 0; j < L.nseg; j++) acc = op_combine<Op>(acc, partial[L.first + j]);
     if (pull_apply<Op>(upd, L.row, acc))
       atomicOr((unsigned long long *)(dirty + (L.row >> 6)), 1ull << (L.row & 63));
@@ -345,13 +348,26 @@ grd_status prp_pull_rows_fixup(hipStream_t st, const PullLong *longs, uint32_t n
   return GRD_OK;
 }
 
+// A state over a graph split by the hot / cold pull (mode 3) runs the hot
+// pass first and then the row-block kernel over the cold arcs, which combines
+// each row's hot partial with its own; otherwise the row-block kernel over
+// the whole CSC.
 template <class Op>
 grd_status bsp_pull_launch(grd_state *s, const uint32_t *cw)
 {
   grd_graph *g = s->g;
   using V = typename Op::V;
-  hipLaunchKernelGGL(k_bsp_pull_rb<Op>, dim3(g->n_pblk), dim3(kBlock), 0, g->ctx->stream, g->pblk, g->pblk_hot, g->crp,
-                     g->ccol, cw, (const V *)s->mprop, g->P, (V *)s->mupd, s->Du, (V *)s->partial);
+  if (bsp_hot_cold(s)) {
+    GRD_TRY(bsp_pull_hot<Op>(s));
+    if (g->n_cblk)
+      hipLaunchKernelGGL(k_bsp_pull_rb<Op>, dim3(g->n_cblk), dim3(kBlock), 0, g->ctx->stream, g->cblk, g->cblk_hot,
+                         g->corp, g->cocol, cw ? g->cow : nullptr, (const V *)s->mprop, g->P, (V *)s->mupd, s->Du,
+                         (V *)s->cseg, (const V *)s->tpart);
+  } else {
+    hipLaunchKernelGGL(k_bsp_pull_rb<Op>, dim3(g->n_pblk), dim3(kBlock), 0, g->ctx->stream, g->pblk, g->pblk_hot,
+                       g->crp, g->ccol, cw, (const V *)s->mprop, g->P, (V *)s->mupd, s->Du, (V *)s->partial,
+                       (const V *)nullptr);
+  }
   GRD_HIP(hipGetLastError());
   return GRD_OK;
 }
@@ -361,9 +377,12 @@ grd_status bsp_pull_fixup(grd_state *s)
 {
   grd_graph *g = s->g;
   using V = typename Op::V;
-  if (!g->n_plong) return GRD_OK;
-  hipLaunchKernelGGL(k_bsp_pull_fixup<Op>, dim3((g->n_plong + kBlock - 1) / kBlock), dim3(kBlock), 0, g->ctx->stream,
-                     g->plong, g->n_plong, (const V *)s->partial, (V *)s->mupd, s->Du);
+  const bool hc = bsp_hot_cold(s);
+  const uint32_t nlong = hc ? g->n_clong : g->n_plong;
+  if (!nlong) return GRD_OK;
+  hipLaunchKernelGGL(k_bsp_pull_fixup<Op>, dim3((nlong + kBlock - 1) / kBlock), dim3(kBlock), 0, g->ctx->stream,
+                     hc ? g->clong : g->plong, nlong, hc ? (const V *)s->cseg : (const V *)s->partial, (V *)s->mupd,
+                     s->Du, hc ? (const V *)s->tpart : (const V *)nullptr);
   GRD_HIP(hipGetLastError());
   return GRD_OK;
 }

@@ -34,6 +34,7 @@
 
 #include "grd_device.hpp"
 #include "grd_internal.hpp"
+#include "grd_ops.hpp"
 
 namespace grd {
 
@@ -501,9 +502,10 @@ __global__ void k_h_split(uint32_t P, const uint32_t *__restrict__ crp, const ui
 // a wave per row: its hot arcs as 16-bit window offsets, its cold arcs as
 // source proxies
 __global__ void k_h_copy(uint32_t P, const uint32_t *__restrict__ crp, const uint32_t *__restrict__ ccol,
-                         const uint32_t *__restrict__ hrp, const uint32_t *__restrict__ corp,
-                         const uint32_t *__restrict__ pb, uint32_t M, uint16_t *__restrict__ hcol,
-                         uint32_t *__restrict__ cocol)
+                         const uint32_t *__restrict__ cw, const uint32_t *__restrict__ hrp,
+                         const uint32_t *__restrict__ corp, const uint32_t *__restrict__ pb, uint32_t M,
+                         uint16_t *__restrict__ hcol, uint32_t *__restrict__ cocol, uint32_t *__restrict__ hw,
+                         uint32_t *__restrict__ cow)
 {
   const uint32_t lane = lane_id();
   const uint64_t stride = (uint64_t)gridDim.x * kWavesPerBlock;
@@ -513,8 +515,13 @@ __global__ void k_h_copy(uint32_t P, const uint32_t *__restrict__ crp, const uin
     const uint32_t h0 = hrp[r], nh = hrp[r + 1] - h0, c0 = corp[r];
     for (uint32_t i = lane; i < n; i += kWave) {
       const uint32_t v = ccol[e0 + i];
-      if (i < nh) hcol[h0 + i] = (uint16_t)(v - base);
-      else cocol[c0 + (i - nh)] = v;
+      if (i < nh) {
+        hcol[h0 + i] = (uint16_t)(v - base);
+        if (cw) hw[h0 + i] = cw[e0 + i];
+      } else {
+        cocol[c0 + (i - nh)] = v;
+        if (cw) cow[c0 + (i - nh)] = cw[e0 + i];
+      }
     }
   }
 }
@@ -678,8 +685,153 @@ __global__ void __launch_bounds__(kBlock) k_prp_hot_fixup(const PullLong *__rest
   }
 }
 
-}  // namespace
+// The same hot pass for the BSP pull (CC, SSSP, reference PageRank): the
+// window holds the sources' broadcast values (the combine identity for the
+// inactive ones), each hot arc contributes Op::edge(value, weight), and a
+// row's contributions are combined with the operator (min, or + in a fixed
+// order) into one partial per row, which the cold row-block kernel combines
+// with its own before it applies the row's update.
+template <class Op>
+__global__ void __launch_bounds__(kTierBlock) k_bsp_pull_hot(const TierTile *__restrict__ ranges,
+                                                             const PullBlock *__restrict__ blocks,
+                                                             const uint32_t *__restrict__ hrp,
+                                                             const uint16_t *__restrict__ hcol,
+                                                             const uint32_t *__restrict__ hw,
+                                                             const typename Op::V *__restrict__ mprop,
+                                                             typename Op::V *__restrict__ hpart,
+                                                             typename Op::V *__restrict__ hseg)
+{
+  using V = typename Op::V;
+  constexpr uint32_t kWaves = kTierBlock / kWave;
+  constexpr int kPer = (kWNnz + kWave - 1) / kWave, kRows = kWRows / kWave, kStage = kHotWin / 4 / kTierBlock;
+  static_assert(sizeof(V) == 4, "the window holds 4-byte values");
+  __shared__ uint4 s_win4[kHotWin / 4];
+  __shared__ V s_v[kWaves][kWNnz];
+  __shared__ uint32_t s_next;
+  const TierTile T = ranges[blockIdx.x];
+  if (threadIdx.x == 0) s_next = T.b0 + kWaves;
+  {
+    const uint4 *__restrict__ src = reinterpret_cast<const uint4 *>(mprop + T.wbase);
+    const uint32_t n4 = T.wlen >> 2;
+    uint4 x[kStage];
+#pragma unroll
+    for (int k = 0; k < kStage; k++) {
+      const uint32_t i = threadIdx.x + k * kTierBlock;
+      x[k] = src[i < n4 ? i : 0];
+    }
+#pragma unroll
+    for (int k = 0; k < kStage; k++) s_win4[threadIdx.x + k * kTierBlock] = x[k];
+  }
+  __syncthreads();
+  const V *s_win = reinterpret_cast<const V *>(s_win4);
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = lane_id();
+  V *sv = s_v[w];
+  const uint32_t bend = T.b0 + T.nb;
+  auto issue = [&](const PullBlock &d, uint32_t (&c)[kPer], uint32_t (&wt)[kPer], uint32_t (&rb)[kRows],
+                   uint32_t (&re)[kRows]) {
+    const uint32_t ne = d.e1 - d.e0;
+    uint32_t G = 64;
+    while (G > 1 && G * d.nrows > (uint32_t)kWave) G >>= 1;
+    const uint32_t ngroups = kWave / G;
+    const uint32_t nr = (d.nrows + ngroups - 1) / ngroups;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = lane + k * kWave;
+      c[k] = hcol[d.e0 + (i < ne ? i : 0)];
+      wt[k] = hw ? hw[d.e0 + (i < ne ? i : 0)] : 1u;
+    }
+#pragma unroll
+    for (int k = 0; k < kRows; k++) {
+      const uint32_t rl = k * ngroups + lane / G;
+      const uint32_t r = d.r0 + ((uint32_t)k < nr && rl < d.nrows ? rl : 0);
+      rb[k] = hrp[r];
+      re[k] = hrp[r + 1];
+    }
+  };
+  auto take = [&]() {
+    uint32_t x = 0;
+    if (lane == 0) x = atomicAdd(&s_next, 1u);
+    return __builtin_amdgcn_readfirstlane(x);
+  };
+  uint32_t b = T.b0 + w;
+  PullBlock d = b < bend ? blocks[b] : PullBlock{0, 0, 0, 0};
+  uint32_t c[kPer], wt[kPer], rb[kRows], re[kRows];
+  issue(d, c, wt, rb, re);
+  while (b < bend) {
+    const uint32_t bn = take();
+    const PullBlock dn = bn < bend ? blocks[bn] : PullBlock{0, 0, 0, 0};
+    const uint32_t ne = d.e1 - d.e0;
+    V v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) v[k] = lane + k * kWave < ne ? Op::edge(s_win[c[k]], wt[k]) : op_identity<Op>();
+    uint32_t cn[kPer], wtn[kPer], rbn[kRows], ren[kRows];
+    issue(dn, cn, wtn, rbn, ren);
+    if (d.nrows == 0) {
+      // a segment of a long row: each lane over its arcs in order, then the
+      // lanes combined by a fixed butterfly
+      V acc = op_identity<Op>();
+#pragma unroll
+      for (int j = 0; j < kPer; j++) acc = op_combine<Op>(acc, v[j]);
+      for (int o = kWave / 2; o > 0; o >>= 1) acc = op_combine<Op>(acc, __shfl_xor(acc, o, kWave));
+      if (lane == 0) hseg[b] = acc;
+    } else {
+#pragma unroll
+      for (int k = 0; k < kPer; k++) {
+        const uint32_t i = lane + k * kWave;
+        if (i < ne) sv[i] = v[k];
+      }
+      wave_lds_sync();
+      uint32_t G = 64;
+      while (G > 1 && G * d.nrows > (uint32_t)kWave) G >>= 1;
+      const uint32_t sub = lane % G, ngroups = kWave / G;
+      const uint32_t nr = (d.nrows + ngroups - 1) / ngroups;
+#pragma unroll
+      for (int k = 0; k < kRows; k++) {
+        if ((uint32_t)k < nr) {  // uniform over the wave
+          const uint32_t rl = k * ngroups + lane / G;
+          V acc = op_identity<Op>();
+          if (rl < d.nrows)
+            for (uint32_t x = rb[k] - d.e0 + sub; x < re[k] - d.e0; x += G) acc = op_combine<Op>(acc, sv[x]);
+          for (uint32_t o = G / 2; o > 0; o >>= 1) acc = op_combine<Op>(acc, __shfl_xor(acc, (int)o, kWave));
+          if (sub == 0 && rl < d.nrows) hpart[d.r0 + rl] = acc;
+        }
+      }
+      wave_lds_sync();
+    }
+    b = bn;
+    d = dn;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      c[k] = cn[k];
+      wt[k] = wtn[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kRows; k++) {
+      rb[k] = rbn[k];
+      re[k] = ren[k];
+    }
+  }
+}
 
+// long hot prefixes of the BSP pull: a wave each, segments combined in a
+// fixed order
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_bsp_hot_fixup(const PullLong *__restrict__ longs, uint32_t nlong,
+                                                          const typename Op::V *__restrict__ hseg,
+                                                          typename Op::V *__restrict__ hpart)
+{
+  using V = typename Op::V;
+  const uint32_t lane = lane_id();
+  for (uint32_t i = blockIdx.x * kWavesPerBlock + wave_in_block(); i < nlong; i += gridDim.x * kWavesPerBlock) {
+    const PullLong L = longs[i];
+    V acc = op_identity<Op>();
+    for (uint32_t j = lane; j < L.nseg; j += kWave) acc = op_combine<Op>(acc, hseg[L.first + j]);
+    for (int o = kWave / 2; o > 0; o >>= 1) acc = op_combine<Op>(acc, __shfl_xor(acc, o, kWave));
+    if (lane == 0) hpart[L.row] = acc;
+  }
+}
+
+}  // namespace
 
 // mode 3: split every row at its partition's window end on the device; the
 // cut into row blocks, rounds and workgroup ranges runs on the host
@@ -715,10 +867,14 @@ static grd_status build_hotcold(grd_graph *g)
   // padded: a wave block without hot arcs still reads the column at its start
   GRD_TRY(gm.alloc_n(&g->tcol, (uint64_t)hrp[P] + kWave));
   GRD_TRY(gm.alloc_n(&g->cocol, std::max<uint64_t>(corp[P], 1)));
+  if (g->cw) {  // arc weights follow their arcs (the BSP pull of weighted SSSP)
+    GRD_TRY(gm.alloc_n(&g->hw, (uint64_t)hrp[P] + kWave));
+    GRD_TRY(gm.alloc_n(&g->cow, std::max<uint64_t>(corp[P], 1)));
+  }
   if (P) {
     const uint64_t nb = std::min<uint64_t>(65535, ((uint64_t)P + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL(k_h_copy, dim3((uint32_t)nb), dim3(kBlock), 0, st, P, g->crp, g->ccol, g->hrp, g->corp, d_pb, M,
-                       g->tcol, g->cocol);
+    hipLaunchKernelGGL(k_h_copy, dim3((uint32_t)nb), dim3(kBlock), 0, st, P, g->crp, g->ccol, g->cw, g->hrp, g->corp,
+                       d_pb, M, g->tcol, g->cocol, g->hw, g->cow);
   }
   GRD_HIP(hipGetLastError());
   // hot: wave blocks per partition, then each partition's blocks split into
@@ -1034,5 +1190,31 @@ grd_status prp_pull_tiled(grd_state *s)
   GRD_HIP(hipGetLastError());
   return GRD_OK;
 }
+
+// the hot pass of the BSP pull over a graph split by mode 3: one partial per
+// row (the operator's identity for a row without hot arcs) into s->tpart
+template <class Op>
+grd_status bsp_pull_hot(grd_state *s)
+{
+  grd_graph *g = s->g;
+  hipStream_t st = g->ctx->stream;
+  using V = typename Op::V;
+  V *hpart = reinterpret_cast<V *>(s->tpart), *hseg = reinterpret_cast<V *>(s->tseg);
+  const uint32_t *hw = std::is_same<Op, OpWeighted>::value ? g->hw : nullptr;
+  if (g->n_ttile)
+    hipLaunchKernelGGL((k_bsp_pull_hot<Op>), dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile, g->tblk, g->hrp,
+                       g->tcol, hw, (const V *)s->mprop, hpart, hseg);
+  if (g->n_tlong) {
+    const uint32_t grid =
+        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(65535, (g->n_tlong + kWavesPerBlock - 1) / kWavesPerBlock));
+    hipLaunchKernelGGL((k_bsp_hot_fixup<Op>), dim3(grid), dim3(kBlock), 0, st, g->tlong, g->n_tlong, hseg, hpart);
+  }
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+template grd_status bsp_pull_hot<OpCC>(grd_state *);
+template grd_status bsp_pull_hot<OpHop>(grd_state *);
+template grd_status bsp_pull_hot<OpWeighted>(grd_state *);
+template grd_status bsp_pull_hot<OpPR>(grd_state *);
 
 }  // namespace grd

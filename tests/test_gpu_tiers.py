@@ -97,6 +97,93 @@ def test_hot_cold_long_hot_and_cold_rows(ctx):
         assert float(np.abs(r.astype(np.float64) - rp.astype(np.float64)).sum()) <= 1e-6, M
 
 
+@pytest.fixture(scope="module")
+def bsp_graphs():
+    gs = O.rmat(12, 16, seed=5, symmetrize=True)
+    return gs, O.rmat(12, 16, seed=7), gs.with_weights(11)
+
+
+@pytest.mark.parametrize("C,M,tb", [(1, 1, 4), (1, 4, 6), (2, 3, 5), (1, 2, 8)])
+@pytest.mark.parametrize("option", [api.OPT_PULL_SPARSE, api.OPT_PULL_DENSE])
+def test_hot_cold_bsp_pull(ctx, bsp_graphs, C, M, tb, option):
+    """The BSP pull through the hot / cold split with small windows (so that
+    rows have hot and cold arcs, and long hot and cold rows exist), every
+    superstep a pull: CC labels, hop and weighted SSSP distances bit-exact
+    with equal iteration counts, reference PageRank within its FP32 bound."""
+    _say("hot/cold bsp pull", C, M, tb, option)
+    gs, gd, gw = bsp_graphs
+    ctx.set_pull_tiling(3, tb, 0)
+    try:
+        ref, st = O.Partition(gs, C, M).cc()
+        G = api.Graph.from_csr(ctx, gs.rp, gs.col, C=C, M=M)
+        S = api.State(G, api.CC)
+        S.init(sched=api.SCHED_FIXED, fixed_option=option)
+        stats = S.run()
+        assert np.array_equal(S.result(), ref)
+        assert stats["iterations"] == int(st[0])
+        S.free()
+        G.free()
+
+        ref, st = O.Partition(gd, C, M).sssp(0)
+        G = api.Graph.from_csr(ctx, gd.rp, gd.col, C=C, M=M)
+        S = api.State(G, api.SSSP)
+        S.init(source=0, sched=api.SCHED_FIXED, fixed_option=option)
+        stats = S.run()
+        assert np.array_equal(S.result(), ref)
+        assert stats["iterations"] == int(st[0])
+        S.free()
+
+        Sp = api.State(G, api.PR)
+        Sp.init(sched=api.SCHED_FIXED, fixed_option=option)
+        stats = Sp.run()
+        out = Sp.result().astype(np.float64)
+        P = O.Partition(gd, C, M)
+        ref64, st = P.pr(fp64=True)
+        ref32, _ = P.pr(fp64=False)
+        l1 = np.abs(out / out.sum() - ref64 / ref64.sum()).sum()
+        l1_32 = np.abs(ref32 / ref32.sum() - ref64 / ref64.sum()).sum()
+        assert l1 <= max(1e-6, 4 * l1_32)
+        assert abs(stats["iterations"] - int(st[0])) <= 3
+        Sp.free()
+        G.free()
+
+        src = int(np.argmax(gw.out_degree() > 0))
+        _, st = O.Partition(gw, C, M).sssp(src, weighted=True, delta=0)
+        G = api.Graph.from_csr(ctx, gw.rp, gw.col, gw.w, C=C, M=M)
+        S = api.State(G, api.SSSP)
+        S.init(source=src, weighted=True, delta=0, max_iter=100000, sched=api.SCHED_FIXED, fixed_option=option)
+        stats = S.run()
+        assert np.array_equal(S.result(), O.dijkstra(gw, src))
+        assert stats["iterations"] == int(st[0])
+        S.free()
+        G.free()
+    finally:
+        ctx.set_pull_tiling(3, 0, 0)
+
+
+def test_bsp_pull_on_graph_tiled_by_another_mode(ctx, bsp_graphs):
+    """A graph regrouped by the LDS-tiled pull first keeps that regrouping;
+    a CC state created on it later pulls over the plain CSC, and is exact."""
+    _say("bsp pull after mode 1")
+    gs = bsp_graphs[0]
+    ref, st = O.Partition(gs, 1, 4).cc()
+    G = api.Graph.from_csr(ctx, gs.rp, gs.col, C=1, M=4)
+    try:
+        ctx.set_pull_tiling(1, 6, 1024)
+        Sp = api.State(G, api.PR_PULL)
+        ctx.set_pull_tiling(3, 6, 0)
+        S = api.State(G, api.CC)
+        S.init(sched=api.SCHED_FIXED, fixed_option=api.OPT_PULL_DENSE)
+        stats = S.run()
+        assert np.array_equal(S.result(), ref)
+        assert stats["iterations"] == int(st[0])
+        S.free()
+        Sp.free()
+    finally:
+        ctx.set_pull_tiling(3, 0, 0)
+        G.free()
+
+
 def test_pull_tiling_argument_checks(ctx):
     with pytest.raises(api.GrdError):
         ctx.set_pull_tiling(1, 3, 0)
