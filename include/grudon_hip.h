@@ -149,7 +149,9 @@ grd_status grd_set_layout(grd_ctx *ctx, int on_device);
  *     2^tier_bits hottest sources (4..15, 0 = 15) summed from an LDS window
  *     into a per-row partial, the rest of the row pulled by the row-block
  *     kernel, which merges that partial first (tile_arcs must be 0); the
- *     default, and the fastest measured.
+ *     default, and the fastest measured.  Mode 3 also serves the pull
+ *     supersteps of CC, SSSP and reference PageRank (states of any algorithm
+ *     created afterwards); modes 1 and 2 serve GRD_PR_PULL only.
  * Modes 1-3 regroup a graph when its first pull-PageRank state is created,
  * and a graph keeps the regrouping it was first given; per row, the
  * partials are merged in a fixed order.  All modes compute the same sums
