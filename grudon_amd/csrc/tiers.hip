@@ -533,11 +533,13 @@ __global__ void k_h_copy(uint32_t P, const uint32_t *__restrict__ crp, const uin
 // one at a time from a counter in LDS and work on their own -- column loads
 // and row bounds, gathers from the window into the wave's LDS slice, a group
 // of G lanes per row -- with no workgroup barrier, so that the waves hide
-// each other's latency.  LDS: 128 KB window + 16 slices of 504 values + the
-// block counter.
+// each other's latency.  The next block's loads are in flight while the
+// current one is gathered and summed.  LDS: 128 KB window + 16 slices of 504
+// values + the block counter.
 constexpr uint32_t kWNnz = 504;   // arcs per wave block (at most 8 per lane; 8 slots short of 512 leave
                                   // LDS room for the block counter)
 constexpr uint32_t kWRows = 256;  // rows per wave block (at most 4 per lane)
+constexpr int kRsrcFlags = 0x00020000;  // raw buffer resource, word 3, on gfx9
 
 __device__ __forceinline__ void wave_lds_sync()
 {
@@ -546,25 +548,78 @@ __device__ __forceinline__ void wave_lds_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
-__global__ void __launch_bounds__(kTierBlock) k_prp_pull_hot(const TierTile *__restrict__ ranges,
-                                                             const PullBlock *__restrict__ blocks,
-                                                             const uint32_t *__restrict__ hrp,
-                                                             const uint16_t *__restrict__ hcol,
-                                                             const float *__restrict__ mprop,
-                                                             float2 *__restrict__ hpart, float2 *__restrict__ hseg)
+// a raw buffer over [base, base + bytes): loads past the end return 0, so the
+// lanes beyond a block's arcs or rows need no clamped index
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t block_buffer(const void *base, uint32_t bytes)
 {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, kRsrcFlags);
+}
+
+// group width for a block of n rows: the largest power of two G with G n <= 64
+__device__ __forceinline__ uint32_t hot_group(uint32_t n)
+{
+  if (n <= 1) return 64;
+  if (n > 64) return 1;
+  return 64u >> (32 - __builtin_clz(n - 1));  // 64 >> ceil(log2 n)
+}
+
+// How the hot pass accumulates.  Pull PageRank: FP32 Kahan per lane, lanes
+// merged by TwoSum, a (sum, compensation) pair per row.  The BSP pull (CC,
+// SSSP, reference PageRank): Op::edge of the source's broadcast value and the
+// arc weight, combined with the operator (min, or + in a fixed order).
+struct HotPR {
+  using V = float;
+  using Part = float2;
+  using Acc = Kahan;
+  static constexpr bool kWeighted = false;
+  __device__ static Acc zero() { return Kahan{}; }
+  __device__ static V value(V v, uint32_t) { return v; }
+  __device__ static void add(Acc &a, V x) { a.add(x); }
+  __device__ static void merge_lanes(Acc &a, int width) { kahan_wave_merge(a, width); }
+  __device__ static void add_part(Acc &a, Part p) { a.merge(p.x, p.y); }
+  __device__ static Part part(const Acc &a) { return make_float2(a.s, a.c); }
+};
+template <class Op>
+struct HotBsp {
+  using V = typename Op::V;
+  using Part = V;
+  using Acc = V;
+  static constexpr bool kWeighted = std::is_same<Op, OpWeighted>::value;
+  __device__ static Acc zero() { return op_identity<Op>(); }
+  __device__ static V value(V v, uint32_t w) { return Op::edge(v, w); }
+  __device__ static void add(Acc &a, V x) { a = op_combine<Op>(a, x); }
+  __device__ static void merge_lanes(Acc &a, int width)
+  {
+    for (int o = width / 2; o > 0; o >>= 1) a = op_combine<Op>(a, __shfl_xor(a, o, kWave));
+  }
+  __device__ static void add_part(Acc &a, Part p) { a = op_combine<Op>(a, p); }
+  __device__ static Part part(const Acc &a) { return a; }
+};
+
+template <class Pol>
+__global__ void __launch_bounds__(kTierBlock) k_pull_hot(const TierTile *__restrict__ ranges,
+                                                         const PullBlock *__restrict__ blocks,
+                                                         const uint32_t *__restrict__ hrp,
+                                                         const uint16_t *__restrict__ hcol,
+                                                         const uint32_t *__restrict__ hw,
+                                                         const typename Pol::V *__restrict__ mprop,
+                                                         typename Pol::Part *__restrict__ hpart,
+                                                         typename Pol::Part *__restrict__ hseg)
+{
+  using V = typename Pol::V;
+  static_assert(sizeof(V) == 4, "the window holds 4-byte values");
   constexpr uint32_t kWaves = kTierBlock / kWave;
   constexpr int kPer = (kWNnz + kWave - 1) / kWave, kRows = kWRows / kWave, kStage = kHotWin / 4 / kTierBlock;
-  __shared__ float4 s_win4[kHotWin / 4];
-  __shared__ float s_v[kWaves][kWNnz];
+  __shared__ uint4 s_win4[kHotWin / 4];
+  __shared__ V s_v[kWaves][kWNnz];
   __shared__ uint32_t s_next;  // the range's next block not yet taken by a wave
   const TierTile T = ranges[blockIdx.x];
   if (threadIdx.x == 0) s_next = T.b0 + kWaves;
   {
-    // wlen is a multiple of 4 and wbase of 64: whole, aligned float4s
-    const float4 *__restrict__ src = reinterpret_cast<const float4 *>(mprop + T.wbase);
+    // wlen is a multiple of 4 and wbase of 64: whole, aligned 16-byte units
+    const uint4 *__restrict__ src = reinterpret_cast<const uint4 *>(mprop + T.wbase);
     const uint32_t n4 = T.wlen >> 2;
-    float4 x[kStage];
+    uint4 x[kStage];
 #pragma unroll
     for (int k = 0; k < kStage; k++) {
       const uint32_t i = threadIdx.x + k * kTierBlock;
@@ -576,204 +631,79 @@ __global__ void __launch_bounds__(kTierBlock) k_prp_pull_hot(const TierTile *__r
     for (int k = 0; k < kStage; k++) s_win4[threadIdx.x + k * kTierBlock] = x[k];
   }
   __syncthreads();
-  const float *s_win = reinterpret_cast<const float *>(s_win4);
-  // the wave index made uniform, so that block descriptors are scalar loads
-  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = lane_id();
-  float *sv = s_v[w];
-  const uint32_t bend = T.b0 + T.nb;
-  // the global loads of a block: its columns and the lanes' row bounds
-  auto issue = [&](const PullBlock &d, uint32_t (&c)[kPer], uint32_t (&rb)[kRows], uint32_t (&re)[kRows]) {
-    const uint32_t ne = d.e1 - d.e0;
-    uint32_t G = 64;
-    while (G > 1 && G * d.nrows > (uint32_t)kWave) G >>= 1;
-    const uint32_t ngroups = kWave / G;
-    const uint32_t nr = (d.nrows + ngroups - 1) / ngroups;
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-      const uint32_t i = lane + k * kWave;
-      c[k] = hcol[d.e0 + (i < ne ? i : 0)];
-    }
-#pragma unroll
-    for (int k = 0; k < kRows; k++) {
-      const uint32_t rl = k * ngroups + lane / G;
-      const uint32_t r = d.r0 + ((uint32_t)k < nr && rl < d.nrows ? rl : 0);
-      rb[k] = hrp[r];
-      re[k] = hrp[r + 1];
-    }
-  };
-  // a wave takes its next block from the range's counter when it starts the
-  // current one (blocks are independent, so the order does not change any
-  // result), and that block's loads are in flight while the current block
-  // is gathered, summed and stored
-  auto take = [&]() {
-    uint32_t x = 0;
-    if (lane == 0) x = atomicAdd(&s_next, 1u);
-    return __builtin_amdgcn_readfirstlane(x);
-  };
-  uint32_t b = T.b0 + w;
-  PullBlock d = b < bend ? blocks[b] : PullBlock{0, 0, 0, 0};
-  uint32_t c[kPer], rb[kRows], re[kRows];
-  issue(d, c, rb, re);
-  while (b < bend) {
-    const uint32_t bn = take();
-    const PullBlock dn = bn < bend ? blocks[bn] : PullBlock{0, 0, 0, 0};
-    const uint32_t ne = d.e1 - d.e0;
-    float v[kPer];
-#pragma unroll
-    for (int k = 0; k < kPer; k++) v[k] = lane + k * kWave < ne ? s_win[c[k]] : 0.0f;
-    uint32_t cn[kPer], rbn[kRows], ren[kRows];
-    issue(dn, cn, rbn, ren);
-    if (d.nrows == 0) {
-      // a segment of a long row: each lane over its arcs in order, then the
-      // wave's lanes merged by a fixed butterfly
-      Kahan k;
-#pragma unroll
-      for (int j = 0; j < kPer; j++)
-        if (lane + j * kWave < ne) k.add(v[j]);
-      kahan_wave_merge(k, kWave);
-      if (lane == 0) hseg[b] = make_float2(k.s, k.c);
-    } else {
-#pragma unroll
-      for (int k = 0; k < kPer; k++) {
-        const uint32_t i = lane + k * kWave;
-        if (i < ne) sv[i] = v[k];
-      }
-      wave_lds_sync();
-      uint32_t G = 64;
-      while (G > 1 && G * d.nrows > (uint32_t)kWave) G >>= 1;
-      const uint32_t sub = lane % G, ngroups = kWave / G;
-      const uint32_t nr = (d.nrows + ngroups - 1) / ngroups;
-#pragma unroll
-      for (int k = 0; k < kRows; k++) {
-        if ((uint32_t)k < nr) {  // uniform over the wave
-          const uint32_t rl = k * ngroups + lane / G;
-          Kahan acc;
-          if (rl < d.nrows)
-            for (uint32_t x = rb[k] - d.e0 + sub; x < re[k] - d.e0; x += G) acc.add(sv[x]);
-          kahan_wave_merge(acc, (int)G);
-          if (sub == 0 && rl < d.nrows) hpart[d.r0 + rl] = make_float2(acc.s, acc.c);
-        }
-      }
-      wave_lds_sync();  // the slice is rewritten by the next block
-    }
-    b = bn;
-    d = dn;
-#pragma unroll
-    for (int k = 0; k < kPer; k++) c[k] = cn[k];
-#pragma unroll
-    for (int k = 0; k < kRows; k++) {
-      rb[k] = rbn[k];
-      re[k] = ren[k];
-    }
-  }
-}
-
-// long hot prefixes: a wave each, segments merged in a fixed order
-__global__ void __launch_bounds__(kBlock) k_prp_hot_fixup(const PullLong *__restrict__ longs, uint32_t nlong,
-                                                          const float2 *__restrict__ hseg, float2 *__restrict__ hpart)
-{
-  const uint32_t lane = lane_id();
-  for (uint32_t i = blockIdx.x * kWavesPerBlock + wave_in_block(); i < nlong; i += gridDim.x * kWavesPerBlock) {
-    const PullLong L = longs[i];
-    Kahan t;
-    for (uint32_t j = lane; j < L.nseg; j += kWave) {
-      const float2 p = hseg[L.first + j];
-      t.merge(p.x, p.y);
-    }
-    kahan_wave_merge(t, kWave);
-    if (lane == 0) hpart[L.row] = make_float2(t.s, t.c);
-  }
-}
-
-// The same hot pass for the BSP pull (CC, SSSP, reference PageRank): the
-// window holds the sources' broadcast values (the combine identity for the
-// inactive ones), each hot arc contributes Op::edge(value, weight), and a
-// row's contributions are combined with the operator (min, or + in a fixed
-// order) into one partial per row, which the cold row-block kernel combines
-// with its own before it applies the row's update.
-template <class Op>
-__global__ void __launch_bounds__(kTierBlock) k_bsp_pull_hot(const TierTile *__restrict__ ranges,
-                                                             const PullBlock *__restrict__ blocks,
-                                                             const uint32_t *__restrict__ hrp,
-                                                             const uint16_t *__restrict__ hcol,
-                                                             const uint32_t *__restrict__ hw,
-                                                             const typename Op::V *__restrict__ mprop,
-                                                             typename Op::V *__restrict__ hpart,
-                                                             typename Op::V *__restrict__ hseg)
-{
-  using V = typename Op::V;
-  constexpr uint32_t kWaves = kTierBlock / kWave;
-  constexpr int kPer = (kWNnz + kWave - 1) / kWave, kRows = kWRows / kWave, kStage = kHotWin / 4 / kTierBlock;
-  static_assert(sizeof(V) == 4, "the window holds 4-byte values");
-  __shared__ uint4 s_win4[kHotWin / 4];
-  __shared__ V s_v[kWaves][kWNnz];
-  __shared__ uint32_t s_next;
-  const TierTile T = ranges[blockIdx.x];
-  if (threadIdx.x == 0) s_next = T.b0 + kWaves;
-  {
-    const uint4 *__restrict__ src = reinterpret_cast<const uint4 *>(mprop + T.wbase);
-    const uint32_t n4 = T.wlen >> 2;
-    uint4 x[kStage];
-#pragma unroll
-    for (int k = 0; k < kStage; k++) {
-      const uint32_t i = threadIdx.x + k * kTierBlock;
-      x[k] = src[i < n4 ? i : 0];
-    }
-#pragma unroll
-    for (int k = 0; k < kStage; k++) s_win4[threadIdx.x + k * kTierBlock] = x[k];
-  }
-  __syncthreads();
   const V *s_win = reinterpret_cast<const V *>(s_win4);
+  // the wave index made uniform, so that block descriptors are scalar loads
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = lane_id();
   V *sv = s_v[w];
   const uint32_t bend = T.b0 + T.nb;
+  // the global loads of a block: its columns (and weights), and the lanes'
+  // row bounds, each through a buffer over the block's own range
   auto issue = [&](const PullBlock &d, uint32_t (&c)[kPer], uint32_t (&wt)[kPer], uint32_t (&rb)[kRows],
                    uint32_t (&re)[kRows]) {
     const uint32_t ne = d.e1 - d.e0;
-    uint32_t G = 64;
-    while (G > 1 && G * d.nrows > (uint32_t)kWave) G >>= 1;
-    const uint32_t ngroups = kWave / G;
-    const uint32_t nr = (d.nrows + ngroups - 1) / ngroups;
+    const uint32_t G = hot_group(d.nrows), ngroups = kWave / G;
+    const auto bc = block_buffer(hcol + d.e0, ne * 2);
 #pragma unroll
-    for (int k = 0; k < kPer; k++) {
-      const uint32_t i = lane + k * kWave;
-      c[k] = hcol[d.e0 + (i < ne ? i : 0)];
-      wt[k] = hw ? hw[d.e0 + (i < ne ? i : 0)] : 1u;
+    for (int k = 0; k < kPer; k++) c[k] = __builtin_amdgcn_raw_buffer_load_b16(bc, (lane + k * kWave) * 2, 0, 0);
+    if constexpr (Pol::kWeighted) {
+      const auto bw = block_buffer(hw + d.e0, ne * 4);
+#pragma unroll
+      for (int k = 0; k < kPer; k++) wt[k] = __builtin_amdgcn_raw_buffer_load_b32(bw, (lane + k * kWave) * 4, 0, 0);
+    } else {
+#pragma unroll
+      for (int k = 0; k < kPer; k++) wt[k] = 1u;
     }
+    const auto br = block_buffer(hrp + d.r0, (d.nrows + 1) * 4);
 #pragma unroll
     for (int k = 0; k < kRows; k++) {
       const uint32_t rl = k * ngroups + lane / G;
-      const uint32_t r = d.r0 + ((uint32_t)k < nr && rl < d.nrows ? rl : 0);
-      rb[k] = hrp[r];
-      re[k] = hrp[r + 1];
+      rb[k] = __builtin_amdgcn_raw_buffer_load_b32(br, rl * 4, 0, 0);
+      re[k] = __builtin_amdgcn_raw_buffer_load_b32(br, rl * 4 + 4, 0, 0);
     }
   };
+  // a wave takes its blocks from the range's counter, each two blocks ahead
+  // of its use (blocks are independent, so the order does not change any
+  // result); descriptors come through a vector load, so that waiting for one
+  // does not wait for the LDS, and read back into scalars one block later.
+  // A descriptor past the range reads as zeros: no rows, no arcs.
   auto take = [&]() {
     uint32_t x = 0;
     if (lane == 0) x = atomicAdd(&s_next, 1u);
     return __builtin_amdgcn_readfirstlane(x);
   };
+  const auto bdesc = block_buffer(blocks, bend * (uint32_t)sizeof(PullBlock));
+  auto load_desc = [&](uint32_t i) {
+    return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(bdesc, i * sizeof(PullBlock), 0, 0));
+  };
+  auto to_block = [](const uint4 &q) {
+    return PullBlock{(uint32_t)__builtin_amdgcn_readfirstlane(q.x), (uint32_t)__builtin_amdgcn_readfirstlane(q.y),
+                     (uint32_t)__builtin_amdgcn_readfirstlane(q.z), (uint32_t)__builtin_amdgcn_readfirstlane(q.w)};
+  };
   uint32_t b = T.b0 + w;
-  PullBlock d = b < bend ? blocks[b] : PullBlock{0, 0, 0, 0};
+  PullBlock d = to_block(load_desc(b));
+  uint32_t b1 = take();
+  uint4 q1 = load_desc(b1);
   uint32_t c[kPer], wt[kPer], rb[kRows], re[kRows];
   issue(d, c, wt, rb, re);
   while (b < bend) {
-    const uint32_t bn = take();
-    const PullBlock dn = bn < bend ? blocks[bn] : PullBlock{0, 0, 0, 0};
+    const uint32_t b2 = take();
+    const uint4 q2 = load_desc(b2);
     const uint32_t ne = d.e1 - d.e0;
     V v[kPer];
 #pragma unroll
-    for (int k = 0; k < kPer; k++) v[k] = lane + k * kWave < ne ? Op::edge(s_win[c[k]], wt[k]) : op_identity<Op>();
+    for (int k = 0; k < kPer; k++) v[k] = Pol::value(s_win[c[k]], wt[k]);
+    const PullBlock dn = to_block(q1);
     uint32_t cn[kPer], wtn[kPer], rbn[kRows], ren[kRows];
     issue(dn, cn, wtn, rbn, ren);
     if (d.nrows == 0) {
       // a segment of a long row: each lane over its arcs in order, then the
-      // lanes combined by a fixed butterfly
-      V acc = op_identity<Op>();
+      // wave's lanes merged in a fixed order
+      auto a = Pol::zero();
 #pragma unroll
-      for (int j = 0; j < kPer; j++) acc = op_combine<Op>(acc, v[j]);
-      for (int o = kWave / 2; o > 0; o >>= 1) acc = op_combine<Op>(acc, __shfl_xor(acc, o, kWave));
-      if (lane == 0) hseg[b] = acc;
+      for (int j = 0; j < kPer; j++)
+        if (lane + j * kWave < ne) Pol::add(a, v[j]);
+      Pol::merge_lanes(a, kWave);
+      if (lane == 0) hseg[b] = Pol::part(a);
     } else {
 #pragma unroll
       for (int k = 0; k < kPer; k++) {
@@ -781,25 +711,26 @@ __global__ void __launch_bounds__(kTierBlock) k_bsp_pull_hot(const TierTile *__r
         if (i < ne) sv[i] = v[k];
       }
       wave_lds_sync();
-      uint32_t G = 64;
-      while (G > 1 && G * d.nrows > (uint32_t)kWave) G >>= 1;
+      const uint32_t G = hot_group(d.nrows);
       const uint32_t sub = lane % G, ngroups = kWave / G;
       const uint32_t nr = (d.nrows + ngroups - 1) / ngroups;
 #pragma unroll
       for (int k = 0; k < kRows; k++) {
         if ((uint32_t)k < nr) {  // uniform over the wave
           const uint32_t rl = k * ngroups + lane / G;
-          V acc = op_identity<Op>();
+          auto a = Pol::zero();
           if (rl < d.nrows)
-            for (uint32_t x = rb[k] - d.e0 + sub; x < re[k] - d.e0; x += G) acc = op_combine<Op>(acc, sv[x]);
-          for (uint32_t o = G / 2; o > 0; o >>= 1) acc = op_combine<Op>(acc, __shfl_xor(acc, (int)o, kWave));
-          if (sub == 0 && rl < d.nrows) hpart[d.r0 + rl] = acc;
+            for (uint32_t x = rb[k] - d.e0 + sub; x < re[k] - d.e0; x += G) Pol::add(a, sv[x]);
+          Pol::merge_lanes(a, (int)G);
+          if (sub == 0 && rl < d.nrows) hpart[d.r0 + rl] = Pol::part(a);
         }
       }
-      wave_lds_sync();
+      wave_lds_sync();  // the slice is rewritten by the next block
     }
-    b = bn;
+    b = b1;
     d = dn;
+    b1 = b2;
+    q1 = q2;
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
       c[k] = cn[k];
@@ -813,21 +744,20 @@ __global__ void __launch_bounds__(kTierBlock) k_bsp_pull_hot(const TierTile *__r
   }
 }
 
-// long hot prefixes of the BSP pull: a wave each, segments combined in a
-// fixed order
-template <class Op>
-__global__ void __launch_bounds__(kBlock) k_bsp_hot_fixup(const PullLong *__restrict__ longs, uint32_t nlong,
-                                                          const typename Op::V *__restrict__ hseg,
-                                                          typename Op::V *__restrict__ hpart)
+// long hot prefixes: a wave each, lane j taking segments j, j + 64, ... and
+// the lanes merged in a fixed order
+template <class Pol>
+__global__ void __launch_bounds__(kBlock) k_hot_fixup(const PullLong *__restrict__ longs, uint32_t nlong,
+                                                      const typename Pol::Part *__restrict__ hseg,
+                                                      typename Pol::Part *__restrict__ hpart)
 {
-  using V = typename Op::V;
   const uint32_t lane = lane_id();
   for (uint32_t i = blockIdx.x * kWavesPerBlock + wave_in_block(); i < nlong; i += gridDim.x * kWavesPerBlock) {
     const PullLong L = longs[i];
-    V acc = op_identity<Op>();
-    for (uint32_t j = lane; j < L.nseg; j += kWave) acc = op_combine<Op>(acc, hseg[L.first + j]);
-    for (int o = kWave / 2; o > 0; o >>= 1) acc = op_combine<Op>(acc, __shfl_xor(acc, o, kWave));
-    if (lane == 0) hpart[L.row] = acc;
+    auto a = Pol::zero();
+    for (uint32_t j = lane; j < L.nseg; j += kWave) Pol::add_part(a, hseg[L.first + j]);
+    Pol::merge_lanes(a, kWave);
+    if (lane == 0) hpart[L.row] = Pol::part(a);
   }
 }
 
@@ -960,7 +890,8 @@ static grd_status build_hotcold(grd_graph *g)
       chot.push_back(window_after(r0));
     }
   }
-  if (hblk.size() >= 0x7FFFFFFFull || cblk.size() >= 0x7FFFFFFFull)
+  // the hot kernel reads its descriptors through a buffer of at most 2^31 bytes
+  if (hblk.size() * sizeof(PullBlock) >= 0x7FFFFFFFull || cblk.size() >= 0x7FFFFFFFull)
     return fail(GRD_ERR_ARG, "hot / cold pull: too many row blocks");
   GRD_TRY(t_upload(gm, st, &g->tblk, hblk));
   GRD_TRY(t_upload(gm, st, &g->tlong, hlong));
@@ -1164,10 +1095,10 @@ grd_status prp_pull_tiled(grd_state *s)
   if (g->tier_mode == 3) {
     // hot prefixes from LDS into per-row partials, then the cold rest
     if (g->n_ttile)
-      hipLaunchKernelGGL(k_prp_pull_hot, dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile, g->tblk, g->hrp,
-                         g->tcol, (const float *)s->mprop, s->tpart, s->tseg);
+      hipLaunchKernelGGL(k_pull_hot<HotPR>, dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile, g->tblk, g->hrp,
+                         g->tcol, (const uint32_t *)nullptr, (const float *)s->mprop, s->tpart, s->tseg);
     if (g->n_tlong)
-      hipLaunchKernelGGL(k_prp_hot_fixup, dim3(wave_grid(g->n_tlong)), dim3(kBlock), 0, st, g->tlong, g->n_tlong,
+      hipLaunchKernelGGL(k_hot_fixup<HotPR>, dim3(wave_grid(g->n_tlong)), dim3(kBlock), 0, st, g->tlong, g->n_tlong,
                          s->tseg, s->tpart);
     GRD_HIP(hipGetLastError());
     GRD_TRY(prp_pull_rows(st, g->cblk, g->n_cblk, g->cblk_hot, g->corp, g->cocol, (const float *)s->mprop, g->P, out,
@@ -1202,12 +1133,12 @@ grd_status bsp_pull_hot(grd_state *s)
   V *hpart = reinterpret_cast<V *>(s->tpart), *hseg = reinterpret_cast<V *>(s->tseg);
   const uint32_t *hw = std::is_same<Op, OpWeighted>::value ? g->hw : nullptr;
   if (g->n_ttile)
-    hipLaunchKernelGGL((k_bsp_pull_hot<Op>), dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile, g->tblk, g->hrp,
+    hipLaunchKernelGGL((k_pull_hot<HotBsp<Op>>), dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile, g->tblk, g->hrp,
                        g->tcol, hw, (const V *)s->mprop, hpart, hseg);
   if (g->n_tlong) {
     const uint32_t grid =
         (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(65535, (g->n_tlong + kWavesPerBlock - 1) / kWavesPerBlock));
-    hipLaunchKernelGGL((k_bsp_hot_fixup<Op>), dim3(grid), dim3(kBlock), 0, st, g->tlong, g->n_tlong, hseg, hpart);
+    hipLaunchKernelGGL((k_hot_fixup<HotBsp<Op>>), dim3(grid), dim3(kBlock), 0, st, g->tlong, g->n_tlong, hseg, hpart);
   }
   GRD_HIP(hipGetLastError());
   return GRD_OK;
