@@ -64,7 +64,7 @@ METRIC = "GTEPS on 1 MI355X, RMAT PR/CC/SSSP; learned-vs-plain schedule speedup"
 HBM_PEAK_GBS = 8000.0
 # the kernels of one pull step per pull mode, timed together between HIP events
 PULL_KERNELS = {"plain": "k_prp_pull_rb (+ k_prp_pull_fixup)",
-                "hotcold": "k_prp_pull_hot + k_prp_pull_rb (+ fix-ups)",
+                "hotcold": "k_pull_hot<HotPR> + k_prp_pull_rb (+ fix-ups)",
                 "tiled": "k_prp_pull_tiled + k_prp_tier_merge (+ fix-ups)",
                 "segmented": "k_prp_pull_seg + k_prp_tier_merge (+ fix-ups)"}
 INF = 0xFFFFFFFF
