@@ -1,31 +1,44 @@
-// tiers.hip -- the tiled PageRank pull: every gather served from LDS.
+// tiers.hip -- the source-blocked pulls: gathers served from LDS or L2.
 //
 // The pull sums, for every proxy row r of a memory partition, the values of
 // its in-arcs' source proxies.  The sources are the partition's own proxies,
 // numbered in out-degree order, and a random 4-byte gather from a table of
 // tens of MB is served by L2 at best and by a whole cache line from the
-// Infinity Cache / HBM at worst.  Here the partition's proxy range is cut
-// into tiers of 2^tier_bits (at most 2^15) consecutive proxies, and the arcs
-// of the CSC are regrouped by (memory partition, tier, row):
+// Infinity Cache / HBM at worst.  Three reorganisations of the CSC, built
+// once per graph when its first state is created (grd_set_pull_tiling):
+//
+// mode 3, the hot / cold split (the default, for every pull): a row's arcs
+// from its partition's 2^b hottest sources (b <= 15) are a prefix of the row,
+// because rows keep their sources ascending.  The hot prefixes, with 16-bit
+// offsets into the partition's window, are reduced from a 128 KB LDS window
+// into one partial per row (k_pull_hot, under an accumulation policy: FP32
+// Kahan for pull PageRank, the BSP operator for CC / SSSP / reference
+// PageRank); the cold rest goes through the row-block kernel of pull.hip,
+// which merges the row's partial first.
+//
+// modes 1 and 2, the (partition, tier, row) regrouping (pull PageRank only):
+// the partition's proxy range is cut into tiers of 2^tier_bits consecutive
+// proxies, and the arcs of the CSC are regrouped by (memory partition, tier,
+// row):
 //
 //   pair q       = one (row, tier) with at least one arc; its arcs are the
 //                  contiguous run of the row's CSC arcs whose sources lie in
-//                  the tier (rows keep their sources ascending)
-//   tcol[e]      = the source's 16-bit offset within its tier
+//                  the tier
+//   tcol[e]      = the source's 16-bit offset within its tier (mode 1), or
+//   tcol32[e]    = the source proxy (mode 2)
 //   tile         = a run of row blocks of one (partition, tier) group
 //
-// A 1024-thread workgroup takes a tile and stages the tier's window of mprop
-// (<= 128 KB) into LDS with coalesced 16-byte loads; then its four quarters
-// walk the tile's row blocks four at a time (8 coalesced column loads and 8
-// LDS gathers per thread, a group of lanes per pair, FP32 Kahan per lane
-// merged by TwoSum).  Each pair yields one compensated partial; a row with a
-// single pair writes its value straight into the reduce buffer, the others
-// are merged per row in tier order (k_prp_tier_merge), so the result does not
-// depend on scheduling.  A pair longer than one block is cut into segments
-// merged in order by a fix-up pass.
-//
-// HBM sees only streams: 2 bytes of column per arc, the pair records, and the
-// partials of rows with more than one pair.
+// Mode 1 (LDS tiles, tiers <= 2^15): a 1024-thread workgroup takes a tile and
+// stages the tier's window of mprop (<= 128 KB) into LDS; then its four
+// quarters walk the tile's row blocks four at a time (8 coalesced column
+// loads and 8 LDS gathers per thread, a group of lanes per pair, FP32 Kahan
+// per lane merged by TwoSum).  Mode 2 (L2 segments, tiers <= 2^24): the
+// row-block kernel over the regrouped CSC, blocks ordered by segment so that
+// a segment's values stay in L2.  Each pair yields one compensated partial; a
+// row with a single pair writes its value straight into the reduce buffer,
+// the others are merged per row in tier order (k_prp_tier_merge), so the
+// result does not depend on scheduling.  A pair longer than one block is cut
+// into segments merged in order by a fix-up pass.
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -764,7 +777,7 @@ __global__ void __launch_bounds__(kBlock) k_hot_fixup(const PullLong *__restrict
 }  // namespace
 
 // mode 3: split every row at its partition's window end on the device; the
-// cut into row blocks, rounds and workgroup ranges runs on the host
+// cut into wave blocks and workgroup ranges runs on the host
 static grd_status build_hotcold(grd_graph *g)
 {
   grd_ctx *ctx = g->ctx;
