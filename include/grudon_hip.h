@@ -203,9 +203,11 @@ grd_status grd_graph_write_gr(const char *path, uint32_t n, const uint32_t *row_
                               const uint32_t *weights);
 /* info[0] n, [1] m, [2] C, [3] M, [4] compute slots, [5] memory proxies,
  * [6] cross mirrors, [7] weighted, [8] device bytes held by the graph,
- * [9] pull row blocks, [10] long rows; the tiled pull's [11] (row, tier)
- * pairs, [12] tiles, [13] row blocks, [14] long pairs (0 until the first
- * pull-PageRank state of the graph). */
+ * [9] pull row blocks, [10] long rows; [11]-[14] describe the pull mode's
+ * reorganisation (0 until a state of the graph built it): modes 1 and 2
+ * [11] (row, tier) pairs, [12] tiles, [13] row blocks, [14] long pairs;
+ * mode 3 [11] rows (one partial each), [12] hot workgroup ranges, [13] hot
+ * wave blocks, [14] long hot rows. */
 grd_status grd_graph_info(const grd_graph *g, uint64_t *info16);
 /* Translation table of pair (c, m) as gids in wire-index order: the
  * reference sTranslationTable/rTranslationTable content
