@@ -254,7 +254,9 @@ struct grd_state {
   uint2 *heavy = nullptr;  // push: (proxy, first arc offset) of each heavy-vertex chunk
   void *partial = nullptr;  // pull: one 8-byte partial per row block (long-row segments)
   bool tiled = false;       // PR_PULL: the tiled pull runs (else the plain row-block pull)
-  float2 *tpart = nullptr;  // tiled pull: Kahan partial per pair, then per tier-block segment
+  // tiled pull: Kahan partial per pair, then per tier-block segment; the hot /
+  // cold pull keeps one 4-byte partial per row and per hot segment in them
+  float2 *tpart = nullptr;
   float2 *tseg = nullptr;
   float2 *cseg = nullptr;   // hot / cold pull: partial per cold row block (long-row segments)
   double *red = nullptr;    // PR pull update: (residual, dangling) per workgroup
@@ -323,9 +325,9 @@ grd_status op_ship(grd_state *s);            // edge shipping: memory send + com
 // long-row fix-up; `pre` (or null) holds a Kahan partial per row merged first
 grd_status prp_pull_rows(hipStream_t st, const PullBlock *blocks, uint32_t nblk, const uint32_t *hot_base,
                          const uint32_t *rptr, const uint32_t *col, const float *mprop, uint32_t P, float *out,
-                         float2 *partial, const float2 *pre);
+                         float2 *partial, const float *pre);
 grd_status prp_pull_rows_fixup(hipStream_t st, const PullLong *longs, uint32_t nlong, const float2 *partial,
-                               float *out, const float2 *pre);
+                               float *out, const float *pre);
 // tiers.hip
 grd_status build_tiers(grd_graph *g);
 // does this state pull through the hot / cold split (the BSP pull, CC / SSSP /

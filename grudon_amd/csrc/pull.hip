@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_rb(const PullBlock *__restr
                                                         const uint32_t *__restrict__ ccol,
                                                         const float *__restrict__ mprop, uint32_t P,
                                                         float *__restrict__ out, float2 *__restrict__ partial,
-                                                        const float2 *__restrict__ pre)
+                                                        const float *__restrict__ pre)
 {
   __shared__ float s_v[kPullNnz];
   __shared__ float s_hot[kPullHot];
@@ -139,8 +139,7 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_rb(const PullBlock *__restr
     if (sub == 0 && rl < d.nrows) {
       if (pre) {  // the row's partial from the hot pass comes first
         Kahan t;
-        t.s = pre[r].x;
-        t.c = pre[r].y;
+        t.s = pre[r];
         t.merge(acc.s, acc.c);
         out[r] = t.value();
       } else {
@@ -152,15 +151,12 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_rb(const PullBlock *__restr
 
 __global__ void __launch_bounds__(kBlock) k_prp_pull_fixup(const PullLong *__restrict__ longs, uint32_t nlong,
                                                            const float2 *__restrict__ partial, float *__restrict__ out,
-                                                           const float2 *__restrict__ pre)
+                                                           const float *__restrict__ pre)
 {
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nlong; i += gridDim.x * kBlock) {
     PullLong L = longs[i];
     Kahan t;
-    if (pre) {
-      t.s = pre[L.row].x;
-      t.c = pre[L.row].y;
-    }
+    if (pre) t.s = pre[L.row];
     for (uint32_t j = 0; j < L.nseg; j++) {
       float2 p = partial[L.first + j];
       t.merge(p.x, p.y);
@@ -329,7 +325,7 @@ grd_status prp_pull_fixup(grd_state *s)
 
 grd_status prp_pull_rows(hipStream_t st, const PullBlock *blocks, uint32_t nblk, const uint32_t *hot_base,
                          const uint32_t *rptr, const uint32_t *col, const float *mprop, uint32_t P, float *out,
-                         float2 *partial, const float2 *pre)
+                         float2 *partial, const float *pre)
 {
   if (nblk)
     hipLaunchKernelGGL(k_prp_pull_rb, dim3(nblk), dim3(kBlock), 0, st, blocks, hot_base, rptr, col, mprop, P, out,
@@ -339,7 +335,7 @@ grd_status prp_pull_rows(hipStream_t st, const PullBlock *blocks, uint32_t nblk,
 }
 
 grd_status prp_pull_rows_fixup(hipStream_t st, const PullLong *longs, uint32_t nlong, const float2 *partial,
-                               float *out, const float2 *pre)
+                               float *out, const float *pre)
 {
   if (!nlong) return GRD_OK;
   hipLaunchKernelGGL(k_prp_pull_fixup, dim3((nlong + kBlock - 1) / kBlock), dim3(kBlock), 0, st, longs, nlong,

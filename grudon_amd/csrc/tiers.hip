@@ -580,17 +580,20 @@ __device__ __forceinline__ uint32_t hot_group(uint32_t n)
 // merged by TwoSum, a (sum, compensation) pair per row.  The BSP pull (CC,
 // SSSP, reference PageRank): Op::edge of the source's broadcast value and the
 // arc weight, combined with the operator (min, or + in a fixed order).
+// The hot part of a row is a few arcs on average (2.7 at RMAT-26, M = 4), so
+// its compensated sum is kept as one FP32 value (the sum rounded once): the
+// per-row partial is 4 bytes written and read instead of 8.
 struct HotPR {
   using V = float;
-  using Part = float2;
+  using Part = float;
   using Acc = Kahan;
   static constexpr bool kWeighted = false;
   __device__ static Acc zero() { return Kahan{}; }
   __device__ static V value(V v, uint32_t) { return v; }
   __device__ static void add(Acc &a, V x) { a.add(x); }
   __device__ static void merge_lanes(Acc &a, int width) { kahan_wave_merge(a, width); }
-  __device__ static void add_part(Acc &a, Part p) { a.merge(p.x, p.y); }
-  __device__ static Part part(const Acc &a) { return make_float2(a.s, a.c); }
+  __device__ static void add_part(Acc &a, Part p) { a.merge(p, 0.0f); }
+  __device__ static Part part(const Acc &a) { return a.value(); }
 };
 template <class Op>
 struct HotBsp {
@@ -1107,16 +1110,17 @@ grd_status prp_pull_tiled(grd_state *s)
   };
   if (g->tier_mode == 3) {
     // hot prefixes from LDS into per-row partials, then the cold rest
+    float *hpart = reinterpret_cast<float *>(s->tpart), *hseg = reinterpret_cast<float *>(s->tseg);
     if (g->n_ttile)
       hipLaunchKernelGGL(k_pull_hot<HotPR>, dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile, g->tblk, g->hrp,
-                         g->tcol, (const uint32_t *)nullptr, (const float *)s->mprop, s->tpart, s->tseg);
+                         g->tcol, (const uint32_t *)nullptr, (const float *)s->mprop, hpart, hseg);
     if (g->n_tlong)
       hipLaunchKernelGGL(k_hot_fixup<HotPR>, dim3(wave_grid(g->n_tlong)), dim3(kBlock), 0, st, g->tlong, g->n_tlong,
-                         s->tseg, s->tpart);
+                         (const float *)hseg, hpart);
     GRD_HIP(hipGetLastError());
     GRD_TRY(prp_pull_rows(st, g->cblk, g->n_cblk, g->cblk_hot, g->corp, g->cocol, (const float *)s->mprop, g->P, out,
-                          s->cseg, s->tpart));
-    return prp_pull_rows_fixup(st, g->clong, g->n_clong, s->cseg, out, s->tpart);
+                          s->cseg, hpart));
+    return prp_pull_rows_fixup(st, g->clong, g->n_clong, s->cseg, out, hpart);
   }
   if (g->tier_mode == 1 && g->n_ttile)
     hipLaunchKernelGGL(k_prp_pull_tiled, dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile, g->tblk, g->trec,
