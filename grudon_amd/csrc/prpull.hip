@@ -36,31 +36,60 @@ __global__ void __launch_bounds__(kBlock) k_prp_update(uint32_t nw, const uint32
 {
   const double Dcur = cnt->dangling[parity];
   const float base = (float)((1.0 - (double)damping) * inv_n + (double)damping * Dcur * inv_n);
+  const uint32_t lane = lane_id(), stride = gridDim.x * kWavesPerBlock;
   double res = 0.0, dang = 0.0;
-  for (uint32_t w = blockIdx.x * kWavesPerBlock + wave_in_block(); w < nw; w += gridDim.x * kWavesPerBlock) {
-    uint32_t s = w * 64 + lane_id();
-    if (slot_gid[s] == 0xFFFFFFFFu) continue;
-    uint32_t pw = word_dst[w];
+  // two slot words per wave and pass; the loads that depend on nothing else
+  // (slot, degree, old rank, cross-list bounds, proxy word) are all issued
+  // before any is waited for, then the cross lists, then the partials
+  constexpr int kW = 2;
+  for (uint32_t w0 = blockIdx.x * kWavesPerBlock + wave_in_block(); w0 < nw; w0 += kW * stride) {
+    uint32_t s[kW], pw[kW], x0[kW], nx[kW], dg[kW];
+    float ro[kW];
+    bool ok[kW];
+#pragma unroll
+    for (int j = 0; j < kW; j++) {
+      const uint32_t w = w0 + j * stride;
+      ok[j] = w < nw;
+      s[j] = (ok[j] ? w : w0) * 64 + lane;
+      const uint32_t gid = slot_gid[s[j]];
+      pw[j] = word_dst[ok[j] ? w : w0];
+      x0[j] = xptr[s[j]];
+      nx[j] = xptr[s[j] + 1];
+      dg[j] = deg[s[j]];
+      ro[j] = r[s[j]];
+      ok[j] = ok[j] && gid != 0xFFFFFFFFu && pw[j] != 0xFFFFFFFFu;
+    }
     // the own partial and up to three cross-mirror partials (every vertex
     // at M <= 4) loaded together, then summed in the same order as the loop
     // that takes any further ones
-    const uint32_t x0 = xptr[s], nx = xptr[s + 1] - x0;
-    const uint32_t i0 = nx > 0 ? xlist[x0] : 0u, i1 = nx > 1 ? xlist[x0 + 1] : 0u, i2 = nx > 2 ? xlist[x0 + 2] : 0u;
-    const float own = valR[pw * 64 + lane_id()];
-    const float v0 = nx > 0 ? valR[i0] : 0.0f, v1 = nx > 1 ? valR[i1] : 0.0f, v2 = nx > 2 ? valR[i2] : 0.0f;
-    Kahan k;
-    k.add(own);
-    if (nx > 0) k.add(v0);
-    if (nx > 1) k.add(v1);
-    if (nx > 2) k.add(v2);
-    for (uint32_t x = x0 + 3; x < x0 + nx; x++) k.add(valR[xlist[x]]);
-    float rn = base + damping * k.value();
-    float ro = r[s];
-    res += fabs((double)rn - (double)ro);
-    uint32_t dg = deg[s];
-    if (dg == 0) dang += (double)rn;
-    r[s] = rn;
-    mprop[pw * 64 + lane_id()] = dg ? rn / (float)dg : 0.0f;
+    uint32_t i[kW][3];
+    float own[kW], v[kW][3];
+#pragma unroll
+    for (int j = 0; j < kW; j++) {
+      nx[j] = ok[j] ? nx[j] - x0[j] : 0u;
+#pragma unroll
+      for (int q = 0; q < 3; q++) i[j][q] = nx[j] > (uint32_t)q ? xlist[x0[j] + q] : 0u;
+      own[j] = ok[j] ? valR[pw[j] * 64 + lane] : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < kW; j++)
+#pragma unroll
+      for (int q = 0; q < 3; q++) v[j][q] = nx[j] > (uint32_t)q ? valR[i[j][q]] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < kW; j++) {
+      if (!ok[j]) continue;
+      Kahan k;
+      k.add(own[j]);
+#pragma unroll
+      for (int q = 0; q < 3; q++)
+        if (nx[j] > (uint32_t)q) k.add(v[j][q]);
+      for (uint32_t x = x0[j] + 3; x < x0[j] + nx[j]; x++) k.add(valR[xlist[x]]);
+      const float rn = base + damping * k.value();
+      res += fabs((double)rn - (double)ro[j]);
+      if (dg[j] == 0) dang += (double)rn;
+      r[s[j]] = rn;
+      mprop[pw[j] * 64 + lane] = dg[j] ? rn / (float)dg[j] : 0.0f;
+    }
   }
   res = block_sum(res);
   dang = block_sum(dang);
