@@ -159,11 +159,13 @@ grd_status prp_init(grd_state *s)
   return GRD_OK;
 }
 
-// workgroups of the update: up to 64 per CU, a few slot words per wave
+// workgroups of the update: eight per CU (every wave slot of the CU), each
+// wave striding over slot words; one (residual, dangling) pair per workgroup
+// for k_prp_finish
 uint32_t prp_grid(const grd_graph *g)
 {
   const uint32_t nwS = g->S / 64;
-  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((nwS + 3) / 4, (uint64_t)g->ctx->num_cus * 64));
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((nwS + 3) / 4, (uint64_t)g->ctx->num_cus * 8));
 }
 
 // broadcast step: the dense put of r/deg into the memory partitions' own

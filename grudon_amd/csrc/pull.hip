@@ -27,6 +27,12 @@ namespace grd {
 // per-block LDS bitmap of dirty rows (rows of one block span <= 17 words)
 constexpr int kDirtyWords = kPullMaxRows / 64 + 2;
 
+// workgroups of a wave-per-row fix-up pass
+static inline uint32_t fixup_grid(uint32_t nlong)
+{
+  return std::max<uint32_t>(1, std::min<uint32_t>(65535, (nlong + kWavesPerBlock - 1) / kWavesPerBlock));
+}
+
 __device__ __forceinline__ int group_size(uint32_t nrows)
 {
   int g = 64;
@@ -149,19 +155,27 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_rb(const PullBlock *__restr
   }
 }
 
+// a wave per long row: lane j merges segments j, j + 64, ..., then the lanes
+// are merged in a fixed order after the row's hot partial
 __global__ void __launch_bounds__(kBlock) k_prp_pull_fixup(const PullLong *__restrict__ longs, uint32_t nlong,
                                                            const float2 *__restrict__ partial, float *__restrict__ out,
                                                            const float *__restrict__ pre)
 {
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nlong; i += gridDim.x * kBlock) {
-    PullLong L = longs[i];
-    Kahan t;
-    if (pre) t.s = pre[L.row];
-    for (uint32_t j = 0; j < L.nseg; j++) {
-      float2 p = partial[L.first + j];
-      t.merge(p.x, p.y);
+  const uint32_t lane = lane_id();
+  for (uint32_t i = blockIdx.x * kWavesPerBlock + wave_in_block(); i < nlong; i += gridDim.x * kWavesPerBlock) {
+    const PullLong L = longs[i];
+    Kahan a;
+    for (uint32_t j = lane; j < L.nseg; j += kWave) {
+      const float2 p = partial[L.first + j];
+      a.merge(p.x, p.y);
     }
-    out[L.row] = t.value();
+    kahan_wave_merge(a, kWave);
+    if (lane == 0) {
+      Kahan t;
+      if (pre) t.s = pre[L.row];
+      t.merge(a.s, a.c);
+      out[L.row] = t.value();
+    }
   }
 }
 
@@ -297,12 +311,18 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_fixup(const PullLong *__res
                                                            typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty,
                                                            const typename Op::V *__restrict__ pre)
 {
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nlong; i += gridDim.x * kBlock) {
-    PullLong L = longs[i];
-    typename Op::V acc = pre ? pre[L.row] : op_identity<Op>();  // the row's hot partial first
-    for (uint32_t j = 0; j < L.nseg; j++) acc = op_combine<Op>(acc, partial[L.first + j]);
-    if (pull_apply<Op>(upd, L.row, acc))
-      atomicOr((unsigned long long *)(dirty + (L.row >> 6)), 1ull << (L.row & 63));
+  // a wave per long row, as k_prp_pull_fixup
+  const uint32_t lane = lane_id();
+  for (uint32_t i = blockIdx.x * kWavesPerBlock + wave_in_block(); i < nlong; i += gridDim.x * kWavesPerBlock) {
+    const PullLong L = longs[i];
+    typename Op::V acc = op_identity<Op>();
+    for (uint32_t j = lane; j < L.nseg; j += kWave) acc = op_combine<Op>(acc, partial[L.first + j]);
+    for (int o = kWave / 2; o > 0; o >>= 1) acc = op_combine<Op>(acc, __shfl_xor(acc, o, kWave));
+    if (lane == 0) {
+      if (pre) acc = op_combine<Op>(pre[L.row], acc);  // the row's hot partial first
+      if (pull_apply<Op>(upd, L.row, acc))
+        atomicOr((unsigned long long *)(dirty + (L.row >> 6)), 1ull << (L.row & 63));
+    }
   }
 }
 
@@ -338,7 +358,7 @@ grd_status prp_pull_rows_fixup(hipStream_t st, const PullLong *longs, uint32_t n
                                float *out, const float *pre)
 {
   if (!nlong) return GRD_OK;
-  hipLaunchKernelGGL(k_prp_pull_fixup, dim3((nlong + kBlock - 1) / kBlock), dim3(kBlock), 0, st, longs, nlong,
+  hipLaunchKernelGGL(k_prp_pull_fixup, dim3(fixup_grid(nlong)), dim3(kBlock), 0, st, longs, nlong,
                      partial, out, pre);
   GRD_HIP(hipGetLastError());
   return GRD_OK;
@@ -376,7 +396,7 @@ grd_status bsp_pull_fixup(grd_state *s)
   const bool hc = bsp_hot_cold(s);
   const uint32_t nlong = hc ? g->n_clong : g->n_plong;
   if (!nlong) return GRD_OK;
-  hipLaunchKernelGGL(k_bsp_pull_fixup<Op>, dim3((nlong + kBlock - 1) / kBlock), dim3(kBlock), 0, g->ctx->stream,
+  hipLaunchKernelGGL(k_bsp_pull_fixup<Op>, dim3(fixup_grid(nlong)), dim3(kBlock), 0, g->ctx->stream,
                      hc ? g->clong : g->plong, nlong, hc ? (const V *)s->cseg : (const V *)s->partial, (V *)s->mupd,
                      s->Du, hc ? (const V *)s->tpart : (const V *)nullptr);
   GRD_HIP(hipGetLastError());
