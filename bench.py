@@ -184,26 +184,6 @@ def cc_labels_scipy(rp, col):
     return mins[comp].astype(np.uint32)
 
 
-def sssp_bellman_ok(rp, col, w, dist, src):
-    """Exact check of shortest distances on a symmetric graph with positive
-    weights: d[src] = 0 and d[v] = min over arcs (v,u) of d[u] + w(u,v) for
-    every other v (INF when no neighbour is reached).  With positive weights
-    these equations have exactly one solution."""
-    import numpy as np
-
-    d = dist.astype(np.int64)
-    du = d[col]
-    cand = np.where(du == INF, INF, du + w.astype(np.int64))
-    del du
-    rp64 = rp.astype(np.int64)
-    nonempty = rp64[1:] > rp64[:-1]
-    mins = np.full(len(d), INF, dtype=np.int64)
-    mins[nonempty] = np.minimum.reduceat(cand, rp64[:-1][nonempty])
-    mins = np.minimum(mins, INF)
-    mins[src] = 0
-    return bool(np.array_equal(d, mins))
-
-
 # ---------------------------------------------------------------------------
 # common pieces
 # ---------------------------------------------------------------------------
@@ -428,8 +408,9 @@ def run_cc(api, ctx, args):
     # edges the traversals examined (a pull examines every arc) per second
     extra = {"work_gteps": round(last["timed_edges_examined"] / secs / 1e9, 3)}
     if not args.no_check:
-        extra["labels_bit_exact"] = bool(np.array_equal(S.result(), cc_labels_scipy(rp, col)))
-        log("labels bit-exact vs scipy components: %s" % extra["labels_bit_exact"])
+        import oracle_check
+        extra["labels_bit_exact"] = oracle_check.cc_exact(rp, col, S.result(), cpu_threads(args)[1])
+        log("labels bit-exact vs the parallel union-find: %s" % extra["labels_bit_exact"])
     roof = roofline_of(kms, kl, kb, "k_bsp_traverse", None)
     n, m = G.n, G.m
     S.free()
@@ -466,13 +447,15 @@ def run_sssp(api, ctx, args):
     # edges the traversals examined (a pull examines every arc) per second
     extra = {"work_gteps": round(last["timed_edges_examined"] / secs / 1e9, 3)}
     if not args.no_check:
+        import oracle_check
+        nt = cpu_threads(args)[1]
         ok = []
         for s in sources:
             S.init(source=int(s), **kw)
             S.run()
-            ok.append(sssp_bellman_ok(rp, col, w, S.result(), int(s)))
+            ok.append(oracle_check.sssp_exact(rp, col, w, S.result(), int(s), nt))
+            log("Bellman check, source %d: %s" % (int(s), ok[-1]))
         extra["distances_exact"] = bool(all(ok))
-        log("Bellman check over %d sources: %s" % (len(ok), ok))
     roof = roofline_of(kms, kl, kb, "k_bsp_traverse", None)
     n, m = G.n, G.m
     S.free()

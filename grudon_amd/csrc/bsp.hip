@@ -782,7 +782,10 @@ static grd_status traverse_t(grd_state *s, int option)
     if (timed) {
       GRD_HIP(hipEventRecord(s->kev[1], st));
       s->kern_pending = 1;
-      s->kern_pending_bytes = 8 * g->m + 8ull * g->P;
+      // a column and a gathered value per arc, the weight as stored (one
+      // byte in the hot / cold split when every weight fits), a value per row
+      const uint64_t wb = !cwt ? 0 : (bsp_hot_cold(s) && g->cow8) ? 1 : 4;
+      s->kern_pending_bytes = (8 + wb) * g->m + 8ull * g->P;
     }
     GRD_TRY(bsp_pull_fixup<Op>(s));
     s->edges += g->m;  // a pull examines every arc of the memory partitions

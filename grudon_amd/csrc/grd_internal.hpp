@@ -207,6 +207,8 @@ struct grd_graph {
   uint32_t *cocol = nullptr;        // [cold arcs] source proxies
   uint32_t *hw = nullptr;           // [hot arcs] arc weights (weighted graphs), as tcol
   uint32_t *cow = nullptr;          // [cold arcs] arc weights (weighted graphs), as cocol
+  uint8_t *hw8 = nullptr;           // the same as bytes, in their place when every weight is below 256
+  uint8_t *cow8 = nullptr;
   grd::PullBlock *cblk = nullptr;   // row blocks over the cold arcs
   uint32_t *cblk_hot = nullptr;     // per cold block: its LDS window base (P: none)
   uint32_t n_cblk = 0;

@@ -205,12 +205,13 @@ __device__ __forceinline__ bool pull_apply(typename Op::V *upd, uint32_t r, type
   }
 }
 
-template <class Op>
+// W: the arc weights' storage type (a byte when every weight fits)
+template <class Op, class W = uint32_t>
 __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restrict__ blocks,
                                                         const uint32_t *__restrict__ hot_base,
                                                         const uint32_t *__restrict__ crp,
                                                         const uint32_t *__restrict__ ccol,
-                                                        const uint32_t *__restrict__ cw,
+                                                        const W *__restrict__ cw,
                                                         const typename Op::V *__restrict__ mprop, uint32_t P,
                                                         typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty,
                                                         typename Op::V *__restrict__ partial,
@@ -232,7 +233,7 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
     for (int k = 0; k < kPer; k++) {
       const uint32_t i = threadIdx.x + k * kBlock;
       src[k] = i < ne ? ccol[d.e0 + i] : 0u;
-      wt[k] = (cw && i < ne) ? cw[d.e0 + i] : 1u;
+      wt[k] = (cw && i < ne) ? (uint32_t)cw[d.e0 + i] : 1u;
     }
     const uint32_t hb = hot_base[bid];
     const uint32_t hn = min(kPullHot, P - hb);
@@ -262,7 +263,7 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
     for (int k = 0; k < kPer; k++) {
       uint32_t i = threadIdx.x + k * kBlock;
       src[k] = i < ne ? ccol[d.e0 + i] : 0u;
-      wt[k] = (cw && i < ne) ? cw[d.e0 + i] : 1u;
+      wt[k] = (cw && i < ne) ? (uint32_t)cw[d.e0 + i] : 1u;
     }
     const uint32_t hb = hot_base[bid];
     const uint32_t hn = min(kPullHot, P - hb);
@@ -375,14 +376,18 @@ grd_status bsp_pull_launch(grd_state *s, const uint32_t *cw)
   using V = typename Op::V;
   if (bsp_hot_cold(s)) {
     GRD_TRY(bsp_pull_hot<Op>(s));
-    if (g->n_cblk)
-      hipLaunchKernelGGL(k_bsp_pull_rb<Op>, dim3(g->n_cblk), dim3(kBlock), 0, g->ctx->stream, g->cblk, g->cblk_hot,
-                         g->corp, g->cocol, cw ? g->cow : nullptr, (const V *)s->mprop, g->P, (V *)s->mupd, s->Du,
-                         (V *)s->cseg, (const V *)s->tpart);
+    if (g->n_cblk && cw && g->cow8)
+      hipLaunchKernelGGL((k_bsp_pull_rb<Op, uint8_t>), dim3(g->n_cblk), dim3(kBlock), 0, g->ctx->stream, g->cblk,
+                         g->cblk_hot, g->corp, g->cocol, (const uint8_t *)g->cow8, (const V *)s->mprop, g->P,
+                         (V *)s->mupd, s->Du, (V *)s->cseg, (const V *)s->tpart);
+    else if (g->n_cblk)
+      hipLaunchKernelGGL((k_bsp_pull_rb<Op, uint32_t>), dim3(g->n_cblk), dim3(kBlock), 0, g->ctx->stream, g->cblk,
+                         g->cblk_hot, g->corp, g->cocol, cw ? g->cow : nullptr, (const V *)s->mprop, g->P,
+                         (V *)s->mupd, s->Du, (V *)s->cseg, (const V *)s->tpart);
   } else {
-    hipLaunchKernelGGL(k_bsp_pull_rb<Op>, dim3(g->n_pblk), dim3(kBlock), 0, g->ctx->stream, g->pblk, g->pblk_hot,
-                       g->crp, g->ccol, cw, (const V *)s->mprop, g->P, (V *)s->mupd, s->Du, (V *)s->partial,
-                       (const V *)nullptr);
+    hipLaunchKernelGGL((k_bsp_pull_rb<Op, uint32_t>), dim3(g->n_pblk), dim3(kBlock), 0, g->ctx->stream, g->pblk,
+                       g->pblk_hot, g->crp, g->ccol, cw, (const V *)s->mprop, g->P, (V *)s->mupd, s->Du,
+                       (V *)s->partial, (const V *)nullptr);
   }
   GRD_HIP(hipGetLastError());
   return GRD_OK;

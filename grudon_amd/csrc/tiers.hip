@@ -77,6 +77,13 @@ __device__ __forceinline__ uint32_t part_of(const uint32_t *__restrict__ pb, uin
 }
 
 __global__ void k_t_fill(uint32_t *p, uint64_t n, uint32_t v) { TIER_LOOP(i, n) p[i] = v; }
+__global__ void k_w_max(uint64_t n, const uint32_t *__restrict__ w, uint32_t *__restrict__ out)
+{
+  uint32_t mx = 0;
+  TIER_LOOP(i, n) mx = max(mx, w[i]);
+  for (int o = kWave / 2; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, o, kWave));
+  if (lane_id() == 0 && mx) atomicMax(out, mx);
+}
 
 // head[crp[r]] = r for every row with arcs (positions are distinct)
 __global__ void k_t_heads(uint32_t P, const uint32_t *__restrict__ crp, uint32_t *__restrict__ head)
@@ -513,12 +520,13 @@ __global__ void k_h_split(uint32_t P, const uint32_t *__restrict__ crp, const ui
 }
 
 // a wave per row: its hot arcs as 16-bit window offsets, its cold arcs as
-// source proxies
+// source proxies, and their weights as W (a byte when every weight fits)
+template <class W>
 __global__ void k_h_copy(uint32_t P, const uint32_t *__restrict__ crp, const uint32_t *__restrict__ ccol,
                          const uint32_t *__restrict__ cw, const uint32_t *__restrict__ hrp,
                          const uint32_t *__restrict__ corp, const uint32_t *__restrict__ pb, uint32_t M,
-                         uint16_t *__restrict__ hcol, uint32_t *__restrict__ cocol, uint32_t *__restrict__ hw,
-                         uint32_t *__restrict__ cow)
+                         uint16_t *__restrict__ hcol, uint32_t *__restrict__ cocol, W *__restrict__ hw,
+                         W *__restrict__ cow)
 {
   const uint32_t lane = lane_id();
   const uint64_t stride = (uint64_t)gridDim.x * kWavesPerBlock;
@@ -530,10 +538,10 @@ __global__ void k_h_copy(uint32_t P, const uint32_t *__restrict__ crp, const uin
       const uint32_t v = ccol[e0 + i];
       if (i < nh) {
         hcol[h0 + i] = (uint16_t)(v - base);
-        if (cw) hw[h0 + i] = cw[e0 + i];
+        if (cw) hw[h0 + i] = (W)cw[e0 + i];
       } else {
         cocol[c0 + (i - nh)] = v;
-        if (cw) cow[c0 + (i - nh)] = cw[e0 + i];
+        if (cw) cow[c0 + (i - nh)] = (W)cw[e0 + i];
       }
     }
   }
@@ -612,12 +620,12 @@ struct HotBsp {
   __device__ static Part part(const Acc &a) { return a; }
 };
 
-template <class Pol>
+template <class Pol, class W = uint32_t>
 __global__ void __launch_bounds__(kTierBlock) k_pull_hot(const TierTile *__restrict__ ranges,
                                                          const PullBlock *__restrict__ blocks,
                                                          const uint32_t *__restrict__ hrp,
                                                          const uint16_t *__restrict__ hcol,
-                                                         const uint32_t *__restrict__ hw,
+                                                         const W *__restrict__ hw,
                                                          const typename Pol::V *__restrict__ mprop,
                                                          typename Pol::Part *__restrict__ hpart,
                                                          typename Pol::Part *__restrict__ hseg)
@@ -662,9 +670,12 @@ __global__ void __launch_bounds__(kTierBlock) k_pull_hot(const TierTile *__restr
 #pragma unroll
     for (int k = 0; k < kPer; k++) c[k] = __builtin_amdgcn_raw_buffer_load_b16(bc, (lane + k * kWave) * 2, 0, 0);
     if constexpr (Pol::kWeighted) {
-      const auto bw = block_buffer(hw + d.e0, ne * 4);
+      const auto bw = block_buffer(hw + d.e0, ne * (uint32_t)sizeof(W));
 #pragma unroll
-      for (int k = 0; k < kPer; k++) wt[k] = __builtin_amdgcn_raw_buffer_load_b32(bw, (lane + k * kWave) * 4, 0, 0);
+      for (int k = 0; k < kPer; k++) {
+        if constexpr (sizeof(W) == 1) wt[k] = __builtin_amdgcn_raw_buffer_load_b8(bw, lane + k * kWave, 0, 0);
+        else wt[k] = __builtin_amdgcn_raw_buffer_load_b32(bw, (lane + k * kWave) * 4, 0, 0);
+      }
     } else {
 #pragma unroll
       for (int k = 0; k < kPer; k++) wt[k] = 1u;
@@ -813,14 +824,33 @@ static grd_status build_hotcold(grd_graph *g)
   // padded: a wave block without hot arcs still reads the column at its start
   GRD_TRY(gm.alloc_n(&g->tcol, (uint64_t)hrp[P] + kWave));
   GRD_TRY(gm.alloc_n(&g->cocol, std::max<uint64_t>(corp[P], 1)));
-  if (g->cw) {  // arc weights follow their arcs (the BSP pull of weighted SSSP)
-    GRD_TRY(gm.alloc_n(&g->hw, (uint64_t)hrp[P] + kWave));
-    GRD_TRY(gm.alloc_n(&g->cow, std::max<uint64_t>(corp[P], 1)));
+  // arc weights follow their arcs (the BSP pull of weighted SSSP), one byte
+  // each when every weight is below 256
+  bool w8 = false;
+  if (g->cw) {
+    uint32_t *d_wmax = nullptr, wmax = 0;
+    GRD_TRY(tmp.alloc_n(&d_wmax, 1));
+    hipLaunchKernelGGL(k_t_fill, dim3(1), dim3(kBlock), 0, st, d_wmax, 1, 0u);
+    hipLaunchKernelGGL(k_w_max, dim3(tgrid(g->m)), dim3(kBlock), 0, st, g->m, g->cw, d_wmax);
+    GRD_HIP(hipMemcpyAsync(&wmax, d_wmax, 4, hipMemcpyDeviceToHost, st));
+    GRD_HIP(hipStreamSynchronize(st));
+    w8 = wmax < 256;
+    if (w8) {
+      GRD_TRY(gm.alloc_n(&g->hw8, (uint64_t)hrp[P] + kWave));
+      GRD_TRY(gm.alloc_n(&g->cow8, std::max<uint64_t>(corp[P], 1)));
+    } else {
+      GRD_TRY(gm.alloc_n(&g->hw, (uint64_t)hrp[P] + kWave));
+      GRD_TRY(gm.alloc_n(&g->cow, std::max<uint64_t>(corp[P], 1)));
+    }
   }
   if (P) {
     const uint64_t nb = std::min<uint64_t>(65535, ((uint64_t)P + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL(k_h_copy, dim3((uint32_t)nb), dim3(kBlock), 0, st, P, g->crp, g->ccol, g->cw, g->hrp, g->corp,
-                       d_pb, M, g->tcol, g->cocol, g->hw, g->cow);
+    if (w8)
+      hipLaunchKernelGGL(k_h_copy<uint8_t>, dim3((uint32_t)nb), dim3(kBlock), 0, st, P, g->crp, g->ccol, g->cw,
+                         g->hrp, g->corp, d_pb, M, g->tcol, g->cocol, g->hw8, g->cow8);
+    else
+      hipLaunchKernelGGL(k_h_copy<uint32_t>, dim3((uint32_t)nb), dim3(kBlock), 0, st, P, g->crp, g->ccol, g->cw,
+                         g->hrp, g->corp, d_pb, M, g->tcol, g->cocol, g->hw, g->cow);
   }
   GRD_HIP(hipGetLastError());
   // hot: wave blocks per partition, then each partition's blocks split into
@@ -1148,10 +1178,15 @@ grd_status bsp_pull_hot(grd_state *s)
   hipStream_t st = g->ctx->stream;
   using V = typename Op::V;
   V *hpart = reinterpret_cast<V *>(s->tpart), *hseg = reinterpret_cast<V *>(s->tseg);
-  const uint32_t *hw = std::is_same<Op, OpWeighted>::value ? g->hw : nullptr;
-  if (g->n_ttile)
-    hipLaunchKernelGGL((k_pull_hot<HotBsp<Op>>), dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile, g->tblk, g->hrp,
-                       g->tcol, hw, (const V *)s->mprop, hpart, hseg);
+  if (g->n_ttile) {
+    if (std::is_same<Op, OpWeighted>::value && g->hw8)
+      hipLaunchKernelGGL((k_pull_hot<HotBsp<Op>, uint8_t>), dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile,
+                         g->tblk, g->hrp, g->tcol, (const uint8_t *)g->hw8, (const V *)s->mprop, hpart, hseg);
+    else
+      hipLaunchKernelGGL((k_pull_hot<HotBsp<Op>, uint32_t>), dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile,
+                         g->tblk, g->hrp, g->tcol, std::is_same<Op, OpWeighted>::value ? g->hw : nullptr,
+                         (const V *)s->mprop, hpart, hseg);
+  }
   if (g->n_tlong) {
     const uint32_t grid =
         (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(65535, (g->n_tlong + kWavesPerBlock - 1) / kWavesPerBlock));

@@ -1,5 +1,5 @@
-"""The full-size checks of bench.py (numpy / scipy) against the oracle, and
-planted mistakes that they must catch (no GPU)."""
+"""The full-size checks of bench.py (numpy / scipy / the threaded oracle)
+against the oracle, and planted mistakes that they must catch (no GPU)."""
 import numpy as np
 
 import bench
@@ -22,22 +22,25 @@ def test_cc_labels_scipy_matches_oracle():
 
 
 def test_bellman_check_accepts_dijkstra_and_rejects_mistakes():
+    """The check bench.py runs on SSSP distances (oracle_check.sssp_exact)."""
+    import oracle_check
+
     g = O.rmat(11, 8, seed=23, symmetrize=True).with_weights(2)
     src = int(np.argmax(g.out_degree()))
     d = O.dijkstra(g, src)
-    assert bench.sssp_bellman_ok(g.rp, g.col, g.w, d, src)
+    assert oracle_check.sssp_exact(g.rp, g.col, g.w, d, src, 4)
     # a distance one too large, one too small, and a reached vertex marked unreachable
     reached = np.flatnonzero((d != bench.INF) & (np.arange(g.n) != src))
     for delta in (+1, -1):
         bad = d.astype(np.int64)
         bad[reached[5]] += delta
-        assert not bench.sssp_bellman_ok(g.rp, g.col, g.w, bad.astype(np.uint32), src)
+        assert not oracle_check.sssp_exact(g.rp, g.col, g.w, bad.astype(np.uint32), src, 4)
     bad = d.copy()
     bad[reached[-1]] = bench.INF
-    assert not bench.sssp_bellman_ok(g.rp, g.col, g.w, bad, src)
+    assert not oracle_check.sssp_exact(g.rp, g.col, g.w, bad, src, 4)
     # hop distances (unit weights) of the oracle's BSP pass the same check
     h, _ = O.Partition(g, 1, 3).sssp(src)
-    assert bench.sssp_bellman_ok(g.rp, g.col, np.ones_like(g.w), h, src)
+    assert oracle_check.sssp_exact(g.rp, g.col, np.ones_like(g.w), h, src, 4)
 
 
 def test_bench_refuses_several_gpus():

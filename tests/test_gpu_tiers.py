@@ -161,6 +161,34 @@ def test_hot_cold_bsp_pull(ctx, bsp_graphs, C, M, tb, option):
         ctx.set_pull_tiling(3, 0, 0)
 
 
+@pytest.mark.parametrize("scale_w,add_w", [(1, 0), (1000, 7), (1, 1)])
+@pytest.mark.parametrize("option", [api.OPT_PULL_SPARSE, api.OPT_PULL_DENSE])
+def test_hot_cold_weight_storage(ctx, bsp_graphs, scale_w, add_w, option):
+    """The hot / cold split keeps arc weights as bytes when every weight is
+    below 256 and as 32-bit words otherwise: weights in [1, 255] (bytes),
+    in [1007, 255007] and in [2, 256] (one weight of 256 is enough for
+    words); weighted SSSP through every-superstep pulls is Dijkstra-exact
+    with the oracle's iteration count either way."""
+    _say("hot/cold weight storage", scale_w, add_w, option)
+    gw0 = bsp_graphs[2]
+    gw = O.Graph(gw0.rp, gw0.col, gw0.w.astype(np.uint64) * scale_w + add_w)
+    assert (int(gw.w.max()) < 256) == (scale_w == 1 and add_w == 0)
+    src = int(np.argmax(gw.out_degree() > 0))
+    _, st = O.Partition(gw, 1, 4).sssp(src, weighted=True, delta=0)
+    ctx.set_pull_tiling(3, 6, 0)
+    try:
+        G = api.Graph.from_csr(ctx, gw.rp, gw.col, gw.w, C=1, M=4)
+        S = api.State(G, api.SSSP)
+        S.init(source=src, weighted=True, delta=0, max_iter=100000, sched=api.SCHED_FIXED, fixed_option=option)
+        stats = S.run()
+        assert np.array_equal(S.result(), O.dijkstra(gw, src))
+        assert stats["iterations"] == int(st[0])
+        S.free()
+        G.free()
+    finally:
+        ctx.set_pull_tiling(3, 0, 0)
+
+
 def test_bsp_pull_on_graph_tiled_by_another_mode(ctx, bsp_graphs):
     """A graph regrouped by the LDS-tiled pull first keeps that regrouping;
     a CC state created on it later pulls over the plain CSC, and is exact."""
