@@ -376,26 +376,47 @@ __global__ void __launch_bounds__(kBlock) k_reduce_recv(uint32_t nw, const uint6
                                                         const uint32_t *__restrict__ proxy_slot, int dense,
                                                         typename Op::V *upd, uint64_t *Dc)
 {
+  using V = typename Op::V;
   const uint32_t base = group_base();
   const int l = lane_id();
   const bool ok = l < (int)kGroupWords && base + l < nw;
   const uint64_t mine = ok ? bv[base + l] : 0ull;
-  uint64_t todo = __ballot(mine != 0);
-  while (todo) {
-    const int j = __ffsll((unsigned long long)todo) - 1;
-    todo &= todo - 1;
-    const uint64_t word = shfl_u64(mine, j);
-    const uint32_t w = base + j;
-    uint32_t s = 0xFFFFFFFFu;
-    typename Op::V x{};
-    if ((word >> l) & 1ull) {
-      uint32_t p = w * 64 + l;
-      x = dense ? val[p] : val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())];
-      s = proxy_slot[p];
+  const uint32_t myrank = (ok && !dense) ? wrank[base + l] : 0u;
+  if (__ballot(mine != 0) == 0) return;
+  // the group's words in every lane, then every load of the group (values,
+  // slots, and for min-combines the current values) issued before any is
+  // waited for; the relaxations and marks then follow word by word
+  uint64_t word[kGroupWords];
+  uint32_t s[kGroupWords];
+  V x[kGroupWords];
+#pragma unroll
+  for (int j = 0; j < (int)kGroupWords; j++) {
+    word[j] = shfl_u64(mine, j);
+    const uint32_t rank = __shfl(myrank, j, kWave);
+    s[j] = 0xFFFFFFFFu;
+    x[j] = V{};
+    if ((word[j] >> l) & 1ull) {
+      const uint32_t p = (base + j) * 64 + l;
+      x[j] = dense ? val[p] : val[rank + (uint32_t)__popcll(word[j] & lanemask_lt())];
+      s[j] = proxy_slot[p];
     }
+  }
+  V cur[kGroupWords];
+  if constexpr (Op::kMin) {
+#pragma unroll
+    for (int j = 0; j < (int)kGroupWords; j++)
+      cur[j] = s[j] != 0xFFFFFFFFu ? __atomic_load_n(upd + s[j], __ATOMIC_RELAXED) : V{};
+  }
+#pragma unroll
+  for (int j = 0; j < (int)kGroupWords; j++) {
+    if (word[j] == 0) continue;  // uniform over the wave
     // aggregate(): minUpdate / addUpdate, then the dirty marks of the wave
-    const bool mark = s != 0xFFFFFFFFu && relax_nomark<Op, true>(upd, s, x);
-    mark_wave(Dc, s, mark);
+    bool mark = false;
+    if (s[j] != 0xFFFFFFFFu) {
+      if constexpr (Op::kMin) mark = x[j] < cur[j] && x[j] < atomicMin(upd + s[j], x[j]);
+      else mark = relax_nomark<Op, true>(upd, s[j], x[j]);
+    }
+    mark_wave(Dc, s[j], mark);
   }
 }
 
