@@ -656,9 +656,15 @@ grd_status state_alloc(grd_state *s)
   GRD_TRY(m.alloc(&s->partial, (uint64_t)std::max<uint32_t>(1, g->n_pblk) * 8));
   if (s->algo == GRD_PR_PULL) GRD_TRY(m.alloc_n(&s->red, 2ull * prp_grid(g)));
   if (s->tiled) {
-    GRD_TRY(m.alloc_n(&s->tpart, std::max<uint64_t>(g->n_pairs, 1)));
-    GRD_TRY(m.alloc_n(&s->tseg, std::max<uint32_t>(g->n_tblk, 1)));
-    if (g->tier_mode == 3) GRD_TRY(m.alloc_n(&s->cseg, std::max<uint32_t>(g->n_cblk, 1)));
+    if (g->tier_mode == 3) {
+      // hot / cold: one 4-byte partial per row and per hot segment
+      GRD_TRY(m.alloc(reinterpret_cast<void **>(&s->tpart), std::max<uint64_t>(g->n_pairs, 1) * 4));
+      GRD_TRY(m.alloc(reinterpret_cast<void **>(&s->tseg), (uint64_t)std::max<uint32_t>(g->n_tblk, 1) * 4));
+      GRD_TRY(m.alloc_n(&s->cseg, std::max<uint32_t>(g->n_cblk, 1)));
+    } else {
+      GRD_TRY(m.alloc_n(&s->tpart, std::max<uint64_t>(g->n_pairs, 1)));
+      GRD_TRY(m.alloc_n(&s->tseg, std::max<uint32_t>(g->n_tblk, 1)));
+    }
   }
   GRD_TRY(m.alloc_n(&s->cnt, 1));
   GRD_HIP(hipHostMalloc((void **)&s->hcnt, sizeof(Counters), hipHostMallocDefault));

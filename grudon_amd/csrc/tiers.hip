@@ -11,10 +11,11 @@
 // from its partition's 2^b hottest sources (b <= 15) are a prefix of the row,
 // because rows keep their sources ascending.  The hot prefixes, with 16-bit
 // offsets into the partition's window, are reduced from a 128 KB LDS window
-// into one partial per row (k_pull_hot, under an accumulation policy: FP32
-// Kahan for pull PageRank, the BSP operator for CC / SSSP / reference
+// into one 4-byte partial per row (k_pull_hot, under an accumulation policy:
+// FP32 Kahan for pull PageRank, the BSP operator for CC / SSSP / reference
 // PageRank); the cold rest goes through the row-block kernel of pull.hip,
-// which merges the row's partial first.
+// which merges the row's partial first.  Arc weights (weighted SSSP) follow
+// their arcs, one byte each when every weight is below 256.
 //
 // modes 1 and 2, the (partition, tier, row) regrouping (pull PageRank only):
 // the partition's proxy range is cut into tiers of 2^tier_bits consecutive
@@ -585,7 +586,7 @@ __device__ __forceinline__ uint32_t hot_group(uint32_t n)
 }
 
 // How the hot pass accumulates.  Pull PageRank: FP32 Kahan per lane, lanes
-// merged by TwoSum, a (sum, compensation) pair per row.  The BSP pull (CC,
+// merged by TwoSum, the compensated sum of each row rounded once to FP32.  The BSP pull (CC,
 // SSSP, reference PageRank): Op::edge of the source's broadcast value and the
 // arc weight, combined with the operator (min, or + in a fixed order).
 // The hot part of a row is a few arcs on average (2.7 at RMAT-26, M = 4), so
