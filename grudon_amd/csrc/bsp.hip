@@ -654,7 +654,23 @@ grd_status state_alloc(grd_state *s)
   // heavy chunks: sum over heavy vertices of ceil(deg / kHeavyDeg) <= 2 m / kHeavyDeg
   GRD_TRY(m.alloc_n(&s->heavy, 2 * (g->m / kHeavyDeg) + 64));
   GRD_TRY(m.alloc(&s->partial, (uint64_t)std::max<uint32_t>(1, g->n_pblk) * 8));
-  if (s->algo == GRD_PR_PULL) GRD_TRY(m.alloc_n(&s->red, 2ull * prp_grid(g)));
+  if (s->algo == GRD_PR_PULL) {
+    GRD_TRY(m.alloc_n(&s->red, 2ull * prp_grid(g)));
+    if (!g->word_nvalid) {
+      // a pair group's valid slots are a prefix of its padded range, so the
+      // valid slots of a word are a prefix of the word
+      std::vector<uint8_t> nv(std::max<uint32_t>(nwS, 1), 0);
+      for (size_t k = 0; k < g->cbase.size(); k++) {
+        const uint64_t b0 = g->cbase[k], b1 = b0 + g->ccnt[k];
+        if (b0 % 64) return fail(GRD_ERR_STATE, "layout: a pair group does not start on a slot word");
+        for (uint64_t w = b0 / 64; w * 64 < b1 && w < nwS; w++)
+          nv[w] = (uint8_t)std::min<uint64_t>(64, b1 - std::max<uint64_t>(b0, w * 64));
+      }
+      GRD_TRY(g->mem.alloc_n(&g->word_nvalid, nv.size()));
+      GRD_HIP(hipMemcpyAsync(g->word_nvalid, nv.data(), nv.size(), hipMemcpyHostToDevice, g->ctx->stream));
+      GRD_HIP(hipStreamSynchronize(g->ctx->stream));
+    }
+  }
   if (s->tiled) {
     if (g->tier_mode == 3) {
       // hot / cold: one 4-byte partial per row and per hot segment

@@ -164,6 +164,7 @@ struct grd_graph {
   uint32_t *xptr = nullptr, *xlist = nullptr;               // [S+1], [X] cross proxies per slot
   // edge shipping (NO_OFFLOAD schedule)
   uint8_t *word_c = nullptr;     // [S/64] compute partition of each slot word
+  uint8_t *word_nvalid = nullptr;  // [S/64] valid slots at the start of each slot word (PR pull; built on demand)
   uint64_t *uncov = nullptr;     // [S/64] slots with an out-neighbour of another master (not covered)
   uint64_t n_uncovered = 0;      // vertices not covered
   // pull row blocks (pull.hip): at most kPullNnz arcs per block, longer rows

@@ -25,7 +25,7 @@ grd_status prp_pull_fixup(grd_state *s);
 
 // compute update: combine proxy partials, new rank, FP64 residual and
 // dangling mass, and the next contributions put into the own proxies
-__global__ void __launch_bounds__(kBlock) k_prp_update(uint32_t nw, const uint32_t *__restrict__ slot_gid,
+__global__ void __launch_bounds__(kBlock) k_prp_update(uint32_t nw, const uint8_t *__restrict__ word_nvalid,
                                                        const uint32_t *__restrict__ deg,
                                                        const uint32_t *__restrict__ word_dst,
                                                        const uint32_t *__restrict__ xptr,
@@ -39,8 +39,9 @@ __global__ void __launch_bounds__(kBlock) k_prp_update(uint32_t nw, const uint32
   const uint32_t lane = lane_id(), stride = gridDim.x * kWavesPerBlock;
   double res = 0.0, dang = 0.0;
   // two slot words per wave and pass; the loads that depend on nothing else
-  // (slot, degree, old rank, cross-list bounds, proxy word) are all issued
-  // before any is waited for, then the cross lists, then the partials
+  // (degree, old rank, cross-list bounds, the word's proxy word and valid
+  // count) are all issued before any is waited for, then the cross lists,
+  // then the partials.  A word's valid slots are a prefix of it.
   constexpr int kW = 2;
   for (uint32_t w0 = blockIdx.x * kWavesPerBlock + wave_in_block(); w0 < nw; w0 += kW * stride) {
     uint32_t s[kW], pw[kW], x0[kW], nx[kW], dg[kW];
@@ -51,13 +52,13 @@ __global__ void __launch_bounds__(kBlock) k_prp_update(uint32_t nw, const uint32
       const uint32_t w = w0 + j * stride;
       ok[j] = w < nw;
       s[j] = (ok[j] ? w : w0) * 64 + lane;
-      const uint32_t gid = slot_gid[s[j]];
+      const uint32_t nvalid = word_nvalid[ok[j] ? w : w0];
       pw[j] = word_dst[ok[j] ? w : w0];
       x0[j] = xptr[s[j]];
       nx[j] = xptr[s[j] + 1];
       dg[j] = deg[s[j]];
       ro[j] = r[s[j]];
-      ok[j] = ok[j] && gid != 0xFFFFFFFFu && pw[j] != 0xFFFFFFFFu;
+      ok[j] = ok[j] && lane < nvalid && pw[j] != 0xFFFFFFFFu;
     }
     // the own partial and up to three cross-mirror partials (every vertex
     // at M <= 4) loaded together, then summed in the same order as the loop
@@ -208,7 +209,7 @@ grd_status prp_update(grd_state *s)
   grd_graph *g = s->g;
   const int parity = (int)(s->iteration & 1);
   const uint32_t grid = prp_grid(g);
-  hipLaunchKernelGGL(k_prp_update, dim3(grid), dim3(kBlock), 0, g->ctx->stream, g->S / 64, g->slot_gid, g->slot_deg,
+  hipLaunchKernelGGL(k_prp_update, dim3(grid), dim3(kBlock), 0, g->ctx->stream, g->S / 64, g->word_nvalid, g->slot_deg,
                      g->word_dst, g->xptr, g->xlist, (const float *)s->valR, s->pr, (float *)s->mprop,
                      (float)s->prm.damping, 1.0 / (double)g->n, parity, s->cnt, s->red);
   hipLaunchKernelGGL(k_prp_finish, dim3(1), dim3(kBlock), 0, g->ctx->stream, (const double *)s->red, grid, parity,
