@@ -772,20 +772,28 @@ __global__ void __launch_bounds__(kTierBlock) k_pull_hot(const TierTile *__restr
   }
 }
 
-// long hot prefixes: a wave each, lane j taking segments j, j + 64, ... and
-// the lanes merged in a fixed order
+// long hot prefixes: kFixLanes lanes each (a wave holds kWave / kFixLanes
+// rows), lane j of a row taking segments j, j + kFixLanes, ... and the
+// row's lanes merged in a fixed order; the loop is uniform over the wave
+constexpr uint32_t kFixLanes = 8;
 template <class Pol>
 __global__ void __launch_bounds__(kBlock) k_hot_fixup(const PullLong *__restrict__ longs, uint32_t nlong,
                                                       const typename Pol::Part *__restrict__ hseg,
                                                       typename Pol::Part *__restrict__ hpart)
 {
-  const uint32_t lane = lane_id();
-  for (uint32_t i = blockIdx.x * kWavesPerBlock + wave_in_block(); i < nlong; i += gridDim.x * kWavesPerBlock) {
-    const PullLong L = longs[i];
+  constexpr uint32_t kRowsPerWave = kWave / kFixLanes;
+  const uint32_t lane = lane_id(), sub = lane % kFixLanes;
+  const uint32_t stride = gridDim.x * kWavesPerBlock * kRowsPerWave;
+  for (uint32_t i0 = (blockIdx.x * kWavesPerBlock + wave_in_block()) * kRowsPerWave; i0 < nlong; i0 += stride) {
+    const uint32_t i = i0 + lane / kFixLanes;
     auto a = Pol::zero();
-    for (uint32_t j = lane; j < L.nseg; j += kWave) Pol::add_part(a, hseg[L.first + j]);
-    Pol::merge_lanes(a, kWave);
-    if (lane == 0) hpart[L.row] = Pol::part(a);
+    PullLong L{0, 0, 0, 0};
+    if (i < nlong) {
+      L = longs[i];
+      for (uint32_t j = sub; j < L.nseg; j += kFixLanes) Pol::add_part(a, hseg[L.first + j]);
+    }
+    Pol::merge_lanes(a, (int)kFixLanes);
+    if (i < nlong && sub == 0) hpart[L.row] = Pol::part(a);
   }
 }
 
@@ -1146,7 +1154,7 @@ grd_status prp_pull_tiled(grd_state *s)
       hipLaunchKernelGGL(k_pull_hot<HotPR>, dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile, g->tblk, g->hrp,
                          g->tcol, (const uint32_t *)nullptr, (const float *)s->mprop, hpart, hseg);
     if (g->n_tlong)
-      hipLaunchKernelGGL(k_hot_fixup<HotPR>, dim3(wave_grid(g->n_tlong)), dim3(kBlock), 0, st, g->tlong, g->n_tlong,
+      hipLaunchKernelGGL(k_hot_fixup<HotPR>, dim3(wave_grid((g->n_tlong + 7) / 8)), dim3(kBlock), 0, st, g->tlong, g->n_tlong,
                          (const float *)hseg, hpart);
     GRD_HIP(hipGetLastError());
     GRD_TRY(prp_pull_rows(st, g->cblk, g->n_cblk, g->cblk_hot, g->corp, g->cocol, (const float *)s->mprop, g->P, out,
@@ -1189,8 +1197,8 @@ grd_status bsp_pull_hot(grd_state *s)
                          (const V *)s->mprop, hpart, hseg);
   }
   if (g->n_tlong) {
-    const uint32_t grid =
-        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(65535, (g->n_tlong + kWavesPerBlock - 1) / kWavesPerBlock));
+    const uint64_t waves = (g->n_tlong + kWave / kFixLanes - 1) / (kWave / kFixLanes);
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(65535, (waves + kWavesPerBlock - 1) / kWavesPerBlock));
     hipLaunchKernelGGL((k_hot_fixup<HotBsp<Op>>), dim3(grid), dim3(kBlock), 0, st, g->tlong, g->n_tlong, hseg, hpart);
   }
   GRD_HIP(hipGetLastError());
