@@ -261,7 +261,7 @@ def schedule_speedup(api, ctx, scale, parts, ef=16, weighted=False, reps=3, sym_
     import numpy as np
 
     wpath = os.path.join(ROOT, "grudon_amd", "selector_bf16.bin")
-    ctx.selector_load(np.fromfile(wpath, dtype="<u2"), 0.0)
+    ctx.selector_load(np.fromfile(wpath, dtype="<u2"), api.SELECTOR_MARGIN)
     out = {"graph": "RMAT-%d EF%d seed 1, 1 compute + %d memory partitions%s"
                     % (scale, ef, parts, ", weights [1,255] seed 2" if weighted else "")}
     scheds = [("plain", api.SCHED_PLAIN), ("heuristic", api.SCHED_HEURISTIC), ("learned", api.SCHED_LEARNED)]
@@ -719,7 +719,7 @@ def run_selector(api, ctx, args):
     import numpy as np
 
     w = np.fromfile(os.path.join(ROOT, "grudon_amd", "selector_bf16.bin"), dtype="<u2")
-    ctx.selector_load(w, 0.0)
+    ctx.selector_load(w, api.SELECTOR_MARGIN)
     options = [api.OPT_PUSH_SPARSE, api.OPT_PULL_SPARSE, api.OPT_PUSH_DENSE, api.OPT_PULL_DENSE, api.OPT_SHIP_EDGES]
     graphs = [("rmat22", 22), ("rmat24", 24), ("er22", 0)]
     rows = []

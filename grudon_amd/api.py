@@ -3,6 +3,7 @@
 This is the Python face of the HIP library: it raises if the library is not
 built or a call fails; there is no CPU fallback.
 """
+import atexit
 import ctypes as ct
 import os
 import weakref
@@ -19,6 +20,7 @@ OPT_PUSH_SPARSE, OPT_PULL_SPARSE, OPT_PUSH_DENSE, OPT_PULL_DENSE, OPT_SHIP_EDGES
 SCHED_PLAIN, SCHED_HEURISTIC, SCHED_LEARNED, SCHED_FIXED, SCHED_NDP_ENGINE = 0, 1, 2, 3, 4
 RMAT_ABC = (0.57, 0.19, 0.19)
 MLP_PARAMS = 5508
+SELECTOR_MARGIN = 1.0  # GRD_SELECTOR_MARGIN of include/grudon_hip.h: the packaged weights' overrule margin
 
 # every extern "C" symbol declared in include/grudon_hip.h
 EXPORTS = [
@@ -139,6 +141,21 @@ def _ptr(a):
     return None if a is None else a.ctypes.data
 
 
+_LIVE_CONTEXTS = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts():
+    """Frees every context still open when the interpreter exits (its states,
+    graphs and batches first), while the library and this module are intact,
+    rather than from finalizers run during module teardown."""
+    for c in list(_LIVE_CONTEXTS):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
 class Context:
     def __init__(self, device=0):
         h = ct.c_void_p()
@@ -146,6 +163,7 @@ class Context:
         self.h = h
         self.device = device
         self._deps = []  # weak references to the graphs, states and batches made on it
+        _LIVE_CONTEXTS.add(self)
 
     def _track(self, obj):
         self._deps.append(weakref.ref(obj))

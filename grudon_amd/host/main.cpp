@@ -18,6 +18,7 @@
 #include <type_traits>
 
 #include "GraphAlgorithm.hpp"
+#include "SelectorFile.hpp"
 #include "graph_algorithms/cc.hpp"
 #include "graph_algorithms/pr.hpp"
 #include "graph_algorithms/sssp.hpp"
@@ -221,12 +222,8 @@ int main(int argc, char **argv)
                 "[%s] Number of Threads: %zu\n", "info", o.graph_path.c_str(), "info", o.num_compute, "info",
                 o.num_memory, "info", o.threads);
     if (!o.selector.empty()) {
-      std::ifstream in(o.selector, std::ios::binary);
-      std::vector<uint16_t> w((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
-      if (!in.good() && !in.eof()) throw std::runtime_error("cannot read selector weights " + o.selector);
-      std::vector<uint16_t> ww(w.size() / 2);
-      std::memcpy(ww.data(), w.data(), ww.size() * 2);
-      if (grd_selector_load(net.context(), ww.data(), ww.size(), 0.0f) != GRD_OK)
+      const std::vector<uint16_t> w = read_selector_weights(o.selector);
+      if (grd_selector_load(net.context(), w.data(), w.size(), GRD_SELECTOR_MARGIN) != GRD_OK)
         throw std::runtime_error(std::string("selector: ") + grd_last_error());
     }
     const uint32_t node_id = 0;

@@ -301,8 +301,15 @@ grd_status grd_restore(grd_state *s);
 /* ---- schedule selector (learned MLP 16 -> 64 -> 64 -> 4) ---- */
 /* Load BF16 weights (5,508 values: W1[64][16] b1[64] W2[64][64] b2[64]
  * W3[4][64] b3[4]) into the
- * context; the learned policy falls back to the heuristic when none is loaded. */
+ * context; the learned policy falls back to the heuristic when none is loaded.
+ * The network overrules the heuristic only where its top logit exceeds the
+ * runner-up by at least min_margin. */
 grd_status grd_selector_load(grd_ctx *ctx, const uint16_t *bf16_weights, uint64_t count, float min_margin);
+/* The margin to load the packaged weights (grudon_amd/selector_bf16.bin)
+ * with: the one at which the learned schedule was fastest against the
+ * heuristic on graphs outside the training draw (RMAT-22 seed 9090,
+ * Erdos-Renyi 2^20 seed 8080; PageRank, CC, hop and weighted SSSP). */
+#define GRD_SELECTOR_MARGIN 1.0f
 /* Train on the GPU with the project's forward/backward kernels: feat[n*16],
  * label[n] in 0..3, cost[n*4] (per-option times, used to weight the loss).
  * Writes the trained BF16 weights to out (may be NULL) and loads them. */
