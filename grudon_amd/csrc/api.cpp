@@ -763,7 +763,7 @@ grd_status grd_run(grd_state *s, grd_run_stats *stats, grd_iter_rec *trace, uint
         } else {
           float margin = 0.0f;
           int o = h;
-          if (s->sel_queued) mlp_select_read(ctx, &o, &margin);
+          if (s->sel_queued) mlp_select_read(s, &o, &margin);
           else GRD_TRY(mlp_select(ctx, feat, &o, &margin));
           if (o != h && margin < ctx->sel_margin) {
             option = h;  // not confident enough to overrule the heuristic

@@ -85,6 +85,7 @@ struct Counters {
   unsigned long long frontier_uncovered;  // frontier vertices with an out-neighbour of another master
   unsigned long long ship_error;      // edge shipping: the active list disagreed with the sized wire
   unsigned long long pad;
+  float sel[8];                       // learned selector: the next superstep's 4 logits and argmax
   unsigned long long fpart[64];       // frontier size of each compute partition (C > 1)
 };
 
@@ -363,6 +364,7 @@ grd_status mlp_select(grd_ctx *ctx, const float *feat16_host, int *option, float
 // readback of the superstep also brings the next decision.  prev_frontier is
 // the frontier size before the superstep.
 grd_status mlp_select_queue(grd_state *s, uint64_t prev_frontier);
-// Reads the decision queued by mlp_select_queue (after the stream sync).
-void mlp_select_read(const grd_ctx *ctx, int *option, float *margin);
+// Reads the decision queued by mlp_select_queue from the state's counters
+// (after they were read back).
+void mlp_select_read(const grd_state *s, int *option, float *margin);
 }  // namespace grd

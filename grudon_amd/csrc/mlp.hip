@@ -270,8 +270,9 @@ struct SelInputs {
   uint32_t M, C, algo, pad;
 };
 
-__global__ void k_mlp_select_state(const uint16_t *__restrict__ wb, const Counters *__restrict__ cnt, SelInputs in,
-                                   float *__restrict__ out)
+// the logits and the argmax go into the counters (cnt->sel), so that the
+// superstep's counter read-back brings the decision along
+__global__ void k_mlp_select_state(const uint16_t *__restrict__ wb, Counters *cnt, SelInputs in)
 {
   __shared__ float s_x[kIn];
   __shared__ float s_h1[kWavesPerBlock][kH];
@@ -304,10 +305,10 @@ __global__ void k_mlp_select_state(const uint16_t *__restrict__ wb, const Counte
   if (threadIdx.x == 0) {
     int best = 0;
     for (int o = 0; o < kOut; o++) {
-      out[o] = r.logit[o];
+      cnt->sel[o] = r.logit[o];
       if (r.logit[o] > r.logit[best]) best = o;
     }
-    out[4] = (float)best;
+    cnt->sel[4] = (float)best;
   }
 }
 
@@ -336,16 +337,14 @@ grd_status mlp_select_queue(grd_state *s, uint64_t prev_frontier)
 {
   grd_graph *g = s->g;
   grd_ctx *ctx = g->ctx;
-  GRD_TRY(sel_buffers(ctx));
   SelInputs in{(double)g->n, (double)g->m, (double)g->P, (double)g->X, g->avg_degree, (double)prev_frontier,
                (uint32_t)g->M, (uint32_t)g->C, (uint32_t)s->algo, 0u};
-  hipLaunchKernelGGL(k_mlp_select_state, dim3(1), dim3(kWave), 0, ctx->stream, ctx->sel_w, s->cnt, in, ctx->sel_out);
+  hipLaunchKernelGGL(k_mlp_select_state, dim3(1), dim3(kWave), 0, ctx->stream, ctx->sel_w, s->cnt, in);
   GRD_HIP(hipGetLastError());
-  GRD_HIP(hipMemcpyAsync(ctx->sel_out_host, ctx->sel_out, 8 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   return GRD_OK;
 }
 
-void mlp_select_read(const grd_ctx *ctx, int *option, float *margin) { decode_logits(ctx->sel_out_host, option, margin); }
+void mlp_select_read(const grd_state *s, int *option, float *margin) { decode_logits(s->hcnt->sel, option, margin); }
 
 grd_status mlp_select(grd_ctx *ctx, const float *feat16, int *option, float *margin)
 {
