@@ -70,8 +70,13 @@ Options parse(int argc, char **argv)
   auto num = [](const std::string &s, const std::string &name) -> unsigned long long {
     char *end = nullptr;
     unsigned long long v = std::strtoull(s.c_str(), &end, 10);
-    if (s.empty() || *end) throw std::runtime_error("Argument '" + s + "' failed to parse for option " + name);
+    if (s.empty() || *end || s[0] == '-') throw std::runtime_error("Argument '" + s + "' failed to parse for option " + name);
     return v;
+  };
+  auto num32 = [&](const std::string &s, const std::string &name) -> uint32_t {
+    const unsigned long long v = num(s, name);
+    if (v > 0xFFFFFFFFull) throw std::runtime_error("Argument '" + s + "' failed to parse for option " + name);
+    return (uint32_t)v;
   };
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
@@ -84,15 +89,15 @@ Options parse(int argc, char **argv)
     else if (a == "-h" || a == "--help") o.help = true;
     else if (a == "--sched") o.sched = need(i, a);
     else if (a == "--selector") o.selector = need(i, a);
-    else if (a == "--source") o.source = (uint32_t)num(need(i, a), a);
+    else if (a == "--source") o.source = num32(need(i, a), a);
     else if (a == "--weighted") o.weighted = true;
-    else if (a == "--delta") o.delta = (uint32_t)num(need(i, a), a);
-    else if (a == "--max-iter") o.max_iter = (uint32_t)num(need(i, a), a);
+    else if (a == "--delta") o.delta = num32(need(i, a), a);
+    else if (a == "--max-iter") o.max_iter = num32(need(i, a), a);
     else if (a == "--print-state") o.print_state = true;
     else if (a == "--verify") o.verify = true;
     else if (a == "--write-gr") o.write_gr = need(i, a);
     else if (a == "--result") o.result = need(i, a);
-    else if (a == "--value-bytes") o.value_bytes = (uint32_t)num(need(i, a), a);
+    else if (a == "--value-bytes") o.value_bytes = num32(need(i, a), a);
     else throw std::runtime_error("Option '" + a + "' does not exist");
   }
   return o;
@@ -209,7 +214,12 @@ int main(int argc, char **argv)
   else if (o.sched == "ndp") sched = GRD_SCHED_NDP_ENGINE;
   else if (o.sched.rfind("fixed:", 0) == 0) {
     sched = GRD_SCHED_FIXED;
-    fixed = std::atoi(o.sched.c_str() + 6);
+    const std::string n = o.sched.substr(6);
+    if (n.size() != 1 || n[0] < '0' || n[0] > '4') {
+      std::cerr << "Error parsing options: fixed schedule option must be 0..4, not '" << n << "'" << std::endl;
+      return 1;
+    }
+    fixed = n[0] - '0';
   } else if (o.sched != "plain") {
     std::cerr << "Error parsing options: unknown schedule '" << o.sched << "'" << std::endl;
     return 1;

@@ -71,6 +71,11 @@ def test_cli_help():
     (("-g", "x.gr", "-c", "1", "-m", "1", "--nope"), "Error parsing options"),
     (("-g", "x.gr", "-c", "one", "-m", "1"), "Error parsing options"),
     (("-g", "x.gr", "-c", "1", "-m", "1", "-a", "bfs"), "unknown algorithm"),
+    (("-g", "x.gr", "-c", "1", "-m", "1", "--source", "4294967296"), "failed to parse for option --source"),
+    (("-g", "x.gr", "-c", "1", "-m", "1", "--delta", "-3"), "failed to parse for option --delta"),
+    (("-g", "x.gr", "-c", "1", "-m", "1", "--sched", "fixed:abc"), "fixed schedule option must be 0..4"),
+    (("-g", "x.gr", "-c", "1", "-m", "1", "--sched", "fixed:9"), "fixed schedule option must be 0..4"),
+    (("-g", "x.gr", "-c", "1", "-m", "1", "--value-bytes", "6"), "--value-bytes must be 4 or 8"),
 ])
 def test_cli_argument_errors(args, msg):
     rc, out = _cli(*args)
