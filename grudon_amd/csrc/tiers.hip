@@ -897,7 +897,7 @@ static grd_status build_hotcold(grd_graph *g)
     total += bcost.back();
   }
   std::vector<TierTile> ranges;
-  const uint64_t nwg = 8ull * (uint64_t)std::max(1, ctx->num_cus);  // eight per CU, in turn: dynamic balance
+  const uint64_t nwg = 16ull * (uint64_t)std::max(1, ctx->num_cus);  // sixteen per CU, in turn: dynamic balance
   for (uint32_t mp = 0; mp < M; mp++) {
     const uint32_t b0 = pblk0[mp], b1 = pblk0[mp + 1];
     if (b0 == b1) continue;
