@@ -167,6 +167,7 @@ struct grd_graph {
   uint8_t *word_c = nullptr;     // [S/64] compute partition of each slot word
   uint8_t *word_nvalid = nullptr;  // [S/64] valid slots at the start of each slot word (PR pull; built on demand)
   uint64_t *uncov = nullptr;     // [S/64] slots with an out-neighbour of another master (not covered)
+  uint32_t *scol = nullptr;      // [m] push CSR columns as compute slots (shipped records; built on first use)
   uint64_t n_uncovered = 0;      // vertices not covered
   // pull row blocks (pull.hip): at most kPullNnz arcs per block, longer rows
   // split into segments merged by a fix-up pass

@@ -163,13 +163,23 @@ __device__ __forceinline__ uint32_t ship_owner(const uint32_t *s_off, uint32_t n
   return lo;
 }
 
+__global__ void __launch_bounds__(kBlock) k_slot_col(uint64_t m, const uint32_t *__restrict__ col,
+                                                     const uint32_t *__restrict__ proxy_slot, uint32_t *__restrict__ scol)
+{
+  for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < m; e += (uint64_t)gridDim.x * kBlock)
+    scol[e] = proxy_slot[col[e]];
+}
+
 // Memory side (GraphAlgorithm.cpp:426-474): position 0 of a record is the
 // source's slot, position r > 0 the slot of its r-th out-neighbour (with the
 // arc weight in wbuf for weighted SSSP; the header's weight slot is 0).
+// The out-neighbours' slots come from scol, the push CSR with every column
+// translated to its compute slot once (a streaming read per shipped arc, not
+// a gather from the proxy -> slot table).
 __global__ void __launch_bounds__(kBlock) k_ship_send(const uint32_t *__restrict__ AL, const uint32_t *__restrict__ off,
                                                       const uint32_t *__restrict__ tile, uint32_t k, uint32_t total,
                                                       uint32_t ntiles, const uint32_t *__restrict__ rp,
-                                                      const uint32_t *__restrict__ col, const uint32_t *__restrict__ wt,
+                                                      const uint32_t *__restrict__ scol, const uint32_t *__restrict__ wt,
                                                       const uint32_t *__restrict__ proxy_slot,
                                                       uint32_t *__restrict__ buf, uint32_t *__restrict__ wbuf)
 {
@@ -196,7 +206,7 @@ __global__ void __launch_bounds__(kBlock) k_ship_send(const uint32_t *__restrict
           x = s_src[i];
         } else {
           const uint32_t e = s_beg[i] + r - 1;
-          x = proxy_slot[col[e]];
+          x = scol[e];
           if (wbuf) wx = wt[e];
         }
         buf[q] = x;
@@ -345,10 +355,16 @@ static grd_status ship_t(grd_state *s)
                      s->ship_tile, s->cnt);
   const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(ntiles, (uint32_t)ctx->num_cus * 8));
   const uint32_t *wt = weighted ? g->w : nullptr;
+  if (!g->scol) {  // the push CSR's columns as compute slots, once per graph
+    GRD_TRY(g->mem.alloc_n(&g->scol, std::max<uint64_t>(g->m, 1)));
+    hipLaunchKernelGGL(k_slot_col, dim3(grid_for(g->m, 1u << 16)), dim3(kBlock), 0, st, g->m, g->col, g->proxy_slot,
+                       g->scol);
+    GRD_HIP(hipGetLastError());
+  }
   const bool timed = s->prm.time_kernel != 0;
   if (timed) GRD_HIP(hipEventRecord(s->kev[0], st));
   hipLaunchKernelGGL(k_ship_send, dim3(grid), dim3(kBlock), 0, st, s->AL, off, s->ship_tile, (uint32_t)k,
-                     (uint32_t)total, ntiles, g->rp, g->col, wt, g->proxy_slot, s->ship_buf,
+                     (uint32_t)total, ntiles, g->rp, g->scol, wt, g->proxy_slot, s->ship_buf,
                      weighted ? s->ship_w : nullptr);
   hipLaunchKernelGGL((k_ship_recv<Op>), dim3(grid), dim3(kBlock), 0, st, off, s->ship_tile, (uint32_t)k,
                      (uint32_t)total, ntiles, s->ship_buf, weighted ? s->ship_w : nullptr, (const V *)s->prop,
