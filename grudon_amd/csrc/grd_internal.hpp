@@ -168,6 +168,8 @@ struct grd_graph {
   uint8_t *word_nvalid = nullptr;  // [S/64] valid slots at the start of each slot word (PR pull; built on demand)
   uint64_t *uncov = nullptr;     // [S/64] slots with an out-neighbour of another master (not covered)
   uint32_t *scol = nullptr;      // [m] push CSR columns as compute slots (shipped records; built on first use)
+  uint8_t *w8 = nullptr;         // [m] push CSR weights as bytes, when every weight is below 256 (shipping)
+  int ship_wbytes = 0;           // bytes per shipped weight: 0 not decided yet, 1 or 4
   uint64_t n_uncovered = 0;      // vertices not covered
   // pull row blocks (pull.hip): at most kPullNnz arcs per block, longer rows
   // split into segments merged by a fix-up pass
@@ -268,7 +270,8 @@ struct grd_state {
   // edge shipping wire (grown on demand): record lengths / offsets, the first
   // record of every tile, the records (gids) and their edge weights
   grd::DevMem ship_mem;
-  uint32_t *ship_off = nullptr, *ship_tile = nullptr, *ship_buf = nullptr, *ship_w = nullptr;
+  uint32_t *ship_off = nullptr, *ship_tile = nullptr, *ship_buf = nullptr;
+  void *ship_w = nullptr;  // the records' weights, ship_wbytes each
   uint64_t ship_kcap = 0, ship_cap = 0;
   void *ship_tmp = nullptr;  // rocPRIM scan storage (in ship_mem)
   size_t ship_tmp_bytes = 0;

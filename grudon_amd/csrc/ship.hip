@@ -163,6 +163,20 @@ __device__ __forceinline__ uint32_t ship_owner(const uint32_t *s_off, uint32_t n
   return lo;
 }
 
+__global__ void __launch_bounds__(kBlock) k_ship_wmax(uint64_t m, const uint32_t *__restrict__ w, uint32_t *__restrict__ out)
+{
+  uint32_t mx = 0;
+  for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < m; e += (uint64_t)gridDim.x * kBlock) mx = max(mx, w[e]);
+  for (int o = kWave / 2; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, o, kWave));
+  if (lane_id() == 0 && mx) atomicMax(out, mx);
+}
+
+__global__ void __launch_bounds__(kBlock) k_ship_w8(uint64_t m, const uint32_t *__restrict__ w, uint8_t *__restrict__ w8)
+{
+  for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < m; e += (uint64_t)gridDim.x * kBlock)
+    w8[e] = (uint8_t)w[e];
+}
+
 __global__ void __launch_bounds__(kBlock) k_slot_col(uint64_t m, const uint32_t *__restrict__ col,
                                                      const uint32_t *__restrict__ proxy_slot, uint32_t *__restrict__ scol)
 {
@@ -176,12 +190,14 @@ __global__ void __launch_bounds__(kBlock) k_slot_col(uint64_t m, const uint32_t 
 // The out-neighbours' slots come from scol, the push CSR with every column
 // translated to its compute slot once (a streaming read per shipped arc, not
 // a gather from the proxy -> slot table).
+// W: the weights' storage, a byte when every weight of the graph fits.
+template <class W>
 __global__ void __launch_bounds__(kBlock) k_ship_send(const uint32_t *__restrict__ AL, const uint32_t *__restrict__ off,
                                                       const uint32_t *__restrict__ tile, uint32_t k, uint32_t total,
                                                       uint32_t ntiles, const uint32_t *__restrict__ rp,
-                                                      const uint32_t *__restrict__ scol, const uint32_t *__restrict__ wt,
+                                                      const uint32_t *__restrict__ scol, const W *__restrict__ wt,
                                                       const uint32_t *__restrict__ proxy_slot,
-                                                      uint32_t *__restrict__ buf, uint32_t *__restrict__ wbuf)
+                                                      uint32_t *__restrict__ buf, W *__restrict__ wbuf)
 {
   __shared__ uint32_t s_off[kShipTile + 2];
   __shared__ uint32_t s_beg[kShipTile + 1];
@@ -201,7 +217,8 @@ __global__ void __launch_bounds__(kBlock) k_ship_send(const uint32_t *__restrict
       for (uint32_t q = q0 + threadIdx.x; q < q1; q += kBlock) {
         const uint32_t i = ship_owner(s_off, nr, q);
         const uint32_t r = q - s_off[i];
-        uint32_t x, wx = 0;
+        uint32_t x;
+        W wx = 0;
         if (r == 0) {
           x = s_src[i];
         } else {
@@ -222,11 +239,11 @@ __global__ void __launch_bounds__(kBlock) k_ship_send(const uint32_t *__restrict
 // applies addUpdate here whatever the algorithm (a TODO at :624); this applies
 // the algorithm's own aggregate, which is what makes the schedule give the
 // same result as the offloaded one.
-template <class Op>
+template <class Op, class W>
 __global__ void __launch_bounds__(kBlock) k_ship_recv(const uint32_t *__restrict__ off,
                                                       const uint32_t *__restrict__ tile, uint32_t k, uint32_t total,
                                                       uint32_t ntiles, const uint32_t *__restrict__ buf,
-                                                      const uint32_t *__restrict__ wbuf,
+                                                      const W *__restrict__ wbuf,
                                                       const typename Op::V *__restrict__ prop, typename Op::V *upd,
                                                       uint64_t *Dc)
 {
@@ -246,7 +263,7 @@ __global__ void __launch_bounds__(kBlock) k_ship_recv(const uint32_t *__restrict
       for (uint32_t q = q0 + threadIdx.x; q < q1; q += kBlock) {
         const uint32_t i = ship_owner(s_off, nr, q);
         if (q == s_off[i]) continue;  // the record's header
-        relax<Op, true>(upd, Dc, buf[q], Op::edge(s_val[i], wbuf ? wbuf[q] : 1u));
+        relax<Op, true>(upd, Dc, buf[q], Op::edge(s_val[i], wbuf ? (uint32_t)wbuf[q] : 1u));
       }
     __syncthreads();
   }
@@ -305,13 +322,14 @@ grd_status op_frontier_parts(grd_state *s)
 
 // wire buffers for k records over `total` positions; grown (never shrunk)
 // outside any launch in flight
-static grd_status ship_reserve(grd_state *s, uint64_t k, uint64_t total, bool weighted)
+static grd_status ship_reserve(grd_state *s, uint64_t k, uint64_t total, bool weighted, uint32_t wbytes)
 {
   if (k + 1 <= s->ship_kcap && total <= s->ship_cap && (!weighted || s->ship_w)) return GRD_OK;
   hipStream_t st = s->g->ctx->stream;
   GRD_HIP(hipStreamSynchronize(st));
   s->ship_mem.release();
-  s->ship_off = s->ship_tile = s->ship_buf = s->ship_w = nullptr;
+  s->ship_off = s->ship_tile = s->ship_buf = nullptr;
+  s->ship_w = nullptr;
   s->ship_tmp = nullptr;
   s->ship_tmp_bytes = 0;
   const uint64_t kc = std::max<uint64_t>(k + 1 + (k + 1) / 4, 1024), cap = std::max<uint64_t>(total + total / 4, 4096);
@@ -320,7 +338,7 @@ static grd_status ship_reserve(grd_state *s, uint64_t k, uint64_t total, bool we
   GRD_TRY(s->ship_mem.alloc_n(&s->ship_off, 2 * kc));
   GRD_TRY(s->ship_mem.alloc_n(&s->ship_tile, tc));
   GRD_TRY(s->ship_mem.alloc_n(&s->ship_buf, cap));
-  if (weighted) GRD_TRY(s->ship_mem.alloc_n(&s->ship_w, cap));
+  if (weighted) GRD_TRY(s->ship_mem.alloc(&s->ship_w, cap * wbytes));
   s->ship_kcap = kc;
   s->ship_cap = cap;
   return GRD_OK;
@@ -338,7 +356,23 @@ static grd_status ship_t(grd_state *s)
   const uint64_t total = k + E;
   if (total > 0xFFFFFFFFull - kShipTile) return fail(GRD_ERR_ARG, "edge shipping: the wire would exceed 2^32 entries");
   const bool weighted = std::is_same<Op, OpWeighted>::value;
-  GRD_TRY(ship_reserve(s, k, total, weighted));
+  if (weighted && !g->ship_wbytes) {
+    // the weights travel as bytes when every one of them is below 256
+    uint32_t *d_max = nullptr, wmax = 0;
+    GRD_TRY(g->mem.alloc_n(&d_max, 1));
+    GRD_HIP(hipMemsetAsync(d_max, 0, 4, st));
+    hipLaunchKernelGGL(k_ship_wmax, dim3(grid_for(g->m, 1u << 16)), dim3(kBlock), 0, st, g->m, g->w, d_max);
+    GRD_HIP(hipMemcpyAsync(&wmax, d_max, 4, hipMemcpyDeviceToHost, st));
+    GRD_HIP(hipStreamSynchronize(st));
+    if (wmax < 256) {
+      GRD_TRY(g->mem.alloc_n(&g->w8, std::max<uint64_t>(g->m, 1)));
+      hipLaunchKernelGGL(k_ship_w8, dim3(grid_for(g->m, 1u << 16)), dim3(kBlock), 0, st, g->m, g->w, g->w8);
+      GRD_HIP(hipGetLastError());
+    }
+    g->ship_wbytes = wmax < 256 ? 1 : 4;
+  }
+  const uint32_t wbytes = weighted ? (uint32_t)g->ship_wbytes : 4u;
+  GRD_TRY(ship_reserve(s, k, total, weighted, wbytes));
   uint32_t *len = s->ship_off + s->ship_kcap, *off = s->ship_off;
   const uint32_t ntiles = (uint32_t)((total + kShipTile - 1) / kShipTile);
   hipLaunchKernelGGL(k_ship_len, dim3(grid_for(k + 1)), dim3(kBlock), 0, st, s->AL, s->totals + 0, (uint32_t)k, g->rp,
@@ -354,7 +388,6 @@ static grd_status ship_t(grd_state *s)
   hipLaunchKernelGGL(k_ship_tiles, dim3(grid_for(k)), dim3(kBlock), 0, st, off, (uint32_t)k, (uint32_t)total,
                      s->ship_tile, s->cnt);
   const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(ntiles, (uint32_t)ctx->num_cus * 8));
-  const uint32_t *wt = weighted ? g->w : nullptr;
   if (!g->scol) {  // the push CSR's columns as compute slots, once per graph
     GRD_TRY(g->mem.alloc_n(&g->scol, std::max<uint64_t>(g->m, 1)));
     hipLaunchKernelGGL(k_slot_col, dim3(grid_for(g->m, 1u << 16)), dim3(kBlock), 0, st, g->m, g->col, g->proxy_slot,
@@ -363,21 +396,30 @@ static grd_status ship_t(grd_state *s)
   }
   const bool timed = s->prm.time_kernel != 0;
   if (timed) GRD_HIP(hipEventRecord(s->kev[0], st));
-  hipLaunchKernelGGL(k_ship_send, dim3(grid), dim3(kBlock), 0, st, s->AL, off, s->ship_tile, (uint32_t)k,
-                     (uint32_t)total, ntiles, g->rp, g->scol, wt, g->proxy_slot, s->ship_buf,
-                     weighted ? s->ship_w : nullptr);
-  hipLaunchKernelGGL((k_ship_recv<Op>), dim3(grid), dim3(kBlock), 0, st, off, s->ship_tile, (uint32_t)k,
-                     (uint32_t)total, ntiles, s->ship_buf, weighted ? s->ship_w : nullptr, (const V *)s->prop,
-                     (V *)s->upd, s->Dc);
+  if (weighted && wbytes == 1) {
+    hipLaunchKernelGGL(k_ship_send<uint8_t>, dim3(grid), dim3(kBlock), 0, st, s->AL, off, s->ship_tile, (uint32_t)k,
+                       (uint32_t)total, ntiles, g->rp, g->scol, (const uint8_t *)g->w8, g->proxy_slot, s->ship_buf,
+                       (uint8_t *)s->ship_w);
+    hipLaunchKernelGGL((k_ship_recv<Op, uint8_t>), dim3(grid), dim3(kBlock), 0, st, off, s->ship_tile, (uint32_t)k,
+                       (uint32_t)total, ntiles, s->ship_buf, (const uint8_t *)s->ship_w, (const V *)s->prop,
+                       (V *)s->upd, s->Dc);
+  } else {
+    hipLaunchKernelGGL(k_ship_send<uint32_t>, dim3(grid), dim3(kBlock), 0, st, s->AL, off, s->ship_tile, (uint32_t)k,
+                       (uint32_t)total, ntiles, g->rp, g->scol, weighted ? (const uint32_t *)g->w : nullptr,
+                       g->proxy_slot, s->ship_buf, weighted ? (uint32_t *)s->ship_w : nullptr);
+    hipLaunchKernelGGL((k_ship_recv<Op, uint32_t>), dim3(grid), dim3(kBlock), 0, st, off, s->ship_tile, (uint32_t)k,
+                       (uint32_t)total, ntiles, s->ship_buf, weighted ? (const uint32_t *)s->ship_w : nullptr,
+                       (const V *)s->prop, (V *)s->upd, s->Dc);
+  }
   GRD_HIP(hipGetLastError());
   if (timed) {
-    // per arc: col, the slot lookup, wire write + read, the atomic
-    // read-modify-write (+ weight read, write, read); per record: active
-    // list, row bounds, offsets (twice), slot lookup, header write + read,
-    // property
+    // per arc: its slot (the slot column), wire write + read, the atomic
+    // read-modify-write (+ weight read, write, read, at its stored width);
+    // per record: active list, row bounds, offsets (twice), slot lookup,
+    // header write + read, property
     GRD_HIP(hipEventRecord(s->kev[1], st));
     s->kern_pending = 1;
-    s->kern_pending_bytes = (weighted ? 36 : 24) * E + 36 * k;
+    s->kern_pending_bytes = (20 + (weighted ? 3ull * wbytes : 0ull)) * E + 36 * k;
   }
   s->edges += E;
   // wire bytes (MPI.hpp:44 counts count * sizeof(GNode) per send): one id

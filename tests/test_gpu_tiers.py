@@ -162,13 +162,14 @@ def test_hot_cold_bsp_pull(ctx, bsp_graphs, C, M, tb, option):
 
 
 @pytest.mark.parametrize("scale_w,add_w", [(1, 0), (1000, 7), (1, 1)])
-@pytest.mark.parametrize("option", [api.OPT_PULL_SPARSE, api.OPT_PULL_DENSE])
+@pytest.mark.parametrize("option", [api.OPT_PULL_SPARSE, api.OPT_PULL_DENSE, api.OPT_SHIP_EDGES])
 def test_hot_cold_weight_storage(ctx, bsp_graphs, scale_w, add_w, option):
-    """The hot / cold split keeps arc weights as bytes when every weight is
-    below 256 and as 32-bit words otherwise: weights in [1, 255] (bytes),
-    in [1007, 255007] and in [2, 256] (one weight of 256 is enough for
-    words); weighted SSSP through every-superstep pulls is Dijkstra-exact
-    with the oracle's iteration count either way."""
+    """The hot / cold split, and the edge-shipping wire, keep arc weights as
+    bytes when every weight is below 256 and as 32-bit words otherwise:
+    weights in [1, 255] (bytes), in [1007, 255007] and in [2, 256] (one
+    weight of 256 is enough for words); weighted SSSP through
+    every-superstep pulls or shipped supersteps is Dijkstra-exact with the
+    oracle's iteration count either way."""
     _say("hot/cold weight storage", scale_w, add_w, option)
     gw0 = bsp_graphs[2]
     gw = O.Graph(gw0.rp, gw0.col, gw0.w.astype(np.uint64) * scale_w + add_w)
