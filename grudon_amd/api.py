@@ -42,7 +42,7 @@ class Params(ct.Structure):
     _fields_ = [("source", ct.c_uint32), ("weighted", ct.c_uint32), ("delta", ct.c_uint32),
                 ("max_iter", ct.c_uint32), ("damping", ct.c_double), ("tolerance", ct.c_double),
                 ("sched", ct.c_uint32), ("fixed_option", ct.c_uint32), ("time_iters", ct.c_uint32),
-                ("time_kernel", ct.c_uint32), ("offload_only", ct.c_uint32)]
+                ("time_kernel", ct.c_uint32), ("offload_only", ct.c_uint32), ("pull_every_row", ct.c_uint32)]
 
 
 class IterRec(ct.Structure):
@@ -379,9 +379,10 @@ class State:
         _check(lib().grd_aggregate_one(self.h, compute_partition, lid, _ptr(v)), "grd_aggregate_one")
 
     def init(self, source=0, weighted=False, delta=0, max_iter=1000, damping=0.85, tolerance=1e-6,
-             sched=SCHED_PLAIN, fixed_option=0, time_iters=False, time_kernel=False, offload_only=False):
+             sched=SCHED_PLAIN, fixed_option=0, time_iters=False, time_kernel=False, offload_only=False,
+             pull_every_row=False):
         p = Params(source, int(weighted), delta, max_iter, damping, tolerance, sched, fixed_option,
-                   int(time_iters), int(time_kernel), int(offload_only))
+                   int(time_iters), int(time_kernel), int(offload_only), int(pull_every_row))
         _check(lib().grd_init(self.h, ct.byref(p)), "grd_init")
 
     def run(self, trace_cap=0):

@@ -136,6 +136,16 @@ static uint64_t bitvector_bytes(const grd_graph *g)
 // event time and algorithmic bytes (frontier_k = vertices pushed)
 static grd_status finish_kernel_timing(grd_state *s, uint64_t frontier_k)
 {
+  if (s->skip_pending) {
+    // a CC / SSSP pull left out the blocks whose rows could not improve
+    const uint64_t arcs = s->hcnt->pull_skipped, rows = s->hcnt->pull_skipped_rows;
+    s->edges -= std::min<uint64_t>(arcs, s->edges);
+    if (s->kern_pending == 1) {
+      const uint64_t off = s->skip_arc_bytes * arcs + 8 * rows;
+      s->kern_pending_bytes -= std::min<uint64_t>(off, s->kern_pending_bytes);
+    }
+    s->skip_pending = false;
+  }
   if (!s->kern_pending) return GRD_OK;
   float ms = 0.0f;
   GRD_HIP(hipEventSynchronize(s->kev[1]));

@@ -197,24 +197,61 @@ __global__ void __launch_bounds__(kBlock) k_init_compute(int algo, uint32_t nw, 
 // broadcast: compute -> memory (bitvector = frontier words of the pair)
 // ------------------------------------------------------------------------
 
+// With fmin (a CC / SSSP pull), also the smallest value sent.
 template <typename V>
 __global__ void __launch_bounds__(kBlock) k_bcast_send(uint32_t nw, const uint64_t *__restrict__ F,
                                                        const V *__restrict__ prop, const uint32_t *__restrict__ wrank,
-                                                       int dense, uint64_t *__restrict__ bv, V *__restrict__ val)
+                                                       int dense, uint64_t *__restrict__ bv, V *__restrict__ val,
+                                                       unsigned int *fmin)
 {
+  static_assert(sizeof(V) == 4, "4-byte values on the wire");
   const uint32_t base = group_base();
   const int l = lane_id();
   const bool ok = l < (int)kGroupWords && base + l < nw;
   const uint64_t mine = ok ? F[base + l] : 0ull;
   if (ok) bv[base + l] = mine;
   uint64_t todo = dense ? __ballot(ok) : __ballot(mine != 0);
+  uint32_t lo = 0xFFFFFFFFu;
   while (todo) {
     const int j = __ffsll((unsigned long long)todo) - 1;
     todo &= todo - 1;
     const uint64_t word = shfl_u64(mine, j);
     const uint32_t w = base + j, s = w * 64 + l;
-    if (dense) val[s] = prop[s];
-    else if ((word >> l) & 1ull) val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())] = prop[s];
+    const bool on = (word >> l) & 1ull;
+    if (dense || on) {
+      const V x = prop[s];
+      if (dense) val[s] = x;
+      else val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())] = x;
+      const uint32_t u = __builtin_bit_cast(uint32_t, x);
+      if (on && u < lo) lo = u;
+    }
+  }
+  if (fmin) {
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+      const uint32_t t = __shfl_xor(lo, o, kWave);
+      lo = t < lo ? t : lo;
+    }
+    if (l == 0 && lo != 0xFFFFFFFFu) atomicMin(fmin, lo);
+  }
+}
+
+// Pull of CC / SSSP, compute side: bit r of Rdone is set when memory proxy r
+// belongs to a vertex whose value is at or below the smallest value broadcast
+// this superstep.  Every candidate of the superstep is Op::edge of a
+// broadcast value, never below it (the weights are unsigned and the edge
+// operators saturate), so no candidate can improve such a row and the pull may
+// leave its arcs out.  A wave covers one word of rows.
+__global__ void __launch_bounds__(kBlock) k_rows_done(uint32_t nwP, const uint32_t *__restrict__ proxy_slot,
+                                                      const uint32_t *__restrict__ prop, const Counters *cnt,
+                                                      uint64_t *__restrict__ Rdone)
+{
+  const uint32_t fmin = cnt->fmin;
+  const uint64_t nrow = (uint64_t)nwP * 64;
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < nrow; r += (uint64_t)gridDim.x * kBlock) {
+    const uint32_t sl = proxy_slot[r];
+    const bool done = sl != kNone && prop[sl] <= fmin;
+    const uint64_t m = __ballot(done);
+    if (lane_id() == 0) Rdone[r >> 6] = m;
   }
 }
 
@@ -583,6 +620,9 @@ __global__ void k_zero_counters(Counters *cnt)
     cnt->edges = 0;
     cnt->frontier_maxdeg = 0;
     cnt->pend_min = 0xFFFFFFFFu;
+    cnt->fmin = 0xFFFFFFFFu;
+    cnt->pull_skipped = 0;
+    cnt->pull_skipped_rows = 0;
     cnt->frontier_uncovered = 0;
     cnt->ship_error = 0;
   }
@@ -641,6 +681,7 @@ grd_status state_alloc(grd_state *s)
   GRD_TRY(m.alloc(&s->mprop, P * V));
   GRD_TRY(m.alloc(&s->mupd, P * V));
   GRD_TRY(m.alloc_n(&s->Du, nwP));
+  if (s->algo == GRD_CC || s->algo == GRD_SSSP) GRD_TRY(m.alloc_n(&s->Rdone, std::max<uint32_t>(nwP, 1)));
   GRD_TRY(m.alloc_n(&s->bvB, nwS));
   GRD_TRY(m.alloc_n(&s->bvR, nwP));
   GRD_TRY(m.alloc(&s->valB, S * V));
@@ -681,6 +722,11 @@ grd_status state_alloc(grd_state *s)
       GRD_TRY(m.alloc_n(&s->tpart, std::max<uint64_t>(g->n_pairs, 1)));
       GRD_TRY(m.alloc_n(&s->tseg, std::max<uint32_t>(g->n_tblk, 1)));
     }
+  }
+  if (s->Rdone) {
+    // one flag per row block of the pull this state runs
+    const uint32_t nblk = bsp_hot_cold(s) ? g->n_cblk : g->n_pblk;
+    GRD_TRY(m.alloc_n(&s->bskip, std::max<uint32_t>(nblk, 1)));
   }
   GRD_TRY(m.alloc_n(&s->cnt, 1));
   GRD_HIP(hipHostMalloc((void **)&s->hcnt, sizeof(Counters), hipHostMallocDefault));
@@ -796,16 +842,25 @@ grd_status op_broadcast(grd_state *s, int option)
   s->xr_dense = -1;
   if (need_rank) GRD_TRY(word_ranks(st, s->F, nwS, s->blk, s->wrankB, s->totals + 0));
   const uint32_t grid = group_grid(nwS);
+  // a pull of CC / SSSP also sends the rows that cannot improve (the pull
+  // leaves their arcs out unless the caller asked for every row)
+  const bool rows_done = pull && s->algo != GRD_PR && !s->prm.pull_every_row;
   hipLaunchKernelGGL(k_bcast_send<uint32_t>, dim3(grid), dim3(kBlock), 0, st, nwS, s->F, (const uint32_t *)s->prop,
-                     s->wrankB, (int)dense, s->bvB, (uint32_t *)s->valB);
+                     s->wrankB, (int)dense, s->bvB, (uint32_t *)s->valB, rows_done ? &s->cnt->fmin : nullptr);
   const uint32_t ident = (s->algo == GRD_PR) ? 0u : 0xFFFFFFFFu;  // 0.0f or the min identity
   hipLaunchKernelGGL(k_bcast_recv<uint32_t>, dim3(grid), dim3(kBlock), 0, st, nwS, s->bvB, (const uint32_t *)s->valB,
                      s->wrankB, g->word_dst, (int)dense, (int)!pull, ident, (uint32_t *)s->mprop, s->AL);
+  const uint32_t nwP = g->P / 64;
+  if (rows_done && nwP)
+    hipLaunchKernelGGL(k_rows_done, dim3(loop_grid(g->ctx, nwP)), dim3(kBlock), 0, st, nwP, g->proxy_slot,
+                       (const uint32_t *)s->prop, s->cnt, s->Rdone);
   GRD_HIP(hipGetLastError());
+  s->rdone_sent = rows_done;
   // exchange bytes (GraphAlgorithm.cpp:251-265): bitvector of every pair +
-  // packed values (sparse) or the pair's full own slice (dense)
+  // packed values (sparse) or the pair's full own slice (dense), and the
+  // rows-done bitmap of a CC / SSSP pull
   uint64_t vals = dense ? (uint64_t)g->n : s->last_frontier;
-  s->bytes += bitvector_bytes(g) + vals * s->vbytes;
+  s->bytes += bitvector_bytes(g) + vals * s->vbytes + (rows_done ? 8ull * nwP : 0ull);
   return GRD_OK;
 }
 
@@ -826,12 +881,15 @@ static grd_status traverse_t(grd_state *s, int option)
       GRD_HIP(hipEventRecord(s->kev[1], st));
       s->kern_pending = 1;
       // a column and a gathered value per arc, the weight as stored (one
-      // byte in the hot / cold split when every weight fits), a value per row
+      // byte in the hot / cold split when every weight fits), a value per
+      // row, and the rows-done bitmap; the arcs and rows of the blocks left
+      // out come off when the counters are read
       const uint64_t wb = !cwt ? 0 : (bsp_hot_cold(s) && g->cow8) ? 1 : 4;
-      s->kern_pending_bytes = (8 + wb) * g->m + 8ull * g->P;
+      s->kern_pending_bytes = (8 + wb) * g->m + 8ull * g->P + (s->rdone_sent ? g->P / 8 : 0);
+      s->skip_arc_bytes = 8 + wb;
     }
     GRD_TRY(bsp_pull_fixup<Op>(s));
-    s->edges += g->m;  // a pull examines every arc of the memory partitions
+    s->edges += g->m;  // every arc of the memory partitions, less those left out (counters)
   } else {
     uint32_t want = (uint32_t)std::min<uint64_t>((s->last_frontier + kBlock - 1) / kBlock, (uint64_t)ctx->num_cus * 8);
     uint32_t grid = std::max<uint32_t>(1, want);

@@ -84,7 +84,10 @@ struct Counters {
   unsigned long long heavy_edges;     // push: edges of heavy vertices (k_push_heavy)
   unsigned long long frontier_uncovered;  // frontier vertices with an out-neighbour of another master
   unsigned long long ship_error;      // edge shipping: the active list disagreed with the sized wire
-  unsigned long long pad;
+  unsigned int fmin;                  // CC / SSSP pull: smallest value broadcast this superstep
+  unsigned int pad32;
+  unsigned long long pull_skipped;    // CC / SSSP pull: arcs of rows that could not improve, not read
+  unsigned long long pull_skipped_rows;  // and the rows of the row blocks left out
   float sel[8];                       // learned selector: the next superstep's 4 logits and argmax
   unsigned long long fpart[64];       // frontier size of each compute partition (C > 1)
 };
@@ -166,6 +169,11 @@ struct grd_graph {
   // edge shipping (NO_OFFLOAD schedule)
   uint8_t *word_c = nullptr;     // [S/64] compute partition of each slot word
   uint8_t *word_nvalid = nullptr;  // [S/64] valid slots at the start of each slot word (PR pull; built on demand)
+  // [P/64] rows with at least one arc in the CSC (CC / SSSP pull: a row block
+  // is left out when its rows with arcs are done; built on first use).  The
+  // whole CSC counts for the cold blocks of the hot / cold split too: a row
+  // without cold arcs still takes its hot partial in the cold pass.
+  uint64_t *rarc = nullptr;
   uint64_t *uncov = nullptr;     // [S/64] slots with an out-neighbour of another master (not covered)
   uint32_t *scol = nullptr;      // [m] push CSR columns as compute slots (shipped records; built on first use)
   uint8_t *w8 = nullptr;         // [m] push CSR weights as bytes, when every weight is below 256 (shipping)
@@ -252,6 +260,13 @@ struct grd_state {
   // memory side [P]
   void *mprop = nullptr, *mupd = nullptr;
   uint64_t *Du = nullptr;  // [P/64] dirty proxies
+  // [P/64] CC / SSSP pull: rows whose vertex holds a value at or below the
+  // superstep's smallest broadcast value (sent with a pull broadcast)
+  uint64_t *Rdone = nullptr;
+  bool rdone_sent = false;   // the last broadcast sent Rdone (a pull of CC / SSSP)
+  bool skip_pending = false; // the last pull's skipped arcs are still to be taken from the counters
+  uint64_t skip_arc_bytes = 0;  // algorithmic bytes per arc of that pull
+  uint8_t *bskip = nullptr;     // CC / SSSP pull: per row block, 1 = every row with arcs is done
   // wire
   uint64_t *bvB = nullptr, *bvR = nullptr;
   void *valB = nullptr, *valR = nullptr;

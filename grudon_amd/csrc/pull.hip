@@ -185,6 +185,55 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_fixup(const PullLong *__res
 // ones with their value, inactive ones with the combine identity (Op::edge
 // keeps the identity), so one gather per arc gives the arc's contribution.
 
+// Rows of a CC / SSSP pull that cannot improve (the broadcast's rows-done
+// bitmap, k_rows_done): a row block whose rows with arcs are all done is left
+// out whole.  This pass flags those blocks and counts their arcs and rows
+// (one atomic per workgroup), so that the pull reads one flag per block.
+__global__ void __launch_bounds__(kBlock) k_blocks_done(uint32_t nblk, const PullBlock *__restrict__ blocks,
+                                                        const uint64_t *__restrict__ done,
+                                                        const uint64_t *__restrict__ hasarc,
+                                                        uint8_t *__restrict__ bskip, Counters *cnt)
+{
+  unsigned long long arcs = 0, rows = 0;
+  for (uint32_t b = blockIdx.x * kBlock + threadIdx.x; b < nblk; b += gridDim.x * kBlock) {
+    const PullBlock d = blocks[b];
+    bool all;
+    if (d.nrows == 0) {
+      all = (done[d.r0 >> 6] >> (d.r0 & 63)) & 1ull;
+    } else {
+      all = true;
+      const uint32_t r1 = d.r0 + d.nrows;
+      for (uint32_t w = d.r0 >> 6; all && w <= (r1 - 1) >> 6; w++) {
+        const uint32_t lo = max(d.r0, w * 64) - w * 64, hi = min(r1 - w * 64, 64u);
+        const uint64_t mask = (hi - lo == 64) ? ~0ull : (((1ull << (hi - lo)) - 1ull) << lo);
+        all = ((done[w] | ~hasarc[w]) & mask) == mask;
+      }
+    }
+    bskip[b] = all ? 1 : 0;
+    if (all) {
+      arcs += d.e1 - d.e0;
+      rows += d.nrows;
+    }
+  }
+  arcs = block_sum(arcs);
+  rows = block_sum(rows);
+  if (threadIdx.x == 0) {
+    if (arcs) atomicAdd(&cnt->pull_skipped, arcs);
+    if (rows) atomicAdd(&cnt->pull_skipped_rows, rows);
+  }
+}
+
+// bit r = row r has an arc (rp[r + 1] > rp[r]); a wave covers one word
+__global__ void __launch_bounds__(kBlock) k_rows_with_arcs(uint32_t nwP, const uint32_t *__restrict__ rp,
+                                                           uint64_t *__restrict__ bits)
+{
+  const uint64_t nrow = (uint64_t)nwP * 64;
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < nrow; r += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t m = __ballot(rp[r + 1] > rp[r]);
+    if (lane_id() == 0) bits[r >> 6] = m;
+  }
+}
+
 // apply the pulled value to the proxy's running update; true = dirty
 template <class Op>
 __device__ __forceinline__ bool pull_apply(typename Op::V *upd, uint32_t r, typename Op::V acc)
@@ -215,7 +264,8 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
                                                         const typename Op::V *__restrict__ mprop, uint32_t P,
                                                         typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty,
                                                         typename Op::V *__restrict__ partial,
-                                                        const typename Op::V *__restrict__ pre)
+                                                        const typename Op::V *__restrict__ pre,
+                                                        const uint8_t *__restrict__ bskip)
 {
   using V = typename Op::V;
   __shared__ V s_v[kPullNnz];
@@ -223,6 +273,7 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
   __shared__ unsigned long long s_dirty[kDirtyWords];
   const uint32_t bid = blockIdx.x;
   const PullBlock d = blocks[bid];
+  if (bskip && bskip[bid]) return;  // no row with arcs can improve (k_blocks_done)
   if (d.nrows == 0) {  // segment (<= kPullNnz arcs) of a long row
     // column ids (and weights), then the gathers (hot sources from LDS),
     // then each thread combines its arcs in order
@@ -310,12 +361,15 @@ template <class Op>
 __global__ void __launch_bounds__(kBlock) k_bsp_pull_fixup(const PullLong *__restrict__ longs, uint32_t nlong,
                                                            const typename Op::V *__restrict__ partial,
                                                            typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty,
-                                                           const typename Op::V *__restrict__ pre)
+                                                           const typename Op::V *__restrict__ pre,
+                                                           const uint64_t *__restrict__ done)
 {
-  // a wave per long row, as k_prp_pull_fixup
+  // a wave per long row, as k_prp_pull_fixup; a done row's segments were
+  // left out (k_blocks_done flags every segment of it)
   const uint32_t lane = lane_id();
   for (uint32_t i = blockIdx.x * kWavesPerBlock + wave_in_block(); i < nlong; i += gridDim.x * kWavesPerBlock) {
     const PullLong L = longs[i];
+    if (done && ((done[L.row >> 6] >> (L.row & 63)) & 1ull)) continue;
     typename Op::V acc = op_identity<Op>();
     for (uint32_t j = lane; j < L.nseg; j += kWave) acc = op_combine<Op>(acc, partial[L.first + j]);
     for (int o = kWave / 2; o > 0; o >>= 1) acc = op_combine<Op>(acc, __shfl_xor(acc, o, kWave));
@@ -369,25 +423,58 @@ grd_status prp_pull_rows_fixup(hipStream_t st, const PullLong *longs, uint32_t n
 // pass first and then the row-block kernel over the cold arcs, which combines
 // each row's hot partial with its own; otherwise the row-block kernel over
 // the whole CSC.
+// CC / SSSP after a broadcast that sent the rows-done bitmap: flag the row
+// blocks (of the cold CSC, or of the whole CSC) that the pull leaves out
+template <class Op>
+static grd_status blocks_done(grd_state *s, bool cold, const uint8_t **flags)
+{
+  *flags = nullptr;
+  if (!Op::kMin || !s->rdone_sent) return GRD_OK;
+  grd_graph *g = s->g;
+  hipStream_t st = g->ctx->stream;
+  const uint32_t nwP = g->P / 64;
+  // rows with arcs in the whole CSC: in the hot / cold split a row without
+  // cold arcs still combines its hot partial in the cold pass
+  if (!g->rarc) {
+    GRD_TRY(g->mem.alloc_n(&g->rarc, std::max<uint32_t>(nwP, 1)));
+    if (nwP)
+      hipLaunchKernelGGL(k_rows_with_arcs, dim3(std::min<uint32_t>((nwP + kWavesPerBlock - 1) / kWavesPerBlock,
+                                                                   (uint32_t)g->ctx->num_cus * 8)),
+                         dim3(kBlock), 0, st, nwP, g->crp, g->rarc);
+  }
+  const uint32_t nblk = cold ? g->n_cblk : g->n_pblk;
+  if (nblk)
+    hipLaunchKernelGGL(k_blocks_done, dim3(std::min<uint32_t>((nblk + kBlock - 1) / kBlock, (uint32_t)g->ctx->num_cus * 8)),
+                       dim3(kBlock), 0, st, nblk, cold ? g->cblk : g->pblk, (const uint64_t *)s->Rdone,
+                       (const uint64_t *)g->rarc, s->bskip, s->cnt);
+  GRD_HIP(hipGetLastError());
+  s->skip_pending = true;
+  *flags = s->bskip;
+  return GRD_OK;
+}
+
 template <class Op>
 grd_status bsp_pull_launch(grd_state *s, const uint32_t *cw)
 {
   grd_graph *g = s->g;
   using V = typename Op::V;
+  const uint8_t *skip = nullptr;
   if (bsp_hot_cold(s)) {
+    GRD_TRY(blocks_done<Op>(s, true, &skip));
     GRD_TRY(bsp_pull_hot<Op>(s));
     if (g->n_cblk && cw && g->cow8)
       hipLaunchKernelGGL((k_bsp_pull_rb<Op, uint8_t>), dim3(g->n_cblk), dim3(kBlock), 0, g->ctx->stream, g->cblk,
                          g->cblk_hot, g->corp, g->cocol, (const uint8_t *)g->cow8, (const V *)s->mprop, g->P,
-                         (V *)s->mupd, s->Du, (V *)s->cseg, (const V *)s->tpart);
+                         (V *)s->mupd, s->Du, (V *)s->cseg, (const V *)s->tpart, skip);
     else if (g->n_cblk)
       hipLaunchKernelGGL((k_bsp_pull_rb<Op, uint32_t>), dim3(g->n_cblk), dim3(kBlock), 0, g->ctx->stream, g->cblk,
                          g->cblk_hot, g->corp, g->cocol, cw ? g->cow : nullptr, (const V *)s->mprop, g->P,
-                         (V *)s->mupd, s->Du, (V *)s->cseg, (const V *)s->tpart);
+                         (V *)s->mupd, s->Du, (V *)s->cseg, (const V *)s->tpart, skip);
   } else {
+    GRD_TRY(blocks_done<Op>(s, false, &skip));
     hipLaunchKernelGGL((k_bsp_pull_rb<Op, uint32_t>), dim3(g->n_pblk), dim3(kBlock), 0, g->ctx->stream, g->pblk,
                        g->pblk_hot, g->crp, g->ccol, cw, (const V *)s->mprop, g->P, (V *)s->mupd, s->Du,
-                       (V *)s->partial, (const V *)nullptr);
+                       (V *)s->partial, (const V *)nullptr, skip);
   }
   GRD_HIP(hipGetLastError());
   return GRD_OK;
@@ -403,7 +490,8 @@ grd_status bsp_pull_fixup(grd_state *s)
   if (!nlong) return GRD_OK;
   hipLaunchKernelGGL(k_bsp_pull_fixup<Op>, dim3(fixup_grid(nlong)), dim3(kBlock), 0, g->ctx->stream,
                      hc ? g->clong : g->plong, nlong, hc ? (const V *)s->cseg : (const V *)s->partial, (V *)s->mupd,
-                     s->Du, hc ? (const V *)s->tpart : (const V *)nullptr);
+                     s->Du, hc ? (const V *)s->tpart : (const V *)nullptr,
+                     (Op::kMin && s->rdone_sent) ? (const uint64_t *)s->Rdone : nullptr);
   GRD_HIP(hipGetLastError());
   return GRD_OK;
 }

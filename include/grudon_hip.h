@@ -89,6 +89,12 @@ typedef struct {
   uint32_t offload_only; /* heuristic / learned: never turn a push superstep into
                             edge shipping (by default they ship whenever the
                             frontier is covered, the cheaper push on one GPU) */
+  uint32_t pull_every_row; /* CC / SSSP pull supersteps: gather every row's arcs.
+                              By default a row whose vertex already holds a value
+                              at or below the smallest value broadcast this
+                              superstep is not pulled (no candidate can improve
+                              it); results and iteration counts are the same
+                              either way */
 } grd_params;
 
 /* One record per iteration (grd_run trace).  feat holds the selector's 16

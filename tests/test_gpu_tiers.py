@@ -190,6 +190,58 @@ def test_hot_cold_weight_storage(ctx, bsp_graphs, scale_w, add_w, option):
         ctx.set_pull_tiling(3, 0, 0)
 
 
+@pytest.mark.parametrize("mode,tb", [(0, 0), (3, 6), (3, 9)])
+@pytest.mark.parametrize("C,M", [(1, 4), (2, 3)])
+@pytest.mark.parametrize("algo", ["cc", "hop", "weighted", "delta"])
+def test_pull_leaves_out_done_rows(ctx, bsp_graphs, mode, tb, C, M, algo):
+    """A CC / SSSP pull leaves out the row blocks whose rows with arcs cannot
+    improve (the vertex's value is at or below the smallest value broadcast):
+    every superstep a pull, the labels / distances and iteration counts equal
+    the oracle's and a run that pulls every row, with no more arcs examined and
+    kernel bytes counted (fewer for CC); the heuristic schedule (pushes, pulls
+    and shipping mixed) gives the same results either way."""
+    _say("rows done", mode, tb, C, M, algo)
+    gs, gd, gw = bsp_graphs
+    g = {"cc": gs, "hop": gd, "weighted": gw, "delta": gw}[algo]
+    kw = {"cc": {}, "hop": {"source": 0}, "weighted": {"source": 0, "weighted": True},
+          "delta": {"source": 0, "weighted": True, "delta": 32}}[algo]
+    P = O.Partition(g, C, M)
+    if algo == "cc":
+        ref, st = P.cc()
+    else:
+        ref, st = P.sssp(0, weighted=kw.get("weighted", False), delta=kw.get("delta", 0))
+    ctx.set_pull_tiling(mode, tb, 0)
+    try:
+        G = api.Graph.from_csr(ctx, g.rp, g.col, g.w if algo in ("weighted", "delta") else None, C=C, M=M)
+        algo_id = api.CC if algo == "cc" else api.SSSP
+        runs = {}
+        for every in (True, False):
+            for sched in (api.SCHED_FIXED, api.SCHED_HEURISTIC):
+                S = api.State(G, algo_id)
+                S.init(max_iter=100000, sched=sched, fixed_option=api.OPT_PULL_DENSE, time_kernel=True,
+                       pull_every_row=every, **kw)
+                stats = S.run()
+                assert np.array_equal(S.result(), ref), (every, sched)
+                assert stats["iterations"] == int(st[0]), (every, sched)
+                runs[every, sched] = stats
+                S.free()
+        full, part = runs[True, api.SCHED_FIXED], runs[False, api.SCHED_FIXED]
+        assert full["edges_examined"] == full["iterations"] * G.m
+        # CC labels reach the frontier's smallest label within a few supersteps,
+        # so whole blocks are left out; how many SSSP rows can be left out
+        # depends on the distances (none, on some of these small graphs)
+        if algo == "cc":
+            assert part["edges_examined"] < full["edges_examined"]
+            assert part["kernel_bytes"] < full["kernel_bytes"]
+        assert part["edges_examined"] <= full["edges_examined"]
+        # the pull that may leave rows out also reads the rows-done bitmap
+        assert part["kernel_bytes"] <= full["kernel_bytes"] + part["iterations"] * G.proxies // 8
+        _say("   arcs examined", part["edges_examined"], "of", full["edges_examined"])
+        G.free()
+    finally:
+        ctx.set_pull_tiling(3, 0, 0)
+
+
 def test_bsp_pull_on_graph_tiled_by_another_mode(ctx, bsp_graphs):
     """A graph regrouped by the LDS-tiled pull first keeps that regrouping;
     a CC state created on it later pulls over the plain CSC, and is exact."""
