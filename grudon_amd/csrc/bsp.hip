@@ -197,7 +197,9 @@ __global__ void __launch_bounds__(kBlock) k_init_compute(int algo, uint32_t nw, 
 // broadcast: compute -> memory (bitvector = frontier words of the pair)
 // ------------------------------------------------------------------------
 
-// With fmin (a CC / SSSP pull), also the smallest value sent.
+// With fmin (a CC / SSSP pull), also the smallest value sent: the grid then
+// loops over the groups (loop_grid), so that one atomic per workgroup
+// carries it.
 template <typename V>
 __global__ void __launch_bounds__(kBlock) k_bcast_send(uint32_t nw, const uint64_t *__restrict__ F,
                                                        const V *__restrict__ prop, const uint32_t *__restrict__ wrank,
@@ -205,47 +207,46 @@ __global__ void __launch_bounds__(kBlock) k_bcast_send(uint32_t nw, const uint64
                                                        unsigned int *fmin)
 {
   static_assert(sizeof(V) == 4, "4-byte values on the wire");
-  const uint32_t base = group_base();
   const int l = lane_id();
-  const bool ok = l < (int)kGroupWords && base + l < nw;
-  const uint64_t mine = ok ? F[base + l] : 0ull;
-  if (ok) bv[base + l] = mine;
-  uint64_t todo = dense ? __ballot(ok) : __ballot(mine != 0);
   uint32_t lo = 0xFFFFFFFFu;
-  while (todo) {
-    const int j = __ffsll((unsigned long long)todo) - 1;
-    todo &= todo - 1;
-    const uint64_t word = shfl_u64(mine, j);
-    const uint32_t w = base + j, s = w * 64 + l;
-    const bool on = (word >> l) & 1ull;
-    if (dense || on) {
-      const V x = prop[s];
-      if (dense) val[s] = x;
-      else val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())] = x;
-      const uint32_t u = __builtin_bit_cast(uint32_t, x);
-      if (on && u < lo) lo = u;
+  for (uint32_t base = group_base(); base < nw; base += gridDim.x * kWavesPerBlock * kGroupWords) {
+    const bool ok = l < (int)kGroupWords && base + l < nw;
+    const uint64_t mine = ok ? F[base + l] : 0ull;
+    if (ok) bv[base + l] = mine;
+    uint64_t todo = dense ? __ballot(ok) : __ballot(mine != 0);
+    while (todo) {
+      const int j = __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1;
+      const uint64_t word = shfl_u64(mine, j);
+      const uint32_t w = base + j, s = w * 64 + l;
+      const bool on = (word >> l) & 1ull;
+      if (dense || on) {
+        const V x = prop[s];
+        if (dense) val[s] = x;
+        else val[wrank[w] + (uint32_t)__popcll(word & lanemask_lt())] = x;
+        const uint32_t u = __builtin_bit_cast(uint32_t, x);
+        if (on && u < lo) lo = u;
+      }
     }
   }
-  if (fmin) {
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-      const uint32_t t = __shfl_xor(lo, o, kWave);
-      lo = t < lo ? t : lo;
-    }
-    if (l == 0 && lo != 0xFFFFFFFFu) atomicMin(fmin, lo);
+  if (fmin) {  // uniform: a kernel argument
+    lo = block_min(lo);
+    if (threadIdx.x == 0 && lo != 0xFFFFFFFFu) atomicMin(fmin, lo);
   }
 }
 
 // Pull of CC / SSSP, compute side: bit r of Rdone is set when memory proxy r
-// belongs to a vertex whose value is at or below the smallest value broadcast
-// this superstep.  Every candidate of the superstep is Op::edge of a
-// broadcast value, never below it (the weights are unsigned and the edge
-// operators saturate), so no candidate can improve such a row and the pull may
+// belongs to a vertex whose value is at or below b = the smallest value
+// broadcast this superstep plus the smallest arc increment (0 for CC, 1 for
+// hop counts, the smallest arc weight for weighted SSSP; saturating).  Every
+// candidate of the superstep is Op::edge of a broadcast value over an arc,
+// never below b, so no candidate can improve such a row and the pull may
 // leave its arcs out.  A wave covers one word of rows.
 __global__ void __launch_bounds__(kBlock) k_rows_done(uint32_t nwP, const uint32_t *__restrict__ proxy_slot,
                                                       const uint32_t *__restrict__ prop, const Counters *cnt,
-                                                      uint64_t *__restrict__ Rdone)
+                                                      uint32_t inc, uint64_t *__restrict__ Rdone)
 {
-  const uint32_t fmin = cnt->fmin;
+  const uint32_t f = cnt->fmin, fmin = f + inc < f ? 0xFFFFFFFFu : f + inc;
   const uint64_t nrow = (uint64_t)nwP * 64;
   for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < nrow; r += (uint64_t)gridDim.x * kBlock) {
     const uint32_t sl = proxy_slot[r];
@@ -727,6 +728,7 @@ grd_status state_alloc(grd_state *s)
     // one flag per row block of the pull this state runs
     const uint32_t nblk = bsp_hot_cold(s) ? g->n_cblk : g->n_pblk;
     GRD_TRY(m.alloc_n(&s->bskip, std::max<uint32_t>(nblk, 1)));
+    if (bsp_hot_cold(s)) GRD_TRY(m.alloc_n(&s->hskip, std::max<uint32_t>(g->n_tblk, 1)));
   }
   GRD_TRY(m.alloc_n(&s->cnt, 1));
   GRD_HIP(hipHostMalloc((void **)&s->hcnt, sizeof(Counters), hipHostMallocDefault));
@@ -831,6 +833,37 @@ static uint64_t bitvector_bytes(const grd_graph *g)
   return b;
 }
 
+__global__ void __launch_bounds__(kBlock) k_min_u32(uint64_t n, const uint32_t *__restrict__ x, uint32_t *out)
+{
+  uint32_t lo = 0xFFFFFFFFu;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+    lo = x[i] < lo ? x[i] : lo;
+  lo = block_min(lo);
+  if (threadIdx.x == 0 && lo != 0xFFFFFFFFu) atomicMin(out, lo);
+}
+
+// the smallest arc weight of a weighted graph (once per graph; 2^32 - 1
+// when it has no arcs)
+static grd_status graph_min_weight(grd_graph *g, uint32_t *out)
+{
+  if (!g->wmin_known) {
+    hipStream_t st = g->ctx->stream;
+    DevMem tmp;
+    uint32_t *d = nullptr;
+    GRD_TRY(tmp.alloc_n(&d, 1));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, st, d, 1, 0xFFFFFFFFu);
+    if (g->m && g->cw)
+      hipLaunchKernelGGL(k_min_u32, dim3(std::min<uint64_t>((g->m + kBlock - 1) / kBlock, (uint64_t)g->ctx->num_cus * 8)),
+                         dim3(kBlock), 0, st, g->m, g->cw, d);
+    GRD_HIP(hipGetLastError());
+    GRD_HIP(hipMemcpyAsync(&g->wmin, d, 4, hipMemcpyDeviceToHost, st));
+    GRD_HIP(hipStreamSynchronize(st));
+    g->wmin_known = true;
+  }
+  *out = g->wmin;
+  return GRD_OK;
+}
+
 grd_status op_broadcast(grd_state *s, int option)
 {
   grd_graph *g = s->g;
@@ -845,15 +878,22 @@ grd_status op_broadcast(grd_state *s, int option)
   // a pull of CC / SSSP also sends the rows that cannot improve (the pull
   // leaves their arcs out unless the caller asked for every row)
   const bool rows_done = pull && s->algo != GRD_PR && !s->prm.pull_every_row;
-  hipLaunchKernelGGL(k_bcast_send<uint32_t>, dim3(grid), dim3(kBlock), 0, st, nwS, s->F, (const uint32_t *)s->prop,
-                     s->wrankB, (int)dense, s->bvB, (uint32_t *)s->valB, rows_done ? &s->cnt->fmin : nullptr);
+  hipLaunchKernelGGL(k_bcast_send<uint32_t>, dim3(rows_done ? loop_grid(g->ctx, nwS) : grid), dim3(kBlock), 0, st,
+                     nwS, s->F, (const uint32_t *)s->prop, s->wrankB, (int)dense, s->bvB, (uint32_t *)s->valB,
+                     rows_done ? &s->cnt->fmin : nullptr);
   const uint32_t ident = (s->algo == GRD_PR) ? 0u : 0xFFFFFFFFu;  // 0.0f or the min identity
   hipLaunchKernelGGL(k_bcast_recv<uint32_t>, dim3(grid), dim3(kBlock), 0, st, nwS, s->bvB, (const uint32_t *)s->valB,
                      s->wrankB, g->word_dst, (int)dense, (int)!pull, ident, (uint32_t *)s->mprop, s->AL);
   const uint32_t nwP = g->P / 64;
-  if (rows_done && nwP)
+  if (rows_done && nwP) {
+    uint32_t inc = 0;  // the smallest Op::edge increment over an arc
+    if (s->algo == GRD_SSSP) {
+      if (!s->prm.weighted) inc = 1;
+      else GRD_TRY(graph_min_weight(g, &inc));
+    }
     hipLaunchKernelGGL(k_rows_done, dim3(loop_grid(g->ctx, nwP)), dim3(kBlock), 0, st, nwP, g->proxy_slot,
-                       (const uint32_t *)s->prop, s->cnt, s->Rdone);
+                       (const uint32_t *)s->prop, s->cnt, inc, s->Rdone);
+  }
   GRD_HIP(hipGetLastError());
   s->rdone_sent = rows_done;
   // exchange bytes (GraphAlgorithm.cpp:251-265): bitvector of every pair +

@@ -174,6 +174,8 @@ struct grd_graph {
   // whole CSC counts for the cold blocks of the hot / cold split too: a row
   // without cold arcs still takes its hot partial in the cold pass.
   uint64_t *rarc = nullptr;
+  uint32_t wmin = 0;         // smallest arc weight (weighted SSSP pull; found on first use)
+  bool wmin_known = false;
   uint64_t *uncov = nullptr;     // [S/64] slots with an out-neighbour of another master (not covered)
   uint32_t *scol = nullptr;      // [m] push CSR columns as compute slots (shipped records; built on first use)
   uint8_t *w8 = nullptr;         // [m] push CSR weights as bytes, when every weight is below 256 (shipping)
@@ -267,6 +269,8 @@ struct grd_state {
   bool skip_pending = false; // the last pull's skipped arcs are still to be taken from the counters
   uint64_t skip_arc_bytes = 0;  // algorithmic bytes per arc of that pull
   uint8_t *bskip = nullptr;     // CC / SSSP pull: per row block, 1 = every row with arcs is done
+  uint8_t *hskip = nullptr;     // the same per wave block of the hot pass (hot / cold split)
+  bool hskip_on = false;        // hskip holds this pull's flags
   // wire
   uint64_t *bvB = nullptr, *bvR = nullptr;
   void *valB = nullptr, *valR = nullptr;

@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_fixup(const PullLong *__res
 __global__ void __launch_bounds__(kBlock) k_blocks_done(uint32_t nblk, const PullBlock *__restrict__ blocks,
                                                         const uint64_t *__restrict__ done,
                                                         const uint64_t *__restrict__ hasarc,
-                                                        uint8_t *__restrict__ bskip, Counters *cnt)
+                                                        uint8_t *__restrict__ bskip, Counters *cnt, int count_rows)
 {
   unsigned long long arcs = 0, rows = 0;
   for (uint32_t b = blockIdx.x * kBlock + threadIdx.x; b < nblk; b += gridDim.x * kBlock) {
@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(kBlock) k_blocks_done(uint32_t nblk, const Pul
   rows = block_sum(rows);
   if (threadIdx.x == 0) {
     if (arcs) atomicAdd(&cnt->pull_skipped, arcs);
-    if (rows) atomicAdd(&cnt->pull_skipped_rows, rows);
+    if (rows && count_rows) atomicAdd(&cnt->pull_skipped_rows, rows);
   }
 }
 
@@ -265,7 +265,9 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
                                                         typename Op::V *__restrict__ upd, uint64_t *__restrict__ dirty,
                                                         typename Op::V *__restrict__ partial,
                                                         const typename Op::V *__restrict__ pre,
-                                                        const uint8_t *__restrict__ bskip)
+                                                        const uint8_t *__restrict__ bskip,
+                                                        const uint64_t *__restrict__ done,
+                                                        const uint64_t *__restrict__ hasarc)
 {
   using V = typename Op::V;
   __shared__ V s_v[kPullNnz];
@@ -342,13 +344,20 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
     uint32_t rl = k * ngroups + threadIdx.x / G;
     V acc = op_identity<Op>();
     uint32_t r = d.r0 + rl;
-    if (rl < d.nrows) {
+    // a done row, or one without arcs, has nothing to apply (its hot partial
+    // may not have been written)
+    bool live = rl < d.nrows;
+    if (done && live) {
+      const uint64_t bit = 1ull << (r & 63);
+      live = !(done[r >> 6] & bit) && (hasarc[r >> 6] & bit);
+    }
+    if (live) {
       uint32_t b = crp[r] - d.e0, e = crp[r + 1] - d.e0;
       for (uint32_t x = b + sub; x < e; x += G) acc = op_combine<Op>(acc, s_v[x]);
     }
     for (int o = G / 2; o > 0; o >>= 1) acc = op_combine<Op>(acc, __shfl_xor(acc, o, kWave));
-    if (pre && sub == 0 && rl < d.nrows) acc = op_combine<Op>(pre[r], acc);  // the row's hot partial first
-    if (sub == 0 && rl < d.nrows && pull_apply<Op>(upd, r, acc))
+    if (pre && sub == 0 && live) acc = op_combine<Op>(pre[r], acc);  // the row's hot partial first
+    if (sub == 0 && live && pull_apply<Op>(upd, r, acc))
       atomicOr(&s_dirty[(r >> 6) - w0], 1ull << (r & 63));
   }
   __syncthreads();
@@ -429,6 +438,7 @@ template <class Op>
 static grd_status blocks_done(grd_state *s, bool cold, const uint8_t **flags)
 {
   *flags = nullptr;
+  s->hskip_on = false;
   if (!Op::kMin || !s->rdone_sent) return GRD_OK;
   grd_graph *g = s->g;
   hipStream_t st = g->ctx->stream;
@@ -442,11 +452,19 @@ static grd_status blocks_done(grd_state *s, bool cold, const uint8_t **flags)
                                                                    (uint32_t)g->ctx->num_cus * 8)),
                          dim3(kBlock), 0, st, nwP, g->crp, g->rarc);
   }
+  const uint32_t cap = (uint32_t)g->ctx->num_cus * 8;
   const uint32_t nblk = cold ? g->n_cblk : g->n_pblk;
   if (nblk)
-    hipLaunchKernelGGL(k_blocks_done, dim3(std::min<uint32_t>((nblk + kBlock - 1) / kBlock, (uint32_t)g->ctx->num_cus * 8)),
-                       dim3(kBlock), 0, st, nblk, cold ? g->cblk : g->pblk, (const uint64_t *)s->Rdone,
-                       (const uint64_t *)g->rarc, s->bskip, s->cnt);
+    hipLaunchKernelGGL(k_blocks_done, dim3(std::min<uint32_t>((nblk + kBlock - 1) / kBlock, cap)), dim3(kBlock), 0, st,
+                       nblk, cold ? g->cblk : g->pblk, (const uint64_t *)s->Rdone, (const uint64_t *)g->rarc, s->bskip,
+                       s->cnt, 1);
+  // the hot pass's wave blocks (their rows are counted once, with the cold pass's)
+  if (cold && g->n_tblk && s->hskip) {
+    hipLaunchKernelGGL(k_blocks_done, dim3(std::min<uint32_t>((g->n_tblk + kBlock - 1) / kBlock, cap)), dim3(kBlock), 0,
+                       st, g->n_tblk, g->tblk, (const uint64_t *)s->Rdone, (const uint64_t *)g->rarc, s->hskip,
+                       s->cnt, 0);
+    s->hskip_on = true;
+  }
   GRD_HIP(hipGetLastError());
   s->skip_pending = true;
   *flags = s->bskip;
@@ -465,16 +483,19 @@ grd_status bsp_pull_launch(grd_state *s, const uint32_t *cw)
     if (g->n_cblk && cw && g->cow8)
       hipLaunchKernelGGL((k_bsp_pull_rb<Op, uint8_t>), dim3(g->n_cblk), dim3(kBlock), 0, g->ctx->stream, g->cblk,
                          g->cblk_hot, g->corp, g->cocol, (const uint8_t *)g->cow8, (const V *)s->mprop, g->P,
-                         (V *)s->mupd, s->Du, (V *)s->cseg, (const V *)s->tpart, skip);
+                         (V *)s->mupd, s->Du, (V *)s->cseg, (const V *)s->tpart, skip,
+                         skip ? (const uint64_t *)s->Rdone : nullptr, skip ? (const uint64_t *)g->rarc : nullptr);
     else if (g->n_cblk)
       hipLaunchKernelGGL((k_bsp_pull_rb<Op, uint32_t>), dim3(g->n_cblk), dim3(kBlock), 0, g->ctx->stream, g->cblk,
                          g->cblk_hot, g->corp, g->cocol, cw ? g->cow : nullptr, (const V *)s->mprop, g->P,
-                         (V *)s->mupd, s->Du, (V *)s->cseg, (const V *)s->tpart, skip);
+                         (V *)s->mupd, s->Du, (V *)s->cseg, (const V *)s->tpart, skip,
+                         skip ? (const uint64_t *)s->Rdone : nullptr, skip ? (const uint64_t *)g->rarc : nullptr);
   } else {
     GRD_TRY(blocks_done<Op>(s, false, &skip));
     hipLaunchKernelGGL((k_bsp_pull_rb<Op, uint32_t>), dim3(g->n_pblk), dim3(kBlock), 0, g->ctx->stream, g->pblk,
                        g->pblk_hot, g->crp, g->ccol, cw, (const V *)s->mprop, g->P, (V *)s->mupd, s->Du,
-                       (V *)s->partial, (const V *)nullptr, skip);
+                       (V *)s->partial, (const V *)nullptr, skip, skip ? (const uint64_t *)s->Rdone : nullptr,
+                       skip ? (const uint64_t *)g->rarc : nullptr);
   }
   GRD_HIP(hipGetLastError());
   return GRD_OK;

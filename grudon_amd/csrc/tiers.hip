@@ -621,7 +621,11 @@ struct HotBsp {
   __device__ static Part part(const Acc &a) { return a; }
 };
 
-template <class Pol, class W = uint32_t>
+// kSkip (CC / SSSP): skip[b] = 1 marks a wave block whose rows are all done
+// or without arcs (k_blocks_done); it is taken as an empty segment (no loads,
+// the identity written to its hseg slot) and its rows' partials are not
+// written, which the cold pass never reads for such rows.
+template <class Pol, class W = uint32_t, bool kSkip = false>
 __global__ void __launch_bounds__(kTierBlock) k_pull_hot(const TierTile *__restrict__ ranges,
                                                          const PullBlock *__restrict__ blocks,
                                                          const uint32_t *__restrict__ hrp,
@@ -629,7 +633,8 @@ __global__ void __launch_bounds__(kTierBlock) k_pull_hot(const TierTile *__restr
                                                          const W *__restrict__ hw,
                                                          const typename Pol::V *__restrict__ mprop,
                                                          typename Pol::Part *__restrict__ hpart,
-                                                         typename Pol::Part *__restrict__ hseg)
+                                                         typename Pol::Part *__restrict__ hseg,
+                                                         const uint8_t *__restrict__ skip)
 {
   using V = typename Pol::V;
   static_assert(sizeof(V) == 4, "the window holds 4-byte values");
@@ -707,20 +712,29 @@ __global__ void __launch_bounds__(kTierBlock) k_pull_hot(const TierTile *__restr
     return PullBlock{(uint32_t)__builtin_amdgcn_readfirstlane(q.x), (uint32_t)__builtin_amdgcn_readfirstlane(q.y),
                      (uint32_t)__builtin_amdgcn_readfirstlane(q.z), (uint32_t)__builtin_amdgcn_readfirstlane(q.w)};
   };
+  // the skip flags travel with the descriptors (a flag past the range reads 0)
+  const auto bflag = block_buffer(skip, kSkip ? bend : 0u);
+  auto load_flag = [&](uint32_t i) { return kSkip ? (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(bflag, i, 0, 0) : 0u; };
+  auto apply_flag = [](PullBlock x, uint32_t f) {
+    if (kSkip && __builtin_amdgcn_readfirstlane(f)) x = PullBlock{x.r0, 0, x.e0, x.e0};
+    return x;
+  };
   uint32_t b = T.b0 + w;
-  PullBlock d = to_block(load_desc(b));
+  PullBlock d = apply_flag(to_block(load_desc(b)), load_flag(b));
   uint32_t b1 = take();
   uint4 q1 = load_desc(b1);
+  uint32_t f1 = load_flag(b1);
   uint32_t c[kPer], wt[kPer], rb[kRows], re[kRows];
   issue(d, c, wt, rb, re);
   while (b < bend) {
     const uint32_t b2 = take();
     const uint4 q2 = load_desc(b2);
+    const uint32_t f2 = load_flag(b2);
     const uint32_t ne = d.e1 - d.e0;
     V v[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; k++) v[k] = Pol::value(s_win[c[k]], wt[k]);
-    const PullBlock dn = to_block(q1);
+    const PullBlock dn = apply_flag(to_block(q1), f1);
     uint32_t cn[kPer], wtn[kPer], rbn[kRows], ren[kRows];
     issue(dn, cn, wtn, rbn, ren);
     if (d.nrows == 0) {
@@ -759,6 +773,7 @@ __global__ void __launch_bounds__(kTierBlock) k_pull_hot(const TierTile *__restr
     d = dn;
     b1 = b2;
     q1 = q2;
+    f1 = f2;
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
       c[k] = cn[k];
@@ -1152,7 +1167,8 @@ grd_status prp_pull_tiled(grd_state *s)
     float *hpart = reinterpret_cast<float *>(s->tpart), *hseg = reinterpret_cast<float *>(s->tseg);
     if (g->n_ttile)
       hipLaunchKernelGGL(k_pull_hot<HotPR>, dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile, g->tblk, g->hrp,
-                         g->tcol, (const uint32_t *)nullptr, (const float *)s->mprop, hpart, hseg);
+                         g->tcol, (const uint32_t *)nullptr, (const float *)s->mprop, hpart, hseg,
+                         (const uint8_t *)nullptr);
     if (g->n_tlong)
       hipLaunchKernelGGL(k_hot_fixup<HotPR>, dim3(wave_grid((g->n_tlong + 7) / 8)), dim3(kBlock), 0, st, g->tlong, g->n_tlong,
                          (const float *)hseg, hpart);
@@ -1188,13 +1204,22 @@ grd_status bsp_pull_hot(grd_state *s)
   using V = typename Op::V;
   V *hpart = reinterpret_cast<V *>(s->tpart), *hseg = reinterpret_cast<V *>(s->tseg);
   if (g->n_ttile) {
-    if (std::is_same<Op, OpWeighted>::value && g->hw8)
+    const uint32_t *hw = std::is_same<Op, OpWeighted>::value ? g->hw : nullptr;
+    const bool w8 = std::is_same<Op, OpWeighted>::value && g->hw8;
+    if (s->hskip_on && w8)
+      hipLaunchKernelGGL((k_pull_hot<HotBsp<Op>, uint8_t, true>), dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile,
+                         g->tblk, g->hrp, g->tcol, (const uint8_t *)g->hw8, (const V *)s->mprop, hpart, hseg,
+                         (const uint8_t *)s->hskip);
+    else if (s->hskip_on)
+      hipLaunchKernelGGL((k_pull_hot<HotBsp<Op>, uint32_t, true>), dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile,
+                         g->tblk, g->hrp, g->tcol, hw, (const V *)s->mprop, hpart, hseg, (const uint8_t *)s->hskip);
+    else if (w8)
       hipLaunchKernelGGL((k_pull_hot<HotBsp<Op>, uint8_t>), dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile,
-                         g->tblk, g->hrp, g->tcol, (const uint8_t *)g->hw8, (const V *)s->mprop, hpart, hseg);
+                         g->tblk, g->hrp, g->tcol, (const uint8_t *)g->hw8, (const V *)s->mprop, hpart, hseg,
+                         (const uint8_t *)nullptr);
     else
       hipLaunchKernelGGL((k_pull_hot<HotBsp<Op>, uint32_t>), dim3(g->n_ttile), dim3(kTierBlock), 0, st, g->ttile,
-                         g->tblk, g->hrp, g->tcol, std::is_same<Op, OpWeighted>::value ? g->hw : nullptr,
-                         (const V *)s->mprop, hpart, hseg);
+                         g->tblk, g->hrp, g->tcol, hw, (const V *)s->mprop, hpart, hseg, (const uint8_t *)nullptr);
   }
   if (g->n_tlong) {
     const uint64_t waves = (g->n_tlong + kWave / kFixLanes - 1) / (kWave / kFixLanes);
