@@ -402,7 +402,8 @@ def run_cc(api, ctx, args):
     log("graph: n=%d m=%d partition+upload %.1fs" % (G.n, G.m, time.perf_counter() - t1))
     S = api.State(G, api.CC)
     secs, iters, (kms, kl, kb), last = timed_solves(S, args, {"sched": api.SCHED_HEURISTIC,
-                                                              "offload_only": args.offload_only})
+                                                              "offload_only": args.offload_only,
+                                                              "pull_every_row": args.pull_every_row})
     gteps = G.m * args.steps / secs / 1e9
     log("timed %d steps: %.3f s, %.1f GTEPS" % (args.steps, secs, gteps))
     # edges the traversals examined (a pull examines every arc) per second
@@ -439,7 +440,8 @@ def run_sssp(api, ctx, args):
     G = api.Graph.from_csr(ctx, rp, col, w, C=args.compute, M=args.parts)
     log("graph: n=%d m=%d partition+upload %.1fs" % (G.n, G.m, time.perf_counter() - t1))
     S = api.State(G, api.SSSP)
-    kw = {"weighted": True, "delta": args.delta, "sched": api.SCHED_HEURISTIC, "offload_only": args.offload_only}
+    kw = {"weighted": True, "delta": args.delta, "sched": api.SCHED_HEURISTIC, "offload_only": args.offload_only,
+          "pull_every_row": args.pull_every_row}
     secs, iters, (kms, kl, kb), last = timed_solves(S, args, kw, sources=sources)
     # harmonic mean over sources of arcs / t_s = 16 arcs / sum t_s
     gteps = G.m * len(sources) * args.steps / secs / 1e9
@@ -852,6 +854,9 @@ def main():
     ap.add_argument("--tier-bits", type=int, default=0,
                     help="tiled / segmented / hot-cold pull: 2^bits sources per tier or window (0 = 15 / 19 / 15)")
     ap.add_argument("--tile-arcs", type=int, default=0, help="tiled pull: arcs per tile (0 = 65536)")
+    ap.add_argument("--pull-every-row", action="store_true",
+                    help="cc / sssp workloads: pull every row (by default a pull leaves out the rows that "
+                         "cannot improve)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 / exactness check")
     ap.add_argument("--cpu-threads", type=int, default=0)
