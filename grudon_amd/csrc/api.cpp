@@ -137,11 +137,15 @@ static uint64_t bitvector_bytes(const grd_graph *g)
 static grd_status finish_kernel_timing(grd_state *s, uint64_t frontier_k)
 {
   if (s->skip_pending) {
-    // a CC / SSSP pull left out the blocks whose rows could not improve
-    const uint64_t arcs = s->hcnt->pull_skipped, rows = s->hcnt->pull_skipped_rows;
+    // a CC / SSSP pull left out the blocks whose rows could not improve (their
+    // arcs are not examined), and in partially done blocks read the columns
+    // but gathered no value for the done rows' arcs
+    const Counters &c = *s->hcnt;
+    const uint64_t arcs = c.pull_skipped + c.pull_skipped_hot, rows = c.pull_skipped_rows;
+    const uint64_t part = c.pull_done_arcs > c.pull_skipped ? c.pull_done_arcs - c.pull_skipped : 0;
     s->edges -= std::min<uint64_t>(arcs, s->edges);
     if (s->kern_pending == 1) {
-      const uint64_t off = s->skip_arc_bytes * arcs + 8 * rows;
+      const uint64_t off = s->skip_arc_bytes * arcs + 8 * rows + 4 * part;
       s->kern_pending_bytes -= std::min<uint64_t>(off, s->kern_pending_bytes);
     }
     s->skip_pending = false;

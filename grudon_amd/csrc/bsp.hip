@@ -623,7 +623,9 @@ __global__ void k_zero_counters(Counters *cnt)
     cnt->pend_min = 0xFFFFFFFFu;
     cnt->fmin = 0xFFFFFFFFu;
     cnt->pull_skipped = 0;
+    cnt->pull_skipped_hot = 0;
     cnt->pull_skipped_rows = 0;
+    cnt->pull_done_arcs = 0;
     cnt->frontier_uncovered = 0;
     cnt->ship_error = 0;
   }
@@ -929,6 +931,7 @@ static grd_status traverse_t(grd_state *s, int option)
       s->skip_arc_bytes = 8 + wb;
     }
     GRD_TRY(bsp_pull_fixup<Op>(s));
+    if (timed && s->skip_pending) GRD_TRY(pull_done_arcs(s));  // outside the timed region
     s->edges += g->m;  // every arc of the memory partitions, less those left out (counters)
   } else {
     uint32_t want = (uint32_t)std::min<uint64_t>((s->last_frontier + kBlock - 1) / kBlock, (uint64_t)ctx->num_cus * 8);

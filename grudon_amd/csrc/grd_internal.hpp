@@ -86,8 +86,10 @@ struct Counters {
   unsigned long long ship_error;      // edge shipping: the active list disagreed with the sized wire
   unsigned int fmin;                  // CC / SSSP pull: smallest value broadcast this superstep
   unsigned int pad32;
-  unsigned long long pull_skipped;    // CC / SSSP pull: arcs of rows that could not improve, not read
-  unsigned long long pull_skipped_rows;  // and the rows of the row blocks left out
+  unsigned long long pull_skipped;    // CC / SSSP pull: arcs of the cold row blocks left out (not read)
+  unsigned long long pull_skipped_hot;   // and of the hot wave blocks left out
+  unsigned long long pull_skipped_rows;  // rows of the cold row blocks left out
+  unsigned long long pull_done_arcs;     // cold arcs of done rows, none gathered (measured with time_kernel)
   float sel[8];                       // learned selector: the next superstep's 4 logits and argmax
   unsigned long long fpart[64];       // frontier size of each compute partition (C > 1)
 };

@@ -125,6 +125,8 @@ template <class Op>
 grd_status bsp_pull_launch(grd_state *s, const uint32_t *cw);
 template <class Op>
 grd_status bsp_pull_fixup(grd_state *s);
+// CC / SSSP pull measured with time_kernel: counts the cold arcs of done rows
+grd_status pull_done_arcs(grd_state *s);
 // the hot pass of the hot / cold split for the BSP pull (tiers.hip)
 template <class Op>
 grd_status bsp_pull_hot(grd_state *s);

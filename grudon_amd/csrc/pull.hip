@@ -187,29 +187,33 @@ __global__ void __launch_bounds__(kBlock) k_prp_pull_fixup(const PullLong *__res
 
 // Rows of a CC / SSSP pull that cannot improve (the broadcast's rows-done
 // bitmap, k_rows_done): a row block whose rows with arcs are all done is left
-// out whole.  This pass flags those blocks and counts their arcs and rows
-// (one atomic per workgroup), so that the pull reads one flag per block.
+// out whole (flag 1); one with some done rows (flag 2) gathers only the arcs
+// of its other rows.  This pass sets the flags and counts the arcs and rows of
+// the blocks left out (one atomic per workgroup), so that the pull reads one
+// flag per block.
 __global__ void __launch_bounds__(kBlock) k_blocks_done(uint32_t nblk, const PullBlock *__restrict__ blocks,
                                                         const uint64_t *__restrict__ done,
                                                         const uint64_t *__restrict__ hasarc,
-                                                        uint8_t *__restrict__ bskip, Counters *cnt, int count_rows)
+                                                        uint8_t *__restrict__ bskip, Counters *cnt, int cold)
 {
   unsigned long long arcs = 0, rows = 0;
   for (uint32_t b = blockIdx.x * kBlock + threadIdx.x; b < nblk; b += gridDim.x * kBlock) {
     const PullBlock d = blocks[b];
-    bool all;
+    bool all, any = false;
     if (d.nrows == 0) {
       all = (done[d.r0 >> 6] >> (d.r0 & 63)) & 1ull;
     } else {
       all = true;
       const uint32_t r1 = d.r0 + d.nrows;
-      for (uint32_t w = d.r0 >> 6; all && w <= (r1 - 1) >> 6; w++) {
+      for (uint32_t w = d.r0 >> 6; (all || !any) && w <= (r1 - 1) >> 6; w++) {
         const uint32_t lo = max(d.r0, w * 64) - w * 64, hi = min(r1 - w * 64, 64u);
         const uint64_t mask = (hi - lo == 64) ? ~0ull : (((1ull << (hi - lo)) - 1ull) << lo);
-        all = ((done[w] | ~hasarc[w]) & mask) == mask;
+        const uint64_t dn = done[w], ha = hasarc[w];
+        all = all && ((dn | ~ha) & mask) == mask;
+        any = any || (dn & ha & mask) != 0;
       }
     }
-    bskip[b] = all ? 1 : 0;
+    bskip[b] = all ? 1 : any ? 2 : 0;
     if (all) {
       arcs += d.e1 - d.e0;
       rows += d.nrows;
@@ -218,9 +222,23 @@ __global__ void __launch_bounds__(kBlock) k_blocks_done(uint32_t nblk, const Pul
   arcs = block_sum(arcs);
   rows = block_sum(rows);
   if (threadIdx.x == 0) {
-    if (arcs) atomicAdd(&cnt->pull_skipped, arcs);
-    if (rows && count_rows) atomicAdd(&cnt->pull_skipped_rows, rows);
+    if (arcs) atomicAdd(cold ? &cnt->pull_skipped : &cnt->pull_skipped_hot, arcs);
+    if (rows && cold) atomicAdd(&cnt->pull_skipped_rows, rows);
   }
+}
+
+// Measurement only (time_kernel): the arcs of done rows in the row-block
+// pull's CSC (rp: the cold CSC, or the whole one), which the pull does not
+// gather; a wave covers one word of rows, one atomic per workgroup.
+__global__ void __launch_bounds__(kBlock) k_done_arcs(uint32_t nwP, const uint32_t *__restrict__ rp,
+                                                      const uint64_t *__restrict__ done, Counters *cnt)
+{
+  unsigned long long arcs = 0;
+  const uint64_t nrow = (uint64_t)nwP * 64;
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < nrow; r += (uint64_t)gridDim.x * kBlock)
+    if ((done[r >> 6] >> (r & 63)) & 1ull) arcs += rp[r + 1] - rp[r];
+  arcs = block_sum(arcs);
+  if (threadIdx.x == 0 && arcs) atomicAdd(&cnt->pull_done_arcs, arcs);
 }
 
 // bit r = row r has an arc (rp[r + 1] > rp[r]); a wave covers one word
@@ -273,9 +291,11 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
   __shared__ V s_v[kPullNnz];
   __shared__ V s_hot[kPullHot];
   __shared__ unsigned long long s_dirty[kDirtyWords];
+  __shared__ uint8_t s_live[kPullNnz];  // a partially done block: 1 = the arc's row may improve
   const uint32_t bid = blockIdx.x;
   const PullBlock d = blocks[bid];
-  if (bskip && bskip[bid]) return;  // no row with arcs can improve (k_blocks_done)
+  const uint32_t flag = bskip ? bskip[bid] : 0u;  // k_blocks_done: 1 = left out, 2 = some rows done
+  if (flag == 1) return;
   if (d.nrows == 0) {  // segment (<= kPullNnz arcs) of a long row
     // column ids (and weights), then the gathers (hot sources from LDS),
     // then each thread combines its arcs in order
@@ -306,6 +326,11 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
   }
   for (int i = threadIdx.x; i < kDirtyWords; i += kBlock) s_dirty[i] = 0ull;
   const uint32_t ne = d.e1 - d.e0;
+  const int G = group_size(d.nrows);
+  const int sub = threadIdx.x % G;
+  const uint32_t ngroups = kBlock / G;
+  const uint32_t rounds = (d.nrows + ngroups - 1) / ngroups;
+  const uint32_t w0 = d.r0 >> 6;
   {
     // batched stages: column ids (and weights), then the gathers; the
     // partition's kPullHot highest-degree sources are read from LDS
@@ -318,6 +343,22 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
       src[k] = i < ne ? ccol[d.e0 + i] : 0u;
       wt[k] = (cw && i < ne) ? (uint32_t)cw[d.e0 + i] : 1u;
     }
+    if (flag == 2) {
+      // a partially done block: each row's group marks the row's arcs live or
+      // not (every arc of the block belongs to one of its rows)
+      for (uint32_t k = 0; k < rounds; k++) {
+        const uint32_t rl = k * ngroups + threadIdx.x / G, r = d.r0 + rl;
+        bool live = false;
+        uint32_t b = 0, e = 0;
+        if (rl < d.nrows) {
+          const uint64_t bit = 1ull << (r & 63);
+          live = !(done[r >> 6] & bit) && (hasarc[r >> 6] & bit);
+          b = crp[r] - d.e0;
+          e = crp[r + 1] - d.e0;
+        }
+        for (uint32_t x = b + sub; x < e; x += G) s_live[x] = live ? 1 : 0;
+      }
+    }
     const uint32_t hb = hot_base[bid];
     const uint32_t hn = min(kPullHot, P - hb);
     for (uint32_t i = threadIdx.x; i < hn; i += kBlock) s_hot[i] = mprop[hb + i];
@@ -326,7 +367,8 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
     for (int k = 0; k < kPer; k++) {
       uint32_t i = threadIdx.x + k * kBlock;
       const uint32_t h = src[k] - hb;
-      v[k] = i < ne ? Op::edge(h < hn ? s_hot[h] : mprop[src[k]], wt[k]) : op_identity<Op>();
+      const bool on = i < ne && (flag != 2 || s_live[i]);
+      v[k] = on ? Op::edge(h < hn ? s_hot[h] : mprop[src[k]], wt[k]) : op_identity<Op>();
     }
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
@@ -335,11 +377,6 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
     }
   }
   __syncthreads();
-  const int G = group_size(d.nrows);
-  const int sub = threadIdx.x % G;
-  const uint32_t ngroups = kBlock / G;
-  const uint32_t rounds = (d.nrows + ngroups - 1) / ngroups;
-  const uint32_t w0 = d.r0 >> 6;
   for (uint32_t k = 0; k < rounds; k++) {
     uint32_t rl = k * ngroups + threadIdx.x / G;
     V acc = op_identity<Op>();
@@ -513,6 +550,19 @@ grd_status bsp_pull_fixup(grd_state *s)
                      hc ? g->clong : g->plong, nlong, hc ? (const V *)s->cseg : (const V *)s->partial, (V *)s->mupd,
                      s->Du, hc ? (const V *)s->tpart : (const V *)nullptr,
                      (Op::kMin && s->rdone_sent) ? (const uint64_t *)s->Rdone : nullptr);
+  GRD_HIP(hipGetLastError());
+  return GRD_OK;
+}
+
+grd_status pull_done_arcs(grd_state *s)
+{
+  grd_graph *g = s->g;
+  const uint32_t nwP = g->P / 64;
+  if (nwP)
+    hipLaunchKernelGGL(k_done_arcs, dim3(std::min<uint32_t>((nwP + kWavesPerBlock - 1) / kWavesPerBlock,
+                                                          (uint32_t)g->ctx->num_cus * 8)),
+                       dim3(kBlock), 0, g->ctx->stream, nwP, bsp_hot_cold(s) ? g->corp : g->crp,
+                       (const uint64_t *)s->Rdone, s->cnt);
   GRD_HIP(hipGetLastError());
   return GRD_OK;
 }

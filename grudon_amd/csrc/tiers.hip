@@ -622,9 +622,10 @@ struct HotBsp {
 };
 
 // kSkip (CC / SSSP): skip[b] = 1 marks a wave block whose rows are all done
-// or without arcs (k_blocks_done); it is taken as an empty segment (no loads,
-// the identity written to its hseg slot) and its rows' partials are not
-// written, which the cold pass never reads for such rows.
+// or without arcs (k_blocks_done; 2, some rows done, runs in full); it is
+// taken as an empty segment (no loads, the identity written to its hseg slot)
+// and its rows' partials are not written, which the cold pass never reads for
+// such rows.
 template <class Pol, class W = uint32_t, bool kSkip = false>
 __global__ void __launch_bounds__(kTierBlock) k_pull_hot(const TierTile *__restrict__ ranges,
                                                          const PullBlock *__restrict__ blocks,
@@ -716,7 +717,7 @@ __global__ void __launch_bounds__(kTierBlock) k_pull_hot(const TierTile *__restr
   const auto bflag = block_buffer(skip, kSkip ? bend : 0u);
   auto load_flag = [&](uint32_t i) { return kSkip ? (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(bflag, i, 0, 0) : 0u; };
   auto apply_flag = [](PullBlock x, uint32_t f) {
-    if (kSkip && __builtin_amdgcn_readfirstlane(f)) x = PullBlock{x.r0, 0, x.e0, x.e0};
+    if (kSkip && __builtin_amdgcn_readfirstlane(f) == 1u) x = PullBlock{x.r0, 0, x.e0, x.e0};
     return x;
   };
   uint32_t b = T.b0 + w;
