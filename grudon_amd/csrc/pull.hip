@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(kBlock) k_blocks_done(uint32_t nblk, const Pul
 
 // Measurement only (time_kernel): the arcs of done rows in the row-block
 // pull's CSC (rp: the cold CSC, or the whole one), which the pull does not
-// gather; a wave covers one word of rows, one atomic per workgroup.
+// gather; a thread per row, one atomic per workgroup.
 __global__ void __launch_bounds__(kBlock) k_done_arcs(uint32_t nwP, const uint32_t *__restrict__ rp,
                                                       const uint64_t *__restrict__ done, Counters *cnt)
 {
