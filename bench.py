@@ -325,6 +325,19 @@ def schedule_speedup(api, ctx, scale, parts, ef=16, weighted=False, reps=3, sym_
 # workloads
 # ---------------------------------------------------------------------------
 
+def gather_ceiling(api, ctx, n, m, parts, kms, kl):
+    """The pull's arc rate (arcs per second of one pull step: hot arcs read
+    from LDS, cold ones gathered through the L2) beside the device's
+    uniform-random gather ceiling for a table of one memory partition's own
+    vertices and for an L2-resident table (4 MB)."""
+    ceil = {"pull_gathers_per_s": round(m / (kms / kl * 1e-3), 1)}
+    for name, entries in (("partition_table", -(-n // parts)), ("l2_table", 1 << 20)):
+        _, rate = api.probe_gather(ctx.device, entries)
+        ceil[name] = {"entries": entries, "uniform_random_gathers_per_s": round(rate, 1)}
+    log("gather ceiling: %s" % json.dumps(ceil))
+    return ceil
+
+
 def run_pagerank(api, ctx, args):
     import numpy as np
 
@@ -362,15 +375,7 @@ def run_pagerank(api, ctx, args):
     S.free()
     G.free()
     if pull and kl:
-        # the pull's gather rate beside the device's random-gather ceiling for
-        # a table of one memory partition's own vertices and for an
-        # L2-resident table (4 MB)
-        ceil = {"pull_gathers_per_s": round(m / (kms / kl * 1e-3), 1)}
-        for name, entries in (("partition_table", -(-n // args.parts)), ("l2_table", 1 << 20)):
-            _, rate = api.probe_gather(ctx.device, entries)
-            ceil[name] = {"entries": entries, "uniform_random_gathers_per_s": round(rate, 1)}
-        extra["gather_ceiling"] = ceil
-        log("gather ceiling: %s" % json.dumps(ceil))
+        extra["gather_ceiling"] = gather_ceiling(api, ctx, n, m, args.parts, kms, kl)
     if pull and args.speedup_scale:
         extra["learned_vs_plain"] = schedule_speedup(api, ctx, args.speedup_scale, args.parts)
     cpu = None
@@ -603,6 +608,8 @@ def run_headline(api, ctx, args):
     roof = roofline_of(kms, kl, kb, PULL_KERNELS[args.pull],
                        traffic_from_profiles("k_prp_pull_%s_rmat%d" % (args.pull, scale)))
     extra = {"work_gteps": round(last["timed_edges_examined"] / secs / 1e9, 3)}
+    if kl:
+        extra["gather_ceiling"] = gather_ceiling(api, ctx, G.n, G.m, args.parts, kms, kl)
     plan = None
     if not args.no_check or not args.no_cpu:
         t2 = time.perf_counter()
