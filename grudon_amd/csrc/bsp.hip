@@ -878,8 +878,10 @@ grd_status op_broadcast(grd_state *s, int option)
   if (need_rank) GRD_TRY(word_ranks(st, s->F, nwS, s->blk, s->wrankB, s->totals + 0));
   const uint32_t grid = group_grid(nwS);
   // a pull of CC / SSSP also sends the rows that cannot improve (the pull
-  // leaves their arcs out unless the caller asked for every row)
-  const bool rows_done = pull && s->algo != GRD_PR && !s->prm.pull_every_row;
+  // leaves their arcs out unless the caller asked for every row); not in the
+  // first superstep, where every vertex still holds its initial value and
+  // at most the frontier's smallest is done
+  const bool rows_done = pull && s->algo != GRD_PR && !s->prm.pull_every_row && s->iteration > 0;
   hipLaunchKernelGGL(k_bcast_send<uint32_t>, dim3(rows_done ? loop_grid(g->ctx, nwS) : grid), dim3(kBlock), 0, st,
                      nwS, s->F, (const uint32_t *)s->prop, s->wrankB, (int)dense, s->bvB, (uint32_t *)s->valB,
                      rows_done ? &s->cnt->fmin : nullptr);
