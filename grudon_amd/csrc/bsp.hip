@@ -877,29 +877,32 @@ grd_status op_broadcast(grd_state *s, int option)
   s->xr_dense = -1;
   if (need_rank) GRD_TRY(word_ranks(st, s->F, nwS, s->blk, s->wrankB, s->totals + 0));
   const uint32_t grid = group_grid(nwS);
-  // a pull of CC / SSSP also sends the rows that cannot improve (the pull
-  // leaves their arcs out unless the caller asked for every row); not in the
-  // first superstep, where every vertex still holds its initial value and
-  // at most the frontier's smallest is done
-  const bool rows_done = pull && s->algo != GRD_PR && !s->prm.pull_every_row && s->iteration > 0;
-  hipLaunchKernelGGL(k_bcast_send<uint32_t>, dim3(rows_done ? loop_grid(g->ctx, nwS) : grid), dim3(kBlock), 0, st,
+  // a pull of CC / SSSP also sends the smallest value broadcast and the rows
+  // that cannot improve (the pull leaves out what cannot change unless the
+  // caller asked for every row); the rows-done bitmap not in the first
+  // superstep, where every vertex still holds its initial value and at most
+  // the frontier's smallest is done
+  const bool bound = pull && s->algo != GRD_PR && !s->prm.pull_every_row;
+  const bool rows_done = bound && s->iteration > 0;
+  hipLaunchKernelGGL(k_bcast_send<uint32_t>, dim3(bound ? loop_grid(g->ctx, nwS) : grid), dim3(kBlock), 0, st,
                      nwS, s->F, (const uint32_t *)s->prop, s->wrankB, (int)dense, s->bvB, (uint32_t *)s->valB,
-                     rows_done ? &s->cnt->fmin : nullptr);
+                     bound ? &s->cnt->fmin : nullptr);
   const uint32_t ident = (s->algo == GRD_PR) ? 0u : 0xFFFFFFFFu;  // 0.0f or the min identity
   hipLaunchKernelGGL(k_bcast_recv<uint32_t>, dim3(grid), dim3(kBlock), 0, st, nwS, s->bvB, (const uint32_t *)s->valB,
                      s->wrankB, g->word_dst, (int)dense, (int)!pull, ident, (uint32_t *)s->mprop, s->AL);
   const uint32_t nwP = g->P / 64;
-  if (rows_done && nwP) {
-    uint32_t inc = 0;  // the smallest Op::edge increment over an arc
-    if (s->algo == GRD_SSSP) {
-      if (!s->prm.weighted) inc = 1;
-      else GRD_TRY(graph_min_weight(g, &inc));
-    }
+  uint32_t inc = 0;  // the smallest Op::edge increment over an arc
+  if (bound && s->algo == GRD_SSSP) {
+    if (!s->prm.weighted) inc = 1;
+    else GRD_TRY(graph_min_weight(g, &inc));
+  }
+  if (rows_done && nwP)
     hipLaunchKernelGGL(k_rows_done, dim3(loop_grid(g->ctx, nwP)), dim3(kBlock), 0, st, nwP, g->proxy_slot,
                        (const uint32_t *)s->prop, s->cnt, inc, s->Rdone);
-  }
   GRD_HIP(hipGetLastError());
   s->rdone_sent = rows_done;
+  s->bound_sent = bound;
+  s->bound_inc = inc;
   // exchange bytes (GraphAlgorithm.cpp:251-265): bitvector of every pair +
   // packed values (sparse) or the pair's full own slice (dense), and the
   // rows-done bitmap of a CC / SSSP pull

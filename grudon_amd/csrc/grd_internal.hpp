@@ -89,7 +89,7 @@ struct Counters {
   unsigned long long pull_skipped;    // CC / SSSP pull: arcs of the cold row blocks left out (not read)
   unsigned long long pull_skipped_hot;   // and of the hot wave blocks left out
   unsigned long long pull_skipped_rows;  // rows of the cold row blocks left out
-  unsigned long long pull_done_arcs;     // cold arcs of done rows, none gathered (measured with time_kernel)
+  unsigned long long pull_done_arcs;     // cold arcs of rows that gathered none (measured with time_kernel)
   float sel[8];                       // learned selector: the next superstep's 4 logits and argmax
   unsigned long long fpart[64];       // frontier size of each compute partition (C > 1)
 };
@@ -268,6 +268,8 @@ struct grd_state {
   // superstep's smallest broadcast value (sent with a pull broadcast)
   uint64_t *Rdone = nullptr;
   bool rdone_sent = false;   // the last broadcast sent Rdone (a pull of CC / SSSP)
+  bool bound_sent = false;   // it sent the smallest broadcast value (counters: fmin)
+  uint32_t bound_inc = 0;    // the smallest Op::edge increment over an arc
   bool skip_pending = false; // the last pull's skipped arcs are still to be taken from the counters
   uint64_t skip_arc_bytes = 0;  // algorithmic bytes per arc of that pull
   uint8_t *bskip = nullptr;     // CC / SSSP pull: per row block, 1 = every row with arcs is done

@@ -227,16 +227,29 @@ __global__ void __launch_bounds__(kBlock) k_blocks_done(uint32_t nblk, const Pul
   }
 }
 
-// Measurement only (time_kernel): the arcs of done rows in the row-block
-// pull's CSC (rp: the cold CSC, or the whole one), which the pull does not
-// gather; a thread per row, one atomic per workgroup.
-__global__ void __launch_bounds__(kBlock) k_done_arcs(uint32_t nwP, const uint32_t *__restrict__ rp,
-                                                      const uint64_t *__restrict__ done, Counters *cnt)
+// the smallest candidate a pull superstep can produce: the smallest value
+// broadcast plus the smallest Op::edge increment, saturating
+__device__ __forceinline__ uint32_t pull_bound(const Counters *cnt, uint32_t inc)
 {
+  const uint32_t f = cnt->fmin;
+  return f + inc < f ? 0xFFFFFFFFu : f + inc;
+}
+
+// Measurement only (time_kernel): the arcs, in the row-block pull's CSC (rp:
+// the cold CSC, or the whole one), of the rows that gathered none: done rows
+// (done may be null) and rows whose hot partial already reached the bound
+// (pre may be null); a thread per row, one atomic per workgroup.
+__global__ void __launch_bounds__(kBlock) k_done_arcs(uint32_t nwP, const uint32_t *__restrict__ rp,
+                                                      const uint64_t *__restrict__ done,
+                                                      const uint32_t *__restrict__ pre, uint32_t inc, Counters *cnt)
+{
+  const uint32_t bnd = pre ? pull_bound(cnt, inc) : 0u;
   unsigned long long arcs = 0;
   const uint64_t nrow = (uint64_t)nwP * 64;
-  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < nrow; r += (uint64_t)gridDim.x * kBlock)
-    if ((done[r >> 6] >> (r & 63)) & 1ull) arcs += rp[r + 1] - rp[r];
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < nrow; r += (uint64_t)gridDim.x * kBlock) {
+    const bool none = (done && ((done[r >> 6] >> (r & 63)) & 1ull)) || (pre && pre[r] <= bnd);
+    if (none) arcs += rp[r + 1] - rp[r];
+  }
   arcs = block_sum(arcs);
   if (threadIdx.x == 0 && arcs) atomicAdd(&cnt->pull_done_arcs, arcs);
 }
@@ -285,7 +298,8 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
                                                         const typename Op::V *__restrict__ pre,
                                                         const uint8_t *__restrict__ bskip,
                                                         const uint64_t *__restrict__ done,
-                                                        const uint64_t *__restrict__ hasarc)
+                                                        const uint64_t *__restrict__ hasarc,
+                                                        const Counters *__restrict__ bcnt, uint32_t inc)
 {
   using V = typename Op::V;
   __shared__ V s_v[kPullNnz];
@@ -296,6 +310,11 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
   const PullBlock d = blocks[bid];
   const uint32_t flag = bskip ? bskip[bid] : 0u;  // k_blocks_done: 1 = left out, 2 = some rows done
   if (flag == 1) return;
+  // with bcnt (a CC / SSSP pull over the hot / cold split): a row whose hot
+  // partial is already at or below the bound takes no cold candidate below it
+  // (every cold candidate is at least the bound), so it gathers nothing and
+  // its result is the hot partial
+  const uint32_t bnd = bcnt ? pull_bound(bcnt, inc) : 0u;
   if (d.nrows == 0) {  // segment (<= kPullNnz arcs) of a long row
     // column ids (and weights), then the gathers (hot sources from LDS),
     // then each thread combines its arcs in order
@@ -307,6 +326,10 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
       const uint32_t i = threadIdx.x + k * kBlock;
       src[k] = i < ne ? ccol[d.e0 + i] : 0u;
       wt[k] = (cw && i < ne) ? (uint32_t)cw[d.e0 + i] : 1u;
+    }
+    if (bcnt && pre[d.r0] <= bnd) {  // uniform over the block
+      if (threadIdx.x == 0) partial[bid] = op_identity<Op>();
+      return;
     }
     const uint32_t hb = hot_base[bid];
     const uint32_t hn = min(kPullHot, P - hb);
@@ -343,20 +366,23 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
       src[k] = i < ne ? ccol[d.e0 + i] : 0u;
       wt[k] = (cw && i < ne) ? (uint32_t)cw[d.e0 + i] : 1u;
     }
-    if (flag == 2) {
-      // a partially done block: each row's group marks the row's arcs live or
-      // not (every arc of the block belongs to one of its rows)
+    const bool marked = flag == 2 || bcnt;  // uniform over the block
+    if (marked) {
+      // each row's group marks the row's arcs to gather or not (every arc of
+      // the block belongs to one of its rows): not for a done row, nor for
+      // one whose hot partial already reached the bound
       for (uint32_t k = 0; k < rounds; k++) {
         const uint32_t rl = k * ngroups + threadIdx.x / G, r = d.r0 + rl;
-        bool live = false;
+        bool gat = false;
         uint32_t b = 0, e = 0;
         if (rl < d.nrows) {
           const uint64_t bit = 1ull << (r & 63);
-          live = !(done[r >> 6] & bit) && (hasarc[r >> 6] & bit);
+          gat = !done || (!(done[r >> 6] & bit) && (hasarc[r >> 6] & bit));
+          if (gat && bcnt) gat = pre[r] > bnd;
           b = crp[r] - d.e0;
           e = crp[r + 1] - d.e0;
         }
-        for (uint32_t x = b + sub; x < e; x += G) s_live[x] = live ? 1 : 0;
+        for (uint32_t x = b + sub; x < e; x += G) s_live[x] = gat ? 1 : 0;
       }
     }
     const uint32_t hb = hot_base[bid];
@@ -367,7 +393,7 @@ __global__ void __launch_bounds__(kBlock) k_bsp_pull_rb(const PullBlock *__restr
     for (int k = 0; k < kPer; k++) {
       uint32_t i = threadIdx.x + k * kBlock;
       const uint32_t h = src[k] - hb;
-      const bool on = i < ne && (flag != 2 || s_live[i]);
+      const bool on = i < ne && (!marked || s_live[i]);
       v[k] = on ? Op::edge(h < hn ? s_hot[h] : mprop[src[k]], wt[k]) : op_identity<Op>();
     }
 #pragma unroll
@@ -476,7 +502,9 @@ static grd_status blocks_done(grd_state *s, bool cold, const uint8_t **flags)
 {
   *flags = nullptr;
   s->hskip_on = false;
-  if (!Op::kMin || !s->rdone_sent) return GRD_OK;
+  if (!Op::kMin || !s->bound_sent) return GRD_OK;
+  s->skip_pending = true;  // the rows that gathered nothing come off with the counters
+  if (!s->rdone_sent) return GRD_OK;
   grd_graph *g = s->g;
   hipStream_t st = g->ctx->stream;
   const uint32_t nwP = g->P / 64;
@@ -516,23 +544,27 @@ grd_status bsp_pull_launch(grd_state *s, const uint32_t *cw)
   const uint8_t *skip = nullptr;
   if (bsp_hot_cold(s)) {
     GRD_TRY(blocks_done<Op>(s, true, &skip));
+    // the hot partials against the bound (CC / SSSP)
+    const Counters *bcnt = (Op::kMin && s->bound_sent) ? s->cnt : nullptr;
     GRD_TRY(bsp_pull_hot<Op>(s));
     if (g->n_cblk && cw && g->cow8)
       hipLaunchKernelGGL((k_bsp_pull_rb<Op, uint8_t>), dim3(g->n_cblk), dim3(kBlock), 0, g->ctx->stream, g->cblk,
                          g->cblk_hot, g->corp, g->cocol, (const uint8_t *)g->cow8, (const V *)s->mprop, g->P,
                          (V *)s->mupd, s->Du, (V *)s->cseg, (const V *)s->tpart, skip,
-                         skip ? (const uint64_t *)s->Rdone : nullptr, skip ? (const uint64_t *)g->rarc : nullptr);
+                         skip ? (const uint64_t *)s->Rdone : nullptr, skip ? (const uint64_t *)g->rarc : nullptr,
+                         bcnt, s->bound_inc);
     else if (g->n_cblk)
       hipLaunchKernelGGL((k_bsp_pull_rb<Op, uint32_t>), dim3(g->n_cblk), dim3(kBlock), 0, g->ctx->stream, g->cblk,
                          g->cblk_hot, g->corp, g->cocol, cw ? g->cow : nullptr, (const V *)s->mprop, g->P,
                          (V *)s->mupd, s->Du, (V *)s->cseg, (const V *)s->tpart, skip,
-                         skip ? (const uint64_t *)s->Rdone : nullptr, skip ? (const uint64_t *)g->rarc : nullptr);
+                         skip ? (const uint64_t *)s->Rdone : nullptr, skip ? (const uint64_t *)g->rarc : nullptr,
+                         bcnt, s->bound_inc);
   } else {
     GRD_TRY(blocks_done<Op>(s, false, &skip));
     hipLaunchKernelGGL((k_bsp_pull_rb<Op, uint32_t>), dim3(g->n_pblk), dim3(kBlock), 0, g->ctx->stream, g->pblk,
                        g->pblk_hot, g->crp, g->ccol, cw, (const V *)s->mprop, g->P, (V *)s->mupd, s->Du,
                        (V *)s->partial, (const V *)nullptr, skip, skip ? (const uint64_t *)s->Rdone : nullptr,
-                       skip ? (const uint64_t *)g->rarc : nullptr);
+                       skip ? (const uint64_t *)g->rarc : nullptr, (const Counters *)nullptr, 0u);
   }
   GRD_HIP(hipGetLastError());
   return GRD_OK;
@@ -562,7 +594,8 @@ grd_status pull_done_arcs(grd_state *s)
     hipLaunchKernelGGL(k_done_arcs, dim3(std::min<uint32_t>((nwP + kWavesPerBlock - 1) / kWavesPerBlock,
                                                           (uint32_t)g->ctx->num_cus * 8)),
                        dim3(kBlock), 0, g->ctx->stream, nwP, bsp_hot_cold(s) ? g->corp : g->crp,
-                       (const uint64_t *)s->Rdone, s->cnt);
+                       s->rdone_sent ? (const uint64_t *)s->Rdone : nullptr,
+                       bsp_hot_cold(s) ? (const uint32_t *)s->tpart : nullptr, s->bound_inc, s->cnt);
   GRD_HIP(hipGetLastError());
   return GRD_OK;
 }
