@@ -213,11 +213,15 @@ def layout_desc(args, scale, sym):
                                                          args.compute, args.parts))
 
 
-def timed_solves(S, args, init_kw, sources=None):
+def timed_solves(S, args, init_kw, sources=None, exact_bytes=False):
     """Warm-up, then exactly args.steps timed steps; a step is one solve (or
     one solve per source).  Returns (seconds, iterations, kernel stats, last
     run stats); the last run's stats also carry "timed_edges_examined", the
-    edges the traversals examined over all timed steps (for work GTEPS)."""
+    edges the traversals examined over all timed steps (for work GTEPS).
+    With exact_bytes (CC / SSSP, whose pulls leave out gathers), the timed
+    steps time the kernels with events only (time_kernel 2) and one more,
+    untimed step counts their algorithmic bytes exactly (time_kernel 1); the
+    solves are deterministic, so its bytes per launch are the timed ones'."""
     srcs = sources if sources is not None else [None]
 
     def step(time_kernel):
@@ -241,14 +245,19 @@ def timed_solves(S, args, init_kw, sources=None):
     iters, kms, kl, kb, ee = 0, 0.0, 0, 0, 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        acc, last = step(True)
+        acc, last = step(2 if exact_bytes else 1)
         iters += acc[0]
         kms += acc[1]
         kl += acc[2]
         kb += acc[3]
         ee += acc[4]
+    secs = time.perf_counter() - t0
+    if exact_bytes and kl:
+        acc, _ = step(1)
+        if acc[2]:
+            kb = int(round(acc[3] / acc[2] * kl))
     last["timed_edges_examined"] = ee
-    return time.perf_counter() - t0, iters, (kms, kl, kb), last
+    return secs, iters, (kms, kl, kb), last
 
 
 def schedule_speedup(api, ctx, scale, parts, ef=16, weighted=False, reps=3, sym_cc=True, graph=None):
@@ -408,7 +417,8 @@ def run_cc(api, ctx, args):
     S = api.State(G, api.CC)
     secs, iters, (kms, kl, kb), last = timed_solves(S, args, {"sched": api.SCHED_HEURISTIC,
                                                               "offload_only": args.offload_only,
-                                                              "pull_every_row": args.pull_every_row})
+                                                              "pull_every_row": args.pull_every_row},
+                                                    exact_bytes=True)
     gteps = G.m * args.steps / secs / 1e9
     log("timed %d steps: %.3f s, %.1f GTEPS" % (args.steps, secs, gteps))
     # edges the traversals examined (a pull examines every arc) per second
@@ -447,7 +457,7 @@ def run_sssp(api, ctx, args):
     S = api.State(G, api.SSSP)
     kw = {"weighted": True, "delta": args.delta, "sched": api.SCHED_HEURISTIC, "offload_only": args.offload_only,
           "pull_every_row": args.pull_every_row}
-    secs, iters, (kms, kl, kb), last = timed_solves(S, args, kw, sources=sources)
+    secs, iters, (kms, kl, kb), last = timed_solves(S, args, kw, sources=sources, exact_bytes=True)
     # harmonic mean over sources of arcs / t_s = 16 arcs / sum t_s
     gteps = G.m * len(sources) * args.steps / secs / 1e9
     log("timed %d steps x %d sources: %.3f s, %.1f GTEPS" % (args.steps, len(sources), secs, gteps))
@@ -568,12 +578,12 @@ def run_batch(api, ctx, args):
     return line(args, gteps, "GTEPS", secs * 1e3 / args.steps, "f32", config, None, cpu, extra)
 
 
-def _timed_leg(api, S, args, init_kw, arcs, sources=None, per_iteration=False):
+def _timed_leg(api, S, args, init_kw, arcs, sources=None, per_iteration=False, exact_bytes=False):
     """Timed solves of one leg: (GTEPS, ms per step, iterations per step,
     kernel stats of its dominant traversal kernel, last run stats, seconds).
     GTEPS = arcs x iterations / time with per_iteration (PageRank), else
     arcs x solves / time (one solve per source)."""
-    secs, iters, (kms, kl, kb), last = timed_solves(S, args, init_kw, sources=sources)
+    secs, iters, (kms, kl, kb), last = timed_solves(S, args, init_kw, sources=sources, exact_bytes=exact_bytes)
     nsrc = 1 if sources is None else len(sources)
     work = arcs * iters if per_iteration else arcs * nsrc * args.steps
     return work / secs / 1e9, secs * 1e3 / args.steps, iters // max(1, args.steps), (kms, kl, kb), last, secs
@@ -642,7 +652,8 @@ def run_headline(api, ctx, args):
     Gs = api.Graph.from_csr(ctx, rps, cols, ws, C=args.compute, M=args.parts)
     log("symmetrized graph: m=%d, gen + partition %.1fs" % (Gs.m, time.perf_counter() - t0))
     Sc = api.State(Gs, api.CC)
-    g_cc, ms_cc, it_cc, kcc, last_cc, secs_cc = _timed_leg(api, Sc, args, {"sched": api.SCHED_HEURISTIC}, Gs.m)
+    g_cc, ms_cc, it_cc, kcc, last_cc, secs_cc = _timed_leg(api, Sc, args, {"sched": api.SCHED_HEURISTIC}, Gs.m,
+                                                           exact_bytes=True)
     leg = {"gteps": round(g_cc, 3), "ms_per_step": round(ms_cc, 3), "arcs": Gs.m, "iterations": it_cc,
            "work_gteps": round(last_cc["timed_edges_examined"] / secs_cc / 1e9, 3),
            "roofline": roofline_of(*kcc, "k_bsp_traverse", None),
@@ -658,7 +669,8 @@ def run_headline(api, ctx, args):
     sources = rng.choice(np.flatnonzero(deg > 0), size=16, replace=False)
     Ss = api.State(Gs, api.SSSP)
     kw = {"weighted": True, "delta": args.delta, "sched": api.SCHED_HEURISTIC, "max_iter": 100000}
-    g_ss, ms_ss, it_ss, kss, last_ss, secs_ss = _timed_leg(api, Ss, args, kw, Gs.m, sources=sources)
+    g_ss, ms_ss, it_ss, kss, last_ss, secs_ss = _timed_leg(api, Ss, args, kw, Gs.m, sources=sources,
+                                                           exact_bytes=True)
     leg = {"gteps": round(g_ss, 3), "ms_per_step": round(ms_ss, 3), "arcs": Gs.m, "iterations_per_step": it_ss,
            "work_gteps": round(last_ss["timed_edges_examined"] / secs_ss / 1e9, 3),
            "roofline": roofline_of(*kss, "k_bsp_traverse", None),

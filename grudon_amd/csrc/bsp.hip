@@ -936,7 +936,8 @@ static grd_status traverse_t(grd_state *s, int option)
       s->skip_arc_bytes = 8 + wb;
     }
     GRD_TRY(bsp_pull_fixup<Op>(s));
-    if (timed && s->skip_pending) GRD_TRY(pull_done_arcs(s));  // outside the timed region
+    // exact gather count (outside the events; time_kernel 2 leaves it out)
+    if (timed && s->skip_pending && s->prm.time_kernel != 2) GRD_TRY(pull_done_arcs(s));
     s->edges += g->m;  // every arc of the memory partitions, less those left out (counters)
   } else {
     uint32_t want = (uint32_t)std::min<uint64_t>((s->last_frontier + kBlock - 1) / kBlock, (uint64_t)ctx->num_cus * 8);

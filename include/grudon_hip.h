@@ -85,7 +85,11 @@ typedef struct {
   uint32_t sched;        /* grd_sched */
   uint32_t fixed_option; /* option for GRD_SCHED_FIXED */
   uint32_t time_iters;   /* record per-iteration device time in the trace */
-  uint32_t time_kernel;  /* bracket the dominant traversal kernel with HIP events */
+  uint32_t time_kernel;  /* bracket the dominant traversal kernel with HIP events;
+                            2 = the events only: a CC / SSSP pull then counts
+                            the gathers of rows that gathered none as made
+                            (1 counts them exactly with a pass of its own,
+                            outside the events but inside the solve's time) */
   uint32_t offload_only; /* heuristic / learned: never turn a push superstep into
                             edge shipping (by default they ship whenever the
                             frontier is covered, the cheaper push on one GPU) */
