@@ -242,6 +242,37 @@ def test_pull_leaves_out_done_rows(ctx, bsp_graphs, mode, tb, C, M, algo):
         ctx.set_pull_tiling(3, 0, 0)
 
 
+@pytest.mark.parametrize("algo", ["cc", "weighted"])
+def test_events_only_timing(ctx, bsp_graphs, algo):
+    """time_kernel 2 times the same launches as time_kernel 1 without the
+    exact count of the gathers that rows left out: the same results,
+    iterations and launches, and kernel bytes that count those gathers as
+    made (never fewer than the exact count; more for CC, whose pulls always
+    leave some out)."""
+    _say("events-only timing", algo)
+    gs, _, gw = bsp_graphs
+    g = gs if algo == "cc" else gw
+    kw = {} if algo == "cc" else {"source": 0, "weighted": True}
+    ctx.set_pull_tiling(3, 6, 0)
+    try:
+        G = api.Graph.from_csr(ctx, g.rp, g.col, g.w if algo == "weighted" else None, C=1, M=4)
+        runs = {}
+        for tk in (1, 2):
+            S = api.State(G, api.CC if algo == "cc" else api.SSSP)
+            S.init(max_iter=100000, sched=api.SCHED_FIXED, fixed_option=api.OPT_PULL_DENSE, time_kernel=tk, **kw)
+            runs[tk] = (S.run(), S.result())
+            S.free()
+        (s1, r1), (s2, r2) = runs[1], runs[2]
+        assert np.array_equal(r1, r2)
+        assert s1["iterations"] == s2["iterations"] and s1["kernel_launches"] == s2["kernel_launches"]
+        assert s2["kernel_bytes"] >= s1["kernel_bytes"]
+        if algo == "cc":
+            assert s2["kernel_bytes"] > s1["kernel_bytes"]
+        G.free()
+    finally:
+        ctx.set_pull_tiling(3, 0, 0)
+
+
 def test_bsp_pull_on_graph_tiled_by_another_mode(ctx, bsp_graphs):
     """A graph regrouped by the LDS-tiled pull first keeps that regrouping;
     a CC state created on it later pulls over the plain CSC, and is exact."""
