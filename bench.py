@@ -421,7 +421,8 @@ def run_cc(api, ctx, args):
                                                     exact_bytes=True)
     gteps = G.m * args.steps / secs / 1e9
     log("timed %d steps: %.3f s, %.1f GTEPS" % (args.steps, secs, gteps))
-    # edges the traversals examined (a pull examines every arc) per second
+    # edges the traversals examined per second (a CC / SSSP pull leaves out
+    # the row blocks whose rows cannot improve)
     extra = {"work_gteps": round(last["timed_edges_examined"] / secs / 1e9, 3)}
     if not args.no_check:
         import oracle_check
@@ -461,7 +462,8 @@ def run_sssp(api, ctx, args):
     # harmonic mean over sources of arcs / t_s = 16 arcs / sum t_s
     gteps = G.m * len(sources) * args.steps / secs / 1e9
     log("timed %d steps x %d sources: %.3f s, %.1f GTEPS" % (args.steps, len(sources), secs, gteps))
-    # edges the traversals examined (a pull examines every arc) per second
+    # edges the traversals examined per second (a CC / SSSP pull leaves out
+    # the row blocks whose rows cannot improve)
     extra = {"work_gteps": round(last["timed_edges_examined"] / secs / 1e9, 3)}
     if not args.no_check:
         import oracle_check
