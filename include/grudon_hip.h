@@ -127,8 +127,9 @@ typedef struct {
   /* time_kernel: the dominant traversal kernel (pull: k_pull / k_prp_pull,
    * push: k_push, edge shipping: k_ship_send + k_ship_recv), summed over
    * launches, with its algorithmic HBM bytes: pull 8 B per edge + 8 B per
-   * row, push 12 B per edge + 16 B per vertex, shipping 24 B per edge
-   * (36 weighted) + 36 B per record */
+   * row (a CC / SSSP pull: of the arcs and rows it reads, the weight as
+   * stored, the rows-done bitmap), push 12 B per edge + 16 B per vertex,
+   * shipping 24 B per edge (36 weighted) + 36 B per record */
   double kernel_ms;
   uint64_t kernel_launches;
   uint64_t kernel_bytes;
